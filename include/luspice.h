@@ -1,0 +1,158 @@
+/* luspice.h -- C ABI of the MI355X (gfx950) mixed-precision sparse-LU path.
+ *
+ * This header adds the batched / resident entry points that the reference
+ * has no counterpart for (it factors one matrix per launch and keeps nothing
+ * on the device between Newton steps).  The reference-compatible headers sit
+ * beside it: lu_gpu.h, lu_gpu_par.h, spmv.h, selector.h, level.h,
+ * cpu_ref_lu.h, lu_static.h.
+ *
+ * Conventions: 0-based COO patterns; return 0 on success, negative LSP_E*
+ * codes on failure (lsp_strerror gives the text).  A "batch" holds one or
+ * more circuits whose symbolic analysis (COLAMD, pivot sequence, L/U pattern,
+ * level sets) is done once on the host at creation; every later call reuses
+ * it on the device.  Calls on one batch are not re-entrant; distinct batches
+ * may be used from distinct threads with distinct streams.
+ */
+#ifndef LUSPICE_H
+#define LUSPICE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    LSP_OK = 0,
+    LSP_E_ARG = -1,
+    LSP_E_ALLOC = -2,
+    LSP_E_HIP = -3,
+    LSP_E_SINGULAR = -4,      /* structurally or numerically singular */
+    LSP_E_STRUCTURE = -5,     /* inconsistent symbolic structure */
+    LSP_E_DEVICE = -6,        /* a device wait gave up (spin limit) */
+    LSP_E_CAPACITY = -7,      /* a column exceeded the scratch capacity */
+    LSP_E_NOT_CONVERGED = -8  /* IR did not reach the tolerance */
+};
+
+enum { LSP_PREC_FP32 = 0, LSP_PREC_FP64 = 1, LSP_PREC_FP16 = 2 };
+
+typedef struct {
+    int n;
+    long nnz;
+    const int *row;        /* 0-based, length nnz */
+    const int *col;        /* 0-based, length nnz */
+    const double *val;     /* values used for the symbolic pivot sequence */
+    const int *q;          /* column order (NULL: COLAMD) */
+} lsp_circuit_in;
+
+typedef struct {
+    double pivot_tol;      /* CPU partial-pivot threshold (splu tol), 1.0 */
+    double refactor_ptol;  /* GPU pivot acceptance |piv| >= ptol*max, 1e-3 */
+    int ir_outer;          /* restarts (default 7)  */
+    int ir_inner;          /* restart length m (default 50) */
+    double ir_inner_tol;   /* |g_{j+1}|/||b|| stop (default 1e-10) */
+    double ir_outer_tol;   /* true residual stop (default 1e-10) */
+    int lds_cap;           /* LDS doubles per wave in the refactor (1024) */
+    int blocks_per_cu;     /* persistent grid density (0 = occupancy) */
+    int chunk_rows;        /* rows per reduction chunk (0 = auto) */
+    int sell_sigma;        /* SELL sort window (0 = whole matrix) */
+    int fallback_fp64;     /* 1: stalled IR refactors in fp64 and retries */
+} lsp_options;
+
+typedef struct {
+    int status;            /* LSP_OK or error */
+    int bad_pivots;        /* columns failing refactor_ptol (all circuits) */
+    int singular_circuits; /* circuits with a zero pivot */
+    int first_bad_circuit; /* -1 if none */
+    double ms;             /* device time of the refactor launch(es) */
+} lsp_refactor_report;
+
+typedef struct {
+    int status;
+    int outer;             /* restarts used (max over circuits) */
+    int inner_total;       /* inner iterations summed over restarts (max) */
+    int converged;         /* circuits that met ir_outer_tol */
+    int fallback;          /* circuits that fell back to fp64 factors */
+    double max_rel_res;    /* max ||b - A x|| / ||b|| over circuits */
+} lsp_solve_report;
+
+typedef struct {
+    int n;
+    long nnz;
+    long lnnz, unnz;       /* including the diagonals, as splu counts them */
+    long n_updates;        /* scatter count of the left-looking refactor */
+    int nlevels, nlv_fwd, nlv_bwd;
+    int max_col;
+    long sell_slots;
+} lsp_circuit_info;
+
+typedef struct {
+    double refactor_ms;    /* summed device time of refactor launches */
+    long refactor_count;
+    double fwd_ms, bwd_ms; /* summed device time of forward / backward solves */
+    long tri_count;        /* trisolve pairs timed */
+} lsp_kernel_stats;
+
+typedef struct lsp_batch lsp_batch;
+
+const char *lsp_strerror(int code);
+int  lsp_options_default(lsp_options *opt);
+
+int  lsp_batch_create(const lsp_circuit_in *circuits, int ncircuits,
+                      const lsp_options *opt, lsp_batch **out);
+void lsp_batch_destroy(lsp_batch *b);
+int  lsp_batch_set_stream(lsp_batch *b, void *hip_stream);
+int  lsp_batch_info(const lsp_batch *b, int circuit, lsp_circuit_info *info);
+int  lsp_batch_sizes(const lsp_batch *b, long *total_n, long *total_nnz);
+
+/* Values for every circuit, concatenated in circuit order, each in its own
+ * COO order.  on_device: 1 if vals is a device pointer. */
+int  lsp_batch_set_values(lsp_batch *b, const double *vals, int on_device);
+/* Numeric refactorization of every circuit (one persistent launch). */
+int  lsp_batch_refactor(lsp_batch *b, int prec, lsp_refactor_report *rep);
+/* Direct solve x = Q U^-1 L^-1 P b with the current factors. */
+int  lsp_batch_trisolve(lsp_batch *b, const double *rhs, double *x, int on_device);
+/* fp64 GMRES-IR with the current factors as right preconditioner. */
+int  lsp_batch_solve(lsp_batch *b, const double *rhs, double *x, int on_device,
+                     lsp_solve_report *rep);
+/* y = A x with fp64 (prec = LSP_PREC_FP64) or fp32 values, SELL-C-64. */
+int  lsp_batch_spmv(lsp_batch *b, const double *x, double *y, int on_device, int prec);
+/* Copy the current factors of one circuit to the host in the splu layout
+ * (L unit diagonal first, U diagonal last, rows = pivot positions, rows
+ * sorted inside each column) as fp64.  Arrays sized from lsp_batch_info. */
+int  lsp_batch_export_factors(lsp_batch *b, int circuit, int *pinv,
+                              int *Lp, int *Li, double *Lx,
+                              int *Up, int *Ui, double *Ux);
+/* Device pointers of the vector arenas (length total_n) for zero-copy use. */
+int  lsp_batch_device_vectors(lsp_batch *b, double **rhs, double **x);
+/* Device time of the last refactor / trisolve launches (ms, hipEvent). */
+int  lsp_batch_last_times(const lsp_batch *b, double *refactor_ms,
+                          double *fwd_ms, double *bwd_ms);
+
+/* Running per-kernel device times since creation (or the last reset). */
+int  lsp_batch_kernel_stats(lsp_batch *b, lsp_kernel_stats *st, int reset);
+
+/* ---- host-side pieces exposed for tests and tools ---------------------- */
+/* COLAMD with all knobs zero on an n x n pattern (0-based COO). */
+int  lsp_colamd(int n, long nnz, const int *row, const int *col, int *q);
+/* CPU symbolic LU with the reference's pivoting rule; factors in the splu
+ * layout.  Returns -1 ok, k singular column, -2 arg, -3 arrays too small. */
+int  lsp_symbolic_factor(int n, long nnz, const int *row, const int *col,
+                         const double *val, const int *q, double tol,
+                         int *pinv, int *Lp, int *Li, double *Lx, long lcap,
+                         int *Up, int *Ui, double *Ux, long ucap,
+                         long *lnnz, long *unnz);
+/* Synthetic MNA circuits.  kind 0: 25x40 grid + 20 sources (n = 1020);
+ * kind 1: 316x316 power grid + 50 pads (n = 99906); kind 2: circuit "like
+ * kind 0" with n unknowns and long-range fraction lr.  Two calls: first
+ * with row == NULL to get n and nnz, then with arrays. */
+int  lsp_mna_generate(int kind, int n_req, double lr, uint64_t seed, int *n,
+                      long *nnz, int *row, int *col, double *val, double *rhs);
+/* Newton perturbation: val = val0 * lognormal(0, sigma), rhs ~ U[-1,1]. */
+void lsp_mna_perturb_values(uint64_t seed, int step, double sigma, long nnz,
+                            const double *val0, double *val, int n, double *rhs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,260 @@
+"""luspice_amd -- MI355X (gfx950) mixed-precision sparse LU for SPICE MNA
+matrices: fp32-factor refactorization + fp64 GMRES-IR behind the reference's
+C API (lu_gpu.h / lu_gpu_par.h / spmv.h / selector.h / lu_static.h) and a
+batched, device-resident C ABI (include/luspice.h).
+
+This package is the Python host mirror of that C ABI; all numerics run in
+libluspice_amd.so (HIP kernels + C++ runtime).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _native as N
+from ._native import check
+
+PREC_FP32, PREC_FP64, PREC_FP16 = 0, 1, 2
+_PREC = {"fp32": PREC_FP32, "fp64": PREC_FP64, "fp16": PREC_FP16}
+
+
+def prec_code(p) -> int:
+    return _PREC[p] if isinstance(p, str) else int(p)
+
+
+class Circuit:
+    """An n x n MNA matrix as a 0-based COO pattern plus values and a RHS."""
+
+    def __init__(self, n, row, col, val, rhs=None, q=None):
+        self.n = int(n)
+        self.row = np.ascontiguousarray(row, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.rhs = None if rhs is None else np.ascontiguousarray(rhs, dtype=np.float64)
+        self.q = None if q is None else np.ascontiguousarray(q, dtype=np.int32)
+
+    @property
+    def nnz(self):
+        return int(self.val.size)
+
+    def dense(self):
+        A = np.zeros((self.n, self.n))
+        np.add.at(A, (self.row, self.col), self.val)
+        return A
+
+    def csc(self, q=None):
+        """q-order CSC with original row ids, rows in COO order (build_csc_q)."""
+        n = self.n
+        qq = np.arange(n, dtype=np.int32) if q is None else np.asarray(q, dtype=np.int32)
+        invq = np.empty(n, dtype=np.int64)
+        invq[qq] = np.arange(n)
+        cols = invq[self.col]
+        order = np.argsort(cols, kind="stable")
+        Ap = np.zeros(n + 1, dtype=np.int32)
+        np.add.at(Ap, cols + 1, 1)
+        Ap = np.cumsum(Ap).astype(np.int32)
+        return Ap, self.row[order].copy(), self.val[order].copy()
+
+
+def mna(kind: int | str, n: int = 0, lr: float = 0.02, seed: int = 0) -> Circuit:
+    """Synthetic MNA circuit.  kind 0 / 'c0': 25x40 grid + 20 sources (n=1020);
+    1 / 'c1': 316x316 power grid + 50 pads (n=99906); 2 / 'sized': n unknowns."""
+    k = {"c0": 0, "c1": 1, "sized": 2}.get(kind, kind)
+    nn = N.C.c_int()
+    nz = N.C.c_long()
+    check(N.lib.lsp_mna_generate(k, int(n), float(lr), int(seed), N.C.byref(nn), N.C.byref(nz),
+                                 None, None, None, None), "mna size")
+    row = np.empty(nz.value, np.int32)
+    col = np.empty(nz.value, np.int32)
+    val = np.empty(nz.value, np.float64)
+    rhs = np.empty(nn.value, np.float64)
+    check(N.lib.lsp_mna_generate(k, int(n), float(lr), int(seed), None, None, N.ptr(row),
+                                 N.ptr(col), N.ptr(val), N.ptr(rhs)), "mna")
+    return Circuit(nn.value, row, col, val, rhs)
+
+
+def perturb(seed: int, step: int, val0: np.ndarray, n: int, sigma: float = 0.1):
+    """Newton-step values val0 * lognormal(0, sigma) and a fresh RHS ~ U[-1,1]."""
+    val0 = np.ascontiguousarray(val0, dtype=np.float64)
+    val = np.empty_like(val0)
+    rhs = np.empty(n, np.float64)
+    N.lib.lsp_mna_perturb_values(int(seed), int(step), float(sigma), val0.size, N.ptr(val0),
+                                 N.ptr(val), int(n), N.ptr(rhs))
+    return val, rhs
+
+
+def colamd(c: Circuit) -> np.ndarray:
+    q = np.empty(c.n, np.int32)
+    check(N.lib.lsp_colamd(c.n, c.nnz, N.ptr(c.row), N.ptr(c.col), N.ptr(q)), "colamd")
+    return q
+
+
+def symbolic_factor(c: Circuit, q=None, tol: float = 1.0):
+    """CPU symbolic LU (the pivot sequence and pattern the device reuses).
+    Returns dict(pinv, Lp, Li, Lx, Up, Ui, Ux) in the splu layout."""
+    q = None if q is None else np.ascontiguousarray(q, np.int32)
+    ln, un = N.C.c_long(), N.C.c_long()
+    rc = N.lib.lsp_symbolic_factor(c.n, c.nnz, N.ptr(c.row), N.ptr(c.col), N.ptr(c.val),
+                                   N.ptr(q), tol, None, None, None, None, 0, None, None, None,
+                                   0, N.C.byref(ln), N.C.byref(un))
+    if rc >= 0:
+        raise RuntimeError(f"symbolic factor: singular column {rc}")
+    if rc not in (-1, -3):
+        raise RuntimeError(f"symbolic factor failed: {rc}")
+    out = dict(pinv=np.empty(c.n, np.int32), Lp=np.empty(c.n + 1, np.int32),
+               Li=np.empty(ln.value, np.int32), Lx=np.empty(ln.value),
+               Up=np.empty(c.n + 1, np.int32), Ui=np.empty(un.value, np.int32),
+               Ux=np.empty(un.value))
+    rc = N.lib.lsp_symbolic_factor(c.n, c.nnz, N.ptr(c.row), N.ptr(c.col), N.ptr(c.val),
+                                   N.ptr(q), tol, N.ptr(out["pinv"]), N.ptr(out["Lp"]),
+                                   N.ptr(out["Li"]), N.ptr(out["Lx"]), ln.value,
+                                   N.ptr(out["Up"]), N.ptr(out["Ui"]), N.ptr(out["Ux"]),
+                                   un.value, N.C.byref(ln), N.C.byref(un))
+    if rc != -1:
+        raise RuntimeError(f"symbolic factor failed: {rc}")
+    return out
+
+
+def cpu_lu_ref(c: Circuit, q=None, rhs=None, tol: float = 1.0):
+    """The reference's cpu_lu_ref contract: fp64 factor + permuted solve."""
+    Ap, Ai, Ax = c.csc(None)
+    b = c.rhs if rhs is None else np.ascontiguousarray(rhs, np.float64)
+    x = np.empty(c.n)
+    ln, un = N.C.c_long(), N.C.c_long()
+    qa = None if q is None else np.ascontiguousarray(q, np.int32)
+    rc = N.lib.cpu_lu_ref(c.n, N.ptr(Ap), N.ptr(Ai), N.ptr(Ax), N.ptr(qa), N.ptr(b), tol,
+                          N.ptr(x), N.C.byref(ln), N.C.byref(un))
+    return rc, x, ln.value, un.value
+
+
+def options(**kw) -> N.Options:
+    o = N.Options()
+    N.lib.lsp_options_default(N.C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+class Batch:
+    """Device-resident batch of circuits (one circuit = batch of one).
+
+    Creation runs the host symbolic phase (COLAMD unless q is given, pivot
+    sequence, L/U pattern, level sets, SELL layout) and uploads everything;
+    refactor()/solve() then run on the GPU."""
+
+    def __init__(self, circuits, **opts):
+        if isinstance(circuits, Circuit):
+            circuits = [circuits]
+        self.circuits = list(circuits)
+        arr = (N.CircuitIn * len(self.circuits))()
+        for i, c in enumerate(self.circuits):
+            arr[i].n = c.n
+            arr[i].nnz = c.nnz
+            arr[i].row = N.ptr(c.row)
+            arr[i].col = N.ptr(c.col)
+            arr[i].val = N.ptr(c.val)
+            arr[i].q = N.ptr(c.q)
+        self.opts = options(**opts)
+        h = N.C.c_void_p()
+        rc = N.lib.lsp_batch_create(arr, len(self.circuits), N.C.byref(self.opts), N.C.byref(h))
+        check(rc, "lsp_batch_create")
+        self._h = h
+        tn, tz = N.C.c_long(), N.C.c_long()
+        N.lib.lsp_batch_sizes(h, N.C.byref(tn), N.C.byref(tz))
+        self.total_n, self.total_nnz = tn.value, tz.value
+        self.voff = np.cumsum([0] + [c.n for c in self.circuits])
+        self.aoff = np.cumsum([0] + [c.nnz for c in self.circuits])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.lib.lsp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self, i=0) -> dict:
+        inf = N.CircuitInfo()
+        check(N.lib.lsp_batch_info(self._h, i, N.C.byref(inf)), "info")
+        return {k: getattr(inf, k) for k, _ in N.CircuitInfo._fields_}
+
+    def set_stream(self, stream_handle: int):
+        check(N.lib.lsp_batch_set_stream(self._h, N.C.c_void_p(stream_handle)), "set_stream")
+
+    def set_values(self, vals: np.ndarray):
+        vals = np.ascontiguousarray(vals, np.float64)
+        assert vals.size == self.total_nnz
+        check(N.lib.lsp_batch_set_values(self._h, vals.ctypes.data_as(N.C.c_void_p), 0),
+              "set_values")
+
+    def set_values_device(self, dptr: int):
+        check(N.lib.lsp_batch_set_values(self._h, N.C.c_void_p(dptr), 1), "set_values")
+
+    def refactor(self, prec="fp32") -> dict:
+        rep = N.RefactorReport()
+        rc = N.lib.lsp_batch_refactor(self._h, prec_code(prec), N.C.byref(rep))
+        out = {k: getattr(rep, k) for k, _ in N.RefactorReport._fields_}
+        out["rc"] = rc
+        return out
+
+    def trisolve(self, rhs: np.ndarray) -> np.ndarray:
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        x = np.empty(self.total_n)
+        check(N.lib.lsp_batch_trisolve(self._h, rhs.ctypes.data_as(N.C.c_void_p),
+                                       x.ctypes.data_as(N.C.c_void_p), 0), "trisolve")
+        return x
+
+    def solve(self, rhs: np.ndarray):
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        x = np.empty(self.total_n)
+        rep = N.SolveReport()
+        rc = N.lib.lsp_batch_solve(self._h, rhs.ctypes.data_as(N.C.c_void_p),
+                                   x.ctypes.data_as(N.C.c_void_p), 0, N.C.byref(rep))
+        out = {k: getattr(rep, k) for k, _ in N.SolveReport._fields_}
+        out["rc"] = rc
+        return x, out
+
+    def solve_device(self):
+        """GMRES-IR on the resident RHS arena (see device_vectors)."""
+        b, x = self.device_vectors()
+        rep = N.SolveReport()
+        rc = N.lib.lsp_batch_solve(self._h, N.C.c_void_p(b), N.C.c_void_p(x), 1,
+                                   N.C.byref(rep))
+        out = {k: getattr(rep, k) for k, _ in N.SolveReport._fields_}
+        out["rc"] = rc
+        return out
+
+    def spmv(self, x: np.ndarray, prec="fp64") -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty(self.total_n)
+        check(N.lib.lsp_batch_spmv(self._h, x.ctypes.data_as(N.C.c_void_p),
+                                   y.ctypes.data_as(N.C.c_void_p), 0, prec_code(prec)), "spmv")
+        return y
+
+    def export_factors(self, i=0) -> dict:
+        inf = self.info(i)
+        n = inf["n"]
+        out = dict(pinv=np.empty(n, np.int32), Lp=np.empty(n + 1, np.int32),
+                   Li=np.empty(inf["lnnz"], np.int32), Lx=np.empty(inf["lnnz"]),
+                   Up=np.empty(n + 1, np.int32), Ui=np.empty(inf["unnz"], np.int32),
+                   Ux=np.empty(inf["unnz"]))
+        check(N.lib.lsp_batch_export_factors(
+            self._h, i, N.ptr(out["pinv"]), N.ptr(out["Lp"]), N.ptr(out["Li"]),
+            N.ptr(out["Lx"]), N.ptr(out["Up"]), N.ptr(out["Ui"]), N.ptr(out["Ux"])), "export")
+        return out
+
+    def device_vectors(self):
+        b, x = N.C.c_void_p(), N.C.c_void_p()
+        check(N.lib.lsp_batch_device_vectors(self._h, N.C.byref(b), N.C.byref(x)), "vectors")
+        return b.value, x.value
+
+    def kernel_stats(self, reset=False) -> dict:
+        st = N.KernelStats()
+        check(N.lib.lsp_batch_kernel_stats(self._h, N.C.byref(st), int(reset)), "stats")
+        return {k: getattr(st, k) for k, _ in N.KernelStats._fields_}
+
+    def last_times(self):
+        a, b, c = N.C.c_double(), N.C.c_double(), N.C.c_double()
+        N.lib.lsp_batch_last_times(self._h, N.C.byref(a), N.C.byref(b), N.C.byref(c))
+        return a.value, b.value, c.value

@@ -1,0 +1,111 @@
+// host_internal.h -- host-side structures shared by the C++ runtime.
+//
+// Index spaces used throughout:
+//   original   row/column ids of A as read from the s0 file (0-based here)
+//   q-order    column k of A(:,q) is original column q[k]
+//   position   row position after partial pivoting: pinv[orig_row] = position
+// The factorization is P*A(:,q) = L*U (src/third_party/splu.h:3-8 of the
+// reference): L unit lower triangular, U upper triangular, both indexed by
+// position.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+// ---------------------------------------------------------------------------
+// COO with 0-based indices; b has n entries.
+struct LspCoo {
+    int n = 0;
+    std::vector<int> row, col;
+    std::vector<double> val;
+    std::vector<double> b;
+};
+
+// ---------------------------------------------------------------------------
+// colamd.cpp
+int lsp_colamd_order(int n_row, int n_col, const int *Ap, const int *Ai, int *q);
+
+// ---------------------------------------------------------------------------
+// Gilbert-Peierls LU with the reference's pivoting rule (symbolic.cpp).
+struct LspGpLu {
+    int n = 0;
+    std::vector<int> pinv;                // orig row -> position
+    std::vector<int> Lp, Li;              // L CSC, Li = positions, unit diag first
+    std::vector<double> Lx;
+    std::vector<int> Up, Ui;              // U CSC, Ui = positions, diag last
+    std::vector<double> Ux;
+    long lnnz = 0, unnz = 0;
+    int singular = -1;                    // -1 ok, else column index
+};
+
+// A in q-order CSC with original row ids (Ap/Ai/Ax), diag_row[k] = q[k]
+// (the row preferred as pivot for column k), tol as in splu.h:185.
+int lsp_gp_factor(int n, const int *Ap, const int *Ai, const double *Ax,
+                  const int *diag_row, double tol, LspGpLu &F);
+// x = Q U^-1 L^-1 P b  (src/third_party/splu.h:210-239)
+void lsp_gp_solve(const LspGpLu &F, const int *q, const double *b, double *x);
+
+// ---------------------------------------------------------------------------
+// Static structure for the GPU numeric refactorization and trisolve.
+struct LspPlan {
+    int n = 0;
+    long nnz_a = 0;
+    // column patterns in position space, sorted ascending:
+    //   slots [cp[k], dp[k])     U(:,k) off-diagonal (positions < k)
+    //   slot  dp[k]              diagonal (position k)
+    //   slots (dp[k], cp[k+1])   L(:,k) off-diagonal (positions > k)
+    std::vector<int64_t> cp;              // n+1
+    std::vector<int> dp;                  // n
+    std::vector<int> pos;                 // nslots
+    // A scatter: q-order CSC entries of A -> local slot in their column
+    std::vector<int> aptr;                // n+1 (q-order CSC column pointers)
+    std::vector<int> asrc;                // nnz_a: index into the caller's value array
+    std::vector<int> aslot;               // nnz_a: local slot, -1 = overwritten duplicate
+    // left-looking update map: for column k, for each U slot (dep j,
+    // ascending), for each L slot t of column j: local slot of pos[t] in k
+    std::vector<int64_t> relp;            // n+1
+    std::vector<int> rel;                 // total updates
+    int max_col = 0;                      // max column pattern length
+    // dependency levels of the factorization (U pattern, level.c semantics)
+    std::vector<int> level;               // n
+    int nlevels = 0;
+    std::vector<int> ftask;               // columns sorted by (level, k)
+    // row-wise copies for the row-gather trisolve
+    std::vector<int> lrp, lci;            // L strictly lower, CSR by position
+    std::vector<int> urp, uci;            // U strictly upper, CSR by position
+    std::vector<int> dst;                 // nslots: index into lrow/urow/diag
+    std::vector<int> lvf, lvb;            // row levels forward / backward
+    int nlv_f = 0, nlv_b = 0;
+    std::vector<int> fwd_task, bwd_task;  // rows sorted by level
+    // permutations
+    std::vector<int> q;                   // n: q[k] = original column
+    std::vector<int> rowat;               // n: position -> original row
+    std::vector<int> pinv;                // n
+    // scatter count (update FMAs) and per-column statistics
+    int64_t n_updates = 0;
+    int64_t n_l = 0, n_u = 0;             // off-diagonal counts
+};
+
+// Build the plan from A (q-order CSC, values only used for nothing here) and
+// the symbolic factors.  src_index maps each q-CSC entry to the caller's value
+// order (nullptr = identity).  Returns 0 ok, <0 on an inconsistent structure.
+int lsp_build_plan(int n, const int *Ap, const int *Ai, const int *src_index,
+                   const LspGpLu &F, const int *q, LspPlan &P);
+
+// ---------------------------------------------------------------------------
+// SELL-C-sigma layout of A in original row/column order (sell.cpp)
+struct LspSell {
+    int n = 0, C = 64, sigma = 0;
+    int nslices = 0;
+    std::vector<int64_t> off;             // nslices+1 (slot offsets)
+    std::vector<int> len;                 // nslices
+    std::vector<int> col;                 // slots (padding col = row's own 0)
+    std::vector<int> src;                 // slots -> caller value index, -1 pad
+    std::vector<int> perm;                // slice-order row k -> original row
+    int64_t slots = 0;
+};
+int lsp_build_sell(int n, const int *rowptr, const int *colidx,
+                   const int *src_index, int C, int sigma, LspSell &S);
+
+// ---------------------------------------------------------------------------
+// level.c / selector.c are plain C with the reference's C API.

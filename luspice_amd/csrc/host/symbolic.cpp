@@ -1,0 +1,302 @@
+// symbolic.cpp -- CPU symbolic analysis: pivot sequence, L/U pattern and the
+// static plan the GPU refactorization and trisolve consume.
+//
+// lsp_gp_factor follows the reference's fp64 LU (src/third_party/splu.h:151-206,
+// wrapped by src/cpu_ref_lu.c:15-58): left-looking Gilbert-Peierls LU with a
+// depth-first reach per column, threshold partial pivoting (first strict
+// maximum |x| among unpivoted rows in reach order; the preferred row
+// diag_row[k] wins when |x| >= tol * max), L unit diagonal stored first, U
+// diagonal stored last, L rows remapped to pivot positions at the end.  The
+// pattern is structural: every reached row is stored, including exact zeros,
+// so P, nnz(L), nnz(U) and the per-column entry order equal the reference's.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "host_internal.h"
+
+int lsp_gp_factor(int n, const int *Ap, const int *Ai, const double *Ax,
+                  const int *diag_row, double tol, LspGpLu &F)
+{
+    F = LspGpLu();
+    if (n < 1) return -2;
+    F.n = n;
+    F.pinv.assign(n, -1);
+    F.Lp.assign(n + 1, 0);
+    F.Up.assign(n + 1, 0);
+    F.Li.reserve((size_t)Ap[n] * 4 + 16);
+    F.Lx.reserve((size_t)Ap[n] * 4 + 16);
+    F.Ui.reserve((size_t)Ap[n] * 4 + 16);
+    F.Ux.reserve((size_t)Ap[n] * 4 + 16);
+    std::vector<int> reach(n), stack(n), resume(n), visited(n, 0);
+    std::vector<double> x(n, 0.0);
+    int *pinv = F.pinv.data();
+    for (int k = 0; k < n; k++) {
+        F.Lp[k] = (int)F.Li.size();
+        F.Up[k] = (int)F.Ui.size();
+        const int stamp = k + 1;
+        // depth-first reach of A(:,k) through the L columns built so far
+        int top = n;
+        for (int p = Ap[k]; p < Ap[k + 1]; p++) {
+            int root = Ai[p];
+            if (visited[root] == stamp) continue;
+            int d = 0;
+            stack[0] = root;
+            while (d >= 0) {
+                int v = stack[d];
+                int J = pinv[v];
+                if (visited[v] != stamp) {
+                    visited[v] = stamp;
+                    resume[d] = J < 0 ? 0 : F.Lp[J];
+                }
+                int stop = J < 0 ? 0 : F.Lp[J + 1];
+                bool descended = false;
+                for (int e = resume[d]; e < stop; e++) {
+                    int w = F.Li[e];
+                    if (visited[w] == stamp) continue;
+                    resume[d] = e;
+                    stack[++d] = w;
+                    descended = true;
+                    break;
+                }
+                if (!descended) {
+                    reach[--top] = v;
+                    d--;
+                }
+            }
+        }
+        // sparse triangular solve over the reach (topological order)
+        for (int t = top; t < n; t++) x[reach[t]] = 0.0;
+        for (int p = Ap[k]; p < Ap[k + 1]; p++) x[Ai[p]] = Ax[p];
+        for (int t = top; t < n; t++) {
+            int v = reach[t];
+            int J = pinv[v];
+            if (J < 0) continue;
+            x[v] /= F.Lx[F.Lp[J]];
+            const double xv = x[v];
+            for (int e = F.Lp[J] + 1; e < F.Lp[J + 1]; e++) x[F.Li[e]] -= F.Lx[e] * xv;
+        }
+        int piv = -1;
+        double best = -1.0;
+        for (int t = top; t < n; t++) {
+            int v = reach[t];
+            if (pinv[v] < 0) {
+                double a = std::fabs(x[v]);
+                if (a > best) { best = a; piv = v; }
+            } else {
+                F.Ui.push_back(pinv[v]);
+                F.Ux.push_back(x[v]);
+            }
+        }
+        if (piv == -1 || best <= 0) { F.singular = k; return k; }
+        const int dr = diag_row ? diag_row[k] : k;
+        if (pinv[dr] < 0 && std::fabs(x[dr]) >= best * tol) piv = dr;
+        const double pv = x[piv];
+        F.Ui.push_back(k);
+        F.Ux.push_back(pv);
+        pinv[piv] = k;
+        F.Li.push_back(piv);
+        F.Lx.push_back(1.0);
+        for (int t = top; t < n; t++) {
+            int v = reach[t];
+            if (pinv[v] < 0) {
+                F.Li.push_back(v);
+                F.Lx.push_back(x[v] / pv);
+            }
+            x[v] = 0.0;
+        }
+    }
+    F.Lp[n] = (int)F.Li.size();
+    F.Up[n] = (int)F.Ui.size();
+    for (auto &r : F.Li) r = pinv[r];
+    F.lnnz = (long)F.Li.size();
+    F.unnz = (long)F.Ui.size();
+    return -1;
+}
+
+void lsp_gp_solve(const LspGpLu &F, const int *q, const double *b, double *x)
+{
+    const int n = F.n;
+    std::vector<int> rowat(n, -1);
+    for (int i = 0; i < n; i++) if (F.pinv[i] >= 0) rowat[F.pinv[i]] = i;
+    std::vector<double> w(n), y(n);
+    for (int k = 0; k < n; k++) w[k] = rowat[k] >= 0 ? b[rowat[k]] : 0.0;
+    for (int k = 0; k < n; k++) {
+        y[k] = w[k] / F.Lx[F.Lp[k]];
+        for (int e = F.Lp[k] + 1; e < F.Lp[k + 1]; e++) w[F.Li[e]] -= F.Lx[e] * y[k];
+    }
+    for (int k = n - 1; k >= 0; k--) {
+        y[k] /= F.Ux[F.Up[k + 1] - 1];
+        for (int e = F.Up[k]; e < F.Up[k + 1] - 1; e++) y[F.Ui[e]] -= F.Ux[e] * y[k];
+    }
+    for (int k = 0; k < n; k++) x[q ? q[k] : k] = y[k];
+}
+
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+static void sort_by_level(const std::vector<int> &lv, int nlv, std::vector<int> &out)
+{
+    const int n = (int)lv.size();
+    std::vector<int> cnt(nlv + 1, 0);
+    for (int i = 0; i < n; i++) cnt[lv[i] + 1]++;
+    for (int l = 0; l < nlv; l++) cnt[l + 1] += cnt[l];
+    out.assign(n, 0);
+    for (int i = 0; i < n; i++) out[cnt[lv[i]]++] = i;
+}
+
+int lsp_build_plan(int n, const int *Ap, const int *Ai, const int *src_index,
+                   const LspGpLu &F, const int *q, LspPlan &P)
+{
+    P = LspPlan();
+    P.n = n;
+    P.nnz_a = Ap[n];
+    P.pinv = F.pinv;
+    P.q.assign(q, q + n);
+    P.rowat.assign(n, -1);
+    for (int i = 0; i < n; i++) {
+        if (F.pinv[i] < 0 || F.pinv[i] >= n) return -1;
+        P.rowat[F.pinv[i]] = i;
+    }
+    // column patterns
+    P.cp.assign(n + 1, 0);
+    P.dp.assign(n, 0);
+    for (int k = 0; k < n; k++) {
+        int nu = F.Up[k + 1] - F.Up[k] - 1;
+        int nl = F.Lp[k + 1] - F.Lp[k] - 1;
+        if (nu < 0 || nl < 0) return -2;
+        if (F.Ui[F.Up[k + 1] - 1] != k || F.Li[F.Lp[k]] != k) return -3;
+        P.cp[k + 1] = P.cp[k] + nu + 1 + nl;
+        P.max_col = std::max(P.max_col, nu + 1 + nl);
+    }
+    const int64_t nslots = P.cp[n];
+    P.pos.assign(nslots, 0);
+    for (int k = 0; k < n; k++) {
+        int64_t s = P.cp[k];
+        for (int e = F.Up[k]; e < F.Up[k + 1] - 1; e++) P.pos[s++] = F.Ui[e];
+        std::sort(P.pos.begin() + P.cp[k], P.pos.begin() + s);
+        P.dp[k] = (int)(s - P.cp[k]);            // local offset of the diagonal
+        P.pos[s++] = k;
+        int64_t l0 = s;
+        for (int e = F.Lp[k] + 1; e < F.Lp[k + 1]; e++) P.pos[s++] = F.Li[e];
+        std::sort(P.pos.begin() + l0, P.pos.begin() + s);
+        for (int64_t t = P.cp[k]; t < P.dp[k] + P.cp[k]; t++)
+            if (P.pos[t] >= k) return -4;
+        for (int64_t t = l0; t < s; t++)
+            if (P.pos[t] <= k) return -4;
+        P.n_u += P.dp[k];
+        P.n_l += s - l0;
+    }
+    // dp as absolute slot index
+    for (int k = 0; k < n; k++) P.dp[k] = (int)(P.cp[k] + P.dp[k]);
+    if (nslots > 0x7fffffffLL) return -5;
+    // position -> local slot lookup, one column at a time
+    std::vector<int> where(n, -1), owner(n, -1);
+    P.aptr.assign(Ap, Ap + n + 1);
+    P.asrc.assign(P.nnz_a, 0);
+    P.aslot.assign(P.nnz_a, -1);
+    std::vector<int64_t> last_writer;
+    // count updates first
+    P.relp.assign(n + 1, 0);
+    for (int k = 0; k < n; k++) {
+        int64_t cnt = 0;
+        for (int64_t s = P.cp[k]; s < P.dp[k]; s++) {
+            int j = P.pos[s];
+            cnt += P.cp[j + 1] - P.dp[j] - 1;
+        }
+        P.relp[k + 1] = P.relp[k] + cnt;
+    }
+    P.n_updates = P.relp[n];
+    P.rel.assign(P.n_updates, 0);
+    std::vector<int> slot_writer;
+    for (int k = 0; k < n; k++) {
+        for (int64_t s = P.cp[k]; s < P.cp[k + 1]; s++) {
+            where[P.pos[s]] = (int)(s - P.cp[k]);
+            owner[P.pos[s]] = k;
+        }
+        // A entries; a later duplicate of the same row overwrites (splu assigns)
+        slot_writer.assign(P.cp[k + 1] - P.cp[k], -1);
+        for (int p = Ap[k]; p < Ap[k + 1]; p++) {
+            int r = Ai[p];
+            if (r < 0 || r >= n) return -6;
+            int ps = F.pinv[r];
+            if (owner[ps] != k) return -7;         // A entry outside the pattern
+            int loc = where[ps];
+            if (slot_writer[loc] >= 0) P.aslot[slot_writer[loc]] = -1;
+            slot_writer[loc] = p;
+            P.aslot[p] = loc;
+            P.asrc[p] = src_index ? src_index[p] : p;
+        }
+        int64_t u = P.relp[k];
+        for (int64_t s = P.cp[k]; s < P.dp[k]; s++) {
+            int j = P.pos[s];
+            for (int64_t t = P.dp[j] + 1; t < P.cp[j + 1]; t++) {
+                int ps = P.pos[t];
+                if (owner[ps] != k) return -8;     // fill outside the pattern
+                P.rel[u++] = where[ps];
+            }
+        }
+    }
+    // factor levels (src/level.c:20-40 on the U pattern)
+    P.level.assign(n, 0);
+    int deepest = 0;
+    for (int k = 0; k < n; k++) {
+        int lv = 0;
+        for (int64_t s = P.cp[k]; s < P.dp[k]; s++) lv = std::max(lv, P.level[P.pos[s]] + 1);
+        P.level[k] = lv;
+        deepest = std::max(deepest, lv);
+    }
+    P.nlevels = deepest + 1;
+    sort_by_level(P.level, P.nlevels, P.ftask);
+    // row-wise L (strict lower) and U (strict upper)
+    P.lrp.assign(n + 1, 0);
+    P.urp.assign(n + 1, 0);
+    for (int k = 0; k < n; k++) {
+        for (int64_t s = P.cp[k]; s < P.dp[k]; s++) P.urp[P.pos[s] + 1]++;
+        for (int64_t s = P.dp[k] + 1; s < P.cp[k + 1]; s++) P.lrp[P.pos[s] + 1]++;
+    }
+    for (int i = 0; i < n; i++) { P.lrp[i + 1] += P.lrp[i]; P.urp[i + 1] += P.urp[i]; }
+    P.lci.assign(P.lrp[n], 0);
+    P.uci.assign(P.urp[n], 0);
+    P.dst.assign(nslots, 0);
+    {
+        std::vector<int> lcur(P.lrp.begin(), P.lrp.end() - 1);
+        std::vector<int> ucur(P.urp.begin(), P.urp.end() - 1);
+        for (int k = 0; k < n; k++) {               // k ascending -> rows sorted by column
+            for (int64_t s = P.cp[k]; s < P.dp[k]; s++) {
+                int i = P.pos[s];
+                int d = ucur[i]++;
+                P.uci[d] = k;
+                P.dst[s] = d;
+            }
+            P.dst[P.dp[k]] = k;
+            for (int64_t s = P.dp[k] + 1; s < P.cp[k + 1]; s++) {
+                int i = P.pos[s];
+                int d = lcur[i]++;
+                P.lci[d] = k;
+                P.dst[s] = d;
+            }
+        }
+    }
+    // trisolve row levels
+    P.lvf.assign(n, 0);
+    P.lvb.assign(n, 0);
+    int df = 0, db = 0;
+    for (int i = 0; i < n; i++) {
+        int lv = 0;
+        for (int e = P.lrp[i]; e < P.lrp[i + 1]; e++) lv = std::max(lv, P.lvf[P.lci[e]] + 1);
+        P.lvf[i] = lv;
+        df = std::max(df, lv);
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        int lv = 0;
+        for (int e = P.urp[i]; e < P.urp[i + 1]; e++) lv = std::max(lv, P.lvb[P.uci[e]] + 1);
+        P.lvb[i] = lv;
+        db = std::max(db, lv);
+    }
+    P.nlv_f = df + 1;
+    P.nlv_b = db + 1;
+    sort_by_level(P.lvf, P.nlv_f, P.fwd_task);
+    sort_by_level(P.lvb, P.nlv_b, P.bwd_task);
+    return 0;
+}

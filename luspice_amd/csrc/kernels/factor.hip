@@ -1,0 +1,208 @@
+// factor.hip -- persistent left-looking numeric refactorization (gfx950).
+//
+// Reference behaviour: the numeric phase of src/third_party/splu.h:151-206
+// (per column: scatter A(:,k), apply every earlier L column j with U(j,k) != 0
+// in ascending pivot order, U(:,k) = x on pivotal rows, pivot = x at the
+// pivot row, L(:,k) = x / pivot elsewhere) with the pivot sequence and
+// pattern fixed by the CPU symbolic analysis, the way a Newton refactor reuses
+// them.  The reference launches one kernel per level
+// (bench/test_lu_wave.cu:200-217, src/mp/multiblock_lu.cu:423-431); here one
+// persistent launch walks every level of every matrix in the batch.
+//
+// Mapping to CDNA4
+//   * one wave64 per column task; tasks come from a global queue in level
+//     order, so a task only waits on tasks handed out earlier, which are held
+//     by running waves: the schedule cannot deadlock whatever the occupancy;
+//   * the column's working vector x (fp64) lives in LDS when it fits
+//     (cap doubles per wave), else in a per-wave global scratch slice;
+//   * dependencies are checked 64 at a time (one lane per dependency), then
+//     applied in order; each dependency is a gather of fp32/fp64 L values and
+//     16-bit local slot indices (precomputed by the plan), so no search and no
+//     atomics touch x;
+//   * the pivot test (|pivot| >= ptol * max |L(:,k)|, in fp64) uses a wave
+//     max-reduction;
+//   * the finished column is stored in column order (read by later columns)
+//     and scattered into row-wise arrays for the trisolve, then published
+//     with a release store of the launch epoch.
+#include "common.hpp"
+#include "launch.hpp"
+
+namespace lsp {
+
+
+template <typename T>
+__device__ void factor_column(const FactorDesc<T> &M, int k, double *x,
+                              int epoch, int *abort_word, double ptol)
+{
+    const int lane = lane_id();
+    const int64_t c0 = M.cp[k];
+    const int m = (int)(M.cp[k + 1] - c0);
+    const int dl = M.dp[k] - (int)c0;          // local slot of the diagonal
+
+    for (int i = lane; i < m; i += kWave) x[i] = 0.0;
+    wave_sync();
+    for (int p = M.aptr[k] + lane; p < M.aptr[k + 1]; p += kWave) {
+        int s = M.aslot[p];
+        if (s >= 0) x[s] = M.ain[M.asrc[p]];
+    }
+    wave_sync();
+
+    int64_t u = M.relp[k];
+    for (int s0 = 0; s0 < dl; s0 += kWave) {
+        const int nd = min(kWave, dl - s0);
+        int l0 = 0, len = 0;
+        bool ok = true;
+        if (lane < nd) {
+            const int j = M.pos[c0 + s0 + lane];
+            ok = wait_epoch(&M.done[j], epoch, abort_word);
+            l0 = M.dp[j] + 1;
+            len = (int)(M.cp[j + 1] - l0);
+        }
+        if (!__all(ok)) return;                    // aborted; kernel drains
+        for (int d = 0; d < nd; d++) {
+            const int jl0 = __builtin_amdgcn_readlane(l0, d);
+            const int jlen = __builtin_amdgcn_readlane(len, d);
+            const double xj = x[s0 + d];
+            if (xj != 0.0) {
+                for (int e = lane; e < jlen; e += kWave) {
+                    const int tgt = M.rel[u + e];
+                    x[tgt] -= to_f64(M.val[jl0 + e]) * xj;
+                }
+            }
+            u += jlen;
+            wave_sync();
+        }
+    }
+
+    const double piv = x[dl];
+    double amax = 0.0;
+    int nonfinite = 0;
+    for (int i = dl + 1 + lane; i < m; i += kWave) {
+        const double a = fabs(x[i]);
+        amax = fmax(amax, a);
+        nonfinite |= !(a <= 1.0e308);
+    }
+    amax = wave_max(amax);
+    nonfinite = __any(nonfinite) || !(fabs(piv) <= 1.0e308);
+    if (lane == 0) {
+        if (piv == 0.0) {
+            atomicMin(&M.mstat[2], k);
+        } else if (!(fabs(piv) >= ptol * amax)) {
+            atomicAdd(&M.mstat[0], 1);
+            atomicMin(&M.mstat[1], k);
+        }
+        if (nonfinite) atomicAdd(&M.mstat[3], 1);
+    }
+    for (int i = lane; i < dl; i += kWave) {
+        const T v = from_f64<T>(x[i]);
+        M.val[c0 + i] = v;
+        M.urow[M.dst[c0 + i]] = v;
+    }
+    if (lane == 0) {
+        const T v = from_f64<T>(piv);
+        M.val[c0 + dl] = v;
+        M.diag[k] = v;
+    }
+    for (int i = dl + 1 + lane; i < m; i += kWave) {
+        const T v = from_f64<T>(x[i] / piv);
+        M.val[c0 + i] = v;
+        M.lrow[M.dst[c0 + i]] = v;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0) st_release(&M.done[k], epoch);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) refactor_kernel(FactorLaunch L)
+{
+    extern __shared__ double lds[];
+    const int lane = lane_id();
+    const int wv = threadIdx.x >> 6;
+    const int waves_per_block = blockDim.x >> 6;
+    double *xl = lds + (long)wv * L.cap;
+    double *xg = L.scratch + ((long)blockIdx.x * waves_per_block + wv) * L.scratch_stride;
+    const FactorDesc<T> *mats = (const FactorDesc<T> *)L.mats;
+
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(&L.ctr[0], 1);
+        t = __shfl(t, 0, kWave);
+        if (t >= L.ntask) break;
+        if (ld_relaxed(L.abort_word) != 0) continue;          // drain
+        const int2 tk = L.task[t];
+        if (L.active && !L.active[tk.x]) continue;
+        const FactorDesc<T> M = mats[tk.x];
+        const int k = tk.y;
+        const int m = (int)(M.cp[k + 1] - M.cp[k]);
+        if (m <= L.cap) {
+            factor_column<T>(M, k, xl, L.epoch, L.abort_word, L.ptol);
+        } else if (m <= L.scratch_stride) {
+            factor_column<T>(M, k, xg, L.epoch, L.abort_word, L.ptol);
+        } else {
+            if (lane == 0) st_relaxed(L.abort_word, kErrCapacity);
+        }
+    }
+    // the last wave out re-arms the queue for the next launch
+    if (lane == 0) {
+        const int fin = atomicAdd(&L.ctr[1], 1);
+        if (fin == L.total_waves - 1) {
+            st_relaxed(&L.ctr[0], 0);
+            st_relaxed(&L.ctr[1], 0);
+        }
+    }
+}
+
+// Per-matrix status words are reset by a kernel (not a memset) so the whole
+// refactor sequence stays capturable in a HIP graph.
+__global__ void reset_mstat_kernel(int *mstat, int nmat, int *abort_word)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nmat) {
+        mstat[4 * i + 0] = 0;
+        mstat[4 * i + 1] = 0x7fffffff;
+        mstat[4 * i + 2] = 0x7fffffff;
+        mstat[4 * i + 3] = 0;
+    }
+    if (i == 0 && abort_word) *abort_word = 0;
+}
+
+}  // namespace lsp
+
+// ---------------------------------------------------------------------------
+// host launchers (C++ linkage, used by the runtime)
+// ---------------------------------------------------------------------------
+namespace lsp {
+
+template <typename T>
+hipError_t launch_refactor(const FactorLaunch &L, int blocks, hipStream_t st)
+{
+    const size_t lds = (size_t)L.cap * sizeof(double) * 4;
+    hipLaunchKernelGGL(refactor_kernel<T>, dim3(blocks), dim3(256), lds, st, L);
+    return hipGetLastError();
+}
+
+template hipError_t launch_refactor<float>(const FactorLaunch &, int, hipStream_t);
+template hipError_t launch_refactor<double>(const FactorLaunch &, int, hipStream_t);
+template hipError_t launch_refactor<__half>(const FactorLaunch &, int, hipStream_t);
+
+hipError_t launch_reset_mstat(int *mstat, int nmat, int *abort_word, hipStream_t st)
+{
+    const int threads = 256;
+    const int blocks = (nmat + threads - 1) / threads;
+    hipLaunchKernelGGL(reset_mstat_kernel, dim3(blocks > 0 ? blocks : 1), dim3(threads), 0, st,
+                       mstat, nmat, abort_word);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t refactor_set_lds(int cap)
+{
+    const int bytes = cap * (int)sizeof(double) * 4;
+    return hipFuncSetAttribute((const void *)refactor_kernel<T>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+template hipError_t refactor_set_lds<float>(int);
+template hipError_t refactor_set_lds<double>(int);
+template hipError_t refactor_set_lds<__half>(int);
+
+}  // namespace lsp

@@ -1,0 +1,105 @@
+// launch.hpp -- launch argument blocks and host launchers of the kernels.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "desc.hpp"
+
+namespace lsp {
+
+struct FactorLaunch {
+    const void *mats;       // FactorDesc<T>[nmat]
+    const int2 *task;       // (matrix, column) in level order
+    const int *active;      // per-matrix mask (nullptr = all)
+    int ntask;
+    int epoch;
+    int *ctr;               // [0] queue head, [1] waves finished
+    int *abort_word;
+    double *scratch;        // per-wave global fallback
+    long scratch_stride;    // doubles per wave slice
+    int cap;                // LDS doubles per wave
+    int total_waves;
+    double ptol;
+};
+
+struct TriLaunch {
+    const void *mats;       // TriDesc<T>[nmat]
+    const int2 *task;       // (matrix, row) in level order
+    const long *voff;       // per-matrix vector offsets
+    const int *active;      // per-matrix mask (nullptr = all)
+    const double *b;        // right-hand sides (original row order)
+    double *x;              // solutions (original column order)
+    double *y, *z;          // work vectors (position order)
+    int ntask;
+    int epoch;
+    int *ctr;
+    int *abort_word;
+    int total_waves;
+    int pad_;
+};
+
+struct KrylovArgs {
+    int nmat;
+    int mmax;               // restart length m
+    long ntot;              // arena length
+    const long *voff;       // per matrix
+    const int4 *chunks;     // (matrix, r0, r1, 0)
+    int nchunks;
+    const int *cbeg;        // nmat+1 chunk ranges per matrix
+    double *partial;        // nchunks * (mmax+1)
+    double *H;              // nmat * (mmax+1) * mmax, H[i*mmax + j]
+    double *cs, *sn;        // nmat * mmax
+    double *g;              // nmat * (mmax+1)
+    double *yv;             // nmat * mmax
+    double *beta;           // nmat: ||r|| at cycle start
+    double *nb;             // nmat: ||b||
+    double *hn;             // nmat: ||w|| of the current step
+    double *hcol;           // nmat * (mmax+1): scratch for one Arnoldi column
+    int *act;               // nmat: 1 while the inner cycle runs
+    int *jfin;              // nmat: inner steps taken in the cycle
+    int *conv;              // nmat: 1 once the outer test passed
+    int *cycled;            // nmat: 1 if the cycle produced an update
+    double inner_tol, outer_tol;
+};
+
+// factor.hip
+template <typename T> hipError_t launch_refactor(const FactorLaunch &L, int blocks, hipStream_t st);
+template <typename T> hipError_t refactor_set_lds(int cap);
+hipError_t launch_reset_mstat(int *mstat, int nmat, int *abort_word, hipStream_t st);
+
+// trisolve.hip
+template <typename T>
+hipError_t launch_trisolve_half(const TriLaunch &L, bool forward, int blocks, hipStream_t st);
+
+// spmv.hip
+template <typename V>
+hipError_t launch_sell_spmv(const SellDesc<V> *mats, const int2 *tasks, int ntask,
+                            const int64_t *voff, const double *x, double *y, const double *b,
+                            const int *active, hipStream_t st);
+hipError_t launch_csr_spmv(int n, const int *rowptr, const int *colidx, const double *values,
+                           const double *x, double *y, hipStream_t st);
+template <typename Vout>
+hipError_t launch_gather_cast(long count, const int *idx, const double *in, Vout *out,
+                              hipStream_t st);
+
+// gmres.hip
+hipError_t k_chunk_dots(const KrylovArgs &K, const double *V, const double *w, int nv, hipStream_t s);
+hipError_t k_reduce(const KrylovArgs &K, int nv, int mode, int j, double *out, hipStream_t s);
+hipError_t k_chunk_update(const KrylovArgs &K, const double *V, double *w, int nv, hipStream_t s);
+hipError_t k_chunk_scale(const KrylovArgs &K, const double *src, double *dst, const double *den,
+                         hipStream_t s);
+hipError_t k_start_cycle(const KrylovArgs &K, int *n_active, hipStream_t s);
+hipError_t k_givens(const KrylovArgs &K, int j, int *n_active, hipStream_t s);
+hipError_t k_backsolve(const KrylovArgs &K, hipStream_t s);
+hipError_t k_chunk_combine(const KrylovArgs &K, const double *V, double *t, hipStream_t s);
+hipError_t k_chunk_axpy(const KrylovArgs &K, double *x, const double *z, hipStream_t s);
+hipError_t k_chunk_fill(const KrylovArgs &K, double *x, double v, hipStream_t s);
+hipError_t k_activate(const KrylovArgs &K, hipStream_t s);
+hipError_t k_set_all_active(const KrylovArgs &K, int v, hipStream_t s);
+hipError_t k_zero(double *p, long n, hipStream_t s);
+hipError_t k_zero_int(int *p, long n, hipStream_t s);
+hipError_t k_copy(const double *src, double *dst, long n, hipStream_t s);
+
+}  // namespace lsp

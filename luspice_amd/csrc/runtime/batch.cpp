@@ -1,0 +1,899 @@
+// batch.cpp -- host runtime of the device-resident batch.
+//
+// create(): per circuit, on host threads: COLAMD (unless q is given), the CPU
+//   symbolic LU with the reference's pivoting rule, the static plan (column
+//   patterns, A scatter map, update map, levels, row-wise copies) and the
+//   SELL-C-64 layout.  Then every structure array of every circuit is
+//   concatenated into one device arena and the per-circuit descriptors are
+//   written.  Task lists of all circuits are merged by level so that one
+//   persistent launch advances every circuit's dependency front at once.
+// refactor(): value repack for SpMV + one persistent refactor launch.
+// solve(): fp64 GMRES-IR, every vector on the device; the host reads one
+//   word per inner step (the number of circuits still iterating).
+#include "batch.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+#include "../host/csc.h"
+
+namespace lsp {
+
+static std::atomic<int> g_epoch{0};
+
+int Batch::next_epoch()
+{
+    int e = ++g_epoch;
+    if (e <= 0) { g_epoch = 1; e = 1; }
+    return e;
+}
+
+Batch::~Batch()
+{
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (auto &m : mem_) if (m.p) (void)hipFree(m.p);
+    if (h_pin_) (void)hipHostFree(h_pin_);
+    for (auto &e : ev_) if (e) (void)hipEventDestroy(e);
+    if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
+}
+
+int Batch::check(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return LSP_OK;
+    last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return LSP_E_HIP;
+}
+
+void *Batch::dalloc(size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    mem_.push_back({p, bytes});
+    return p;
+}
+
+template <typename X>
+X *Batch::upload(const std::vector<X> &v)
+{
+    X *d = (X *)dalloc(v.size() * sizeof(X));
+    if (!d) return nullptr;
+    if (!v.empty() && hipMemcpy(d, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice) != hipSuccess)
+        return nullptr;
+    return d;
+}
+
+int Batch::sync_read_int(const int *d, int *h, int count)
+{
+    int rc = check(hipMemcpyAsync(h, d, count * sizeof(int), hipMemcpyDeviceToHost, stream_),
+                   "read flags");
+    if (rc) return rc;
+    rc = check(hipStreamSynchronize(stream_), "sync");
+    if (!rc) account_trisolve();
+    return rc;
+}
+
+// Adds the device time of the last forward/backward trisolve pair to the
+// running statistics; called after a stream synchronisation, when its events
+// are complete.
+void Batch::account_trisolve()
+{
+    if (!tri_pending_) return;
+    float a = 0, b = 0;
+    if (hipEventElapsedTime(&a, ev_[2], ev_[4]) == hipSuccess &&
+        hipEventElapsedTime(&b, ev_[4], ev_[3]) == hipSuccess) {
+        t_fwd_ = a;
+        t_bwd_ = b;
+        stats_.fwd_ms += a;
+        stats_.bwd_ms += b;
+        stats_.tri_count++;
+    }
+    tri_pending_ = false;
+}
+
+// ---------------------------------------------------------------------------
+static int build_circuit(CircuitHost &C, const lsp_circuit_in &in, const lsp_options &opt)
+{
+    C.n = in.n;
+    C.nnz = in.nnz;
+    C.row.assign(in.row, in.row + in.nnz);
+    C.col.assign(in.col, in.col + in.nnz);
+    C.val0.assign(in.val, in.val + in.nnz);
+    for (long e = 0; e < in.nnz; e++)
+        if (C.row[e] < 0 || C.row[e] >= C.n || C.col[e] < 0 || C.col[e] >= C.n) return LSP_E_ARG;
+    C.q.resize(C.n);
+    if (in.q) {
+        std::vector<char> seen(C.n, 0);
+        for (int k = 0; k < C.n; k++) {
+            if (in.q[k] < 0 || in.q[k] >= C.n || seen[in.q[k]]) return LSP_E_ARG;
+            seen[in.q[k]] = 1;
+            C.q[k] = in.q[k];
+        }
+    } else if (lsp_colamd_coo(C.n, C.nnz, C.row.data(), C.col.data(), C.q.data()) != 0) {
+        return LSP_E_STRUCTURE;
+    }
+    std::vector<int> Ap, Ai, src;
+    lsp_coo_to_csc(C.n, C.nnz, C.row.data(), C.col.data(), C.q.data(), Ap, Ai, src);
+    std::vector<double> Ax(C.nnz);
+    for (long p = 0; p < C.nnz; p++) Ax[p] = C.val0[src[p]];
+    int rc = lsp_gp_factor(C.n, Ap.data(), Ai.data(), Ax.data(), C.q.data(), opt.pivot_tol, C.sym);
+    if (rc != -1) return LSP_E_SINGULAR;
+    if (lsp_build_plan(C.n, Ap.data(), Ai.data(), src.data(), C.sym, C.q.data(), C.plan) != 0)
+        return LSP_E_STRUCTURE;
+    if (C.plan.max_col > 65535) return LSP_E_CAPACITY;
+    std::vector<int> rp, ci, rsrc;
+    lsp_coo_to_csr(C.n, C.nnz, C.row.data(), C.col.data(), rp, ci, rsrc);
+    if (lsp_build_sell(C.n, rp.data(), ci.data(), rsrc.data(), 64, opt.sell_sigma, C.sell) != 0)
+        return LSP_E_STRUCTURE;
+    return LSP_OK;
+}
+
+template <typename T>
+int Batch::ensure_factor_set(FactorSet<T> &fs)
+{
+    if (fs.ready) return LSP_OK;
+    fs.val = (T *)dalloc(nslot_tot_ * sizeof(T));
+    fs.lrow = (T *)dalloc(nl_tot_ * sizeof(T));
+    fs.urow = (T *)dalloc(nu_tot_ * sizeof(T));
+    fs.diag = (T *)dalloc(ntot_ * sizeof(T));
+    if (!fs.val || !fs.lrow || !fs.urow || !fs.diag) return LSP_E_ALLOC;
+    const int nc = (int)circ_.size();
+    std::vector<FactorDesc<T>> fd(nc);
+    std::vector<TriDesc<T>> td(nc);
+    for (int m = 0; m < nc; m++) {
+        const CircuitHost &C = circ_[m];
+        FactorDesc<T> &F = fd[m];
+        F.cp = d_cp_ + C.o_np1;
+        F.dp = d_dp_ + C.o_n;
+        F.pos = d_pos_ + C.o_slot;
+        F.aptr = d_aptr_ + C.o_np1;
+        F.asrc = d_asrc_ + C.o_a;
+        F.aslot = d_aslot_ + C.o_a;
+        F.ain = d_ain_ + C.aoff;
+        F.relp = d_relp_ + C.o_np1;
+        F.rel = d_rel_ + C.o_rel;
+        F.dst = d_dst_ + C.o_slot;
+        F.val = fs.val + C.o_slot;
+        F.lrow = fs.lrow + C.o_l;
+        F.urow = fs.urow + C.o_u;
+        F.diag = fs.diag + C.o_n;
+        F.done = d_done_ + C.o_n;
+        F.mstat = d_mstat_ + 4 * m;
+        F.n = C.n;
+        TriDesc<T> &D = td[m];
+        D.lrp = d_lrp_ + C.o_np1;
+        D.lci = d_lci_ + C.o_l;
+        D.urp = d_urp_ + C.o_np1;
+        D.uci = d_uci_ + C.o_u;
+        D.lrow = F.lrow;
+        D.urow = F.urow;
+        D.diag = F.diag;
+        D.rowat = d_rowat_ + C.o_n;
+        D.q = d_q_ + C.o_n;
+        D.fdone = d_fdone_ + C.o_n;
+        D.bdone = d_bdone_ + C.o_n;
+        D.n = C.n;
+    }
+    fs.fdesc = upload(fd);
+    fs.tdesc = upload(td);
+    if (!fs.fdesc || !fs.tdesc) return LSP_E_ALLOC;
+    fs.ready = true;
+    return LSP_OK;
+}
+
+int Batch::create(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
+{
+    if (!cin || nc <= 0) return LSP_E_ARG;
+    if (opt) opt_ = *opt; else lsp_options_default(&opt_);
+    int dev = 0;
+    hipDeviceProp_t prop;
+    int rc = check(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    rc = check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    if (rc) return rc;
+    cu_count_ = prop.multiProcessorCount;
+    rc = check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream");
+    if (rc) return rc;
+    own_stream_ = true;
+
+    // ---- host symbolic phase, circuits in parallel ----
+    circ_.resize(nc);
+    std::vector<int> rcs(nc, 0);
+    {
+        std::atomic<int> next{0};
+        int nthreads = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
+        nthreads = std::min(nthreads, nc);
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nthreads; t++)
+            pool.emplace_back([&]() {
+                for (int m = next++; m < nc; m = next++) rcs[m] = build_circuit(circ_[m], cin[m], opt_);
+            });
+        for (auto &th : pool) th.join();
+    }
+    for (int m = 0; m < nc; m++) {
+        if (rcs[m] != LSP_OK) {
+            last_error = "circuit " + std::to_string(m) + ": symbolic phase failed";
+            return rcs[m];
+        }
+    }
+
+    // ---- concatenate structures ----
+    long o_np1 = 0, o_n = 0, o_slot = 0, o_a = 0, o_rel = 0, o_l = 0, o_u = 0, o_sl = 0,
+         o_slp1 = 0, o_ss = 0;
+    for (auto &C : circ_) {
+        C.o_np1 = o_np1; C.o_n = o_n; C.o_slot = o_slot; C.o_a = o_a; C.o_rel = o_rel;
+        C.o_l = o_l; C.o_u = o_u; C.o_sl = o_sl; C.o_slp1 = o_slp1; C.o_ss = o_ss;
+        C.voff = o_n;
+        C.aoff = o_a;
+        o_np1 += C.n + 1; o_n += C.n; o_slot += C.plan.cp[C.n]; o_a += C.nnz;
+        o_rel += C.plan.n_updates; o_l += C.plan.n_l; o_u += C.plan.n_u;
+        o_sl += C.sell.nslices; o_slp1 += C.sell.nslices + 1; o_ss += C.sell.slots;
+    }
+    ntot_ = o_n; nnz_tot_ = o_a; nslot_tot_ = o_slot; nl_tot_ = o_l; nu_tot_ = o_u;
+    nsell_tot_ = o_ss;
+    {
+        std::vector<int64_t> cp, relp, soff;
+        std::vector<int> dp, pos, aptr, asrc, aslot, dst, lrp, lci, urp, uci, rowat, q;
+        std::vector<int> srlen, scol, sperm, ssrc;
+        std::vector<uint16_t> rel;
+        cp.reserve(o_np1); relp.reserve(o_np1); rel.reserve(o_rel);
+        for (auto &C : circ_) {
+            const LspPlan &P = C.plan;
+            cp.insert(cp.end(), P.cp.begin(), P.cp.end());
+            relp.insert(relp.end(), P.relp.begin(), P.relp.end());
+            dp.insert(dp.end(), P.dp.begin(), P.dp.end());
+            pos.insert(pos.end(), P.pos.begin(), P.pos.end());
+            aptr.insert(aptr.end(), P.aptr.begin(), P.aptr.end());
+            asrc.insert(asrc.end(), P.asrc.begin(), P.asrc.end());
+            aslot.insert(aslot.end(), P.aslot.begin(), P.aslot.end());
+            dst.insert(dst.end(), P.dst.begin(), P.dst.end());
+            lrp.insert(lrp.end(), P.lrp.begin(), P.lrp.end());
+            lci.insert(lci.end(), P.lci.begin(), P.lci.end());
+            urp.insert(urp.end(), P.urp.begin(), P.urp.end());
+            uci.insert(uci.end(), P.uci.begin(), P.uci.end());
+            rowat.insert(rowat.end(), P.rowat.begin(), P.rowat.end());
+            q.insert(q.end(), P.q.begin(), P.q.end());
+            for (int r : P.rel) rel.push_back((uint16_t)r);
+            const LspSell &S = C.sell;
+            soff.insert(soff.end(), S.off.begin(), S.off.end());
+            std::vector<int> rl(C.n);
+            for (int k = 0; k < C.n; k++) rl[k] = 0;
+            // stored row lengths in slice order
+            {
+                std::vector<int> rp, ci, rsrc;
+                lsp_coo_to_csr(C.n, C.nnz, C.row.data(), C.col.data(), rp, ci, rsrc);
+                for (int k = 0; k < C.n; k++) rl[k] = rp[S.perm[k] + 1] - rp[S.perm[k]];
+            }
+            srlen.insert(srlen.end(), rl.begin(), rl.end());
+            scol.insert(scol.end(), S.col.begin(), S.col.end());
+            sperm.insert(sperm.end(), S.perm.begin(), S.perm.end());
+            for (int s : S.src) ssrc.push_back(s >= 0 ? (int)(s + C.aoff) : -1);
+        }
+        d_cp_ = upload(cp); d_relp_ = upload(relp); d_soff_ = upload(soff);
+        d_dp_ = upload(dp); d_pos_ = upload(pos); d_aptr_ = upload(aptr);
+        d_asrc_ = upload(asrc); d_aslot_ = upload(aslot); d_dst_ = upload(dst);
+        d_lrp_ = upload(lrp); d_lci_ = upload(lci); d_urp_ = upload(urp); d_uci_ = upload(uci);
+        d_rowat_ = upload(rowat); d_q_ = upload(q); d_rel_ = upload(rel);
+        d_srlen_ = upload(srlen); d_scol_ = upload(scol); d_sperm_ = upload(sperm);
+        d_ssrc_ = upload(ssrc);
+        if (!d_cp_ || !d_relp_ || !d_soff_ || !d_dp_ || !d_pos_ || !d_aptr_ || !d_asrc_ ||
+            !d_aslot_ || !d_dst_ || !d_lrp_ || !d_lci_ || !d_urp_ || !d_uci_ || !d_rowat_ ||
+            !d_q_ || !d_rel_ || !d_srlen_ || !d_scol_ || !d_sperm_ || !d_ssrc_)
+            return LSP_E_ALLOC;
+    }
+    // flags start at 0; epochs start at 1
+    {
+        std::vector<int> zeros(ntot_, 0);
+        d_done_ = upload(zeros); d_fdone_ = upload(zeros); d_bdone_ = upload(zeros);
+        std::vector<int> z6(8, 0);
+        d_ctr_ = upload(z6); d_abort_ = upload(z6);
+        std::vector<int> ms(4 * nc, 0);
+        d_mstat_ = upload(ms);
+        if (!d_done_ || !d_fdone_ || !d_bdone_ || !d_ctr_ || !d_abort_ || !d_mstat_) return LSP_E_ALLOC;
+    }
+    d_ain_ = (double *)dalloc(nnz_tot_ * sizeof(double));
+    d_sval64_ = (double *)dalloc(nsell_tot_ * sizeof(double));
+    d_sval32_ = (float *)dalloc(nsell_tot_ * sizeof(float));
+    if (!d_ain_ || !d_sval64_ || !d_sval32_) return LSP_E_ALLOC;
+    {
+        std::vector<double> v0(nnz_tot_);
+        for (auto &C : circ_) std::copy(C.val0.begin(), C.val0.end(), v0.begin() + C.aoff);
+        rc = check(hipMemcpy(d_ain_, v0.data(), v0.size() * sizeof(double), hipMemcpyHostToDevice),
+                   "upload values");
+        if (rc) return rc;
+    }
+    // SELL descriptors and slice tasks
+    {
+        std::vector<SellDesc<double>> s64(nc);
+        std::vector<SellDesc<float>> s32(nc);
+        std::vector<int2> st;
+        for (int m = 0; m < nc; m++) {
+            const CircuitHost &C = circ_[m];
+            s64[m].off = d_soff_ + C.o_slp1;
+            s64[m].rlen = d_srlen_ + C.o_n;
+            s64[m].col = d_scol_ + C.o_ss;
+            s64[m].val = d_sval64_ + C.o_ss;
+            s64[m].perm = d_sperm_ + C.o_n;
+            s64[m].nslices = C.sell.nslices;
+            s64[m].n = C.n;
+            s32[m].off = s64[m].off; s32[m].rlen = s64[m].rlen; s32[m].col = s64[m].col;
+            s32[m].val = d_sval32_ + C.o_ss; s32[m].perm = s64[m].perm;
+            s32[m].nslices = C.sell.nslices; s32[m].n = C.n;
+            for (int s = 0; s < C.sell.nslices; s++) st.push_back(make_int2(m, s));
+        }
+        d_sdesc64_ = upload(s64);
+        d_sdesc32_ = upload(s32);
+        d_stask_ = upload(st);
+        nstask_ = (int)st.size();
+        if (!d_sdesc64_ || !d_sdesc32_ || !d_stask_) return LSP_E_ALLOC;
+    }
+    // merged task lists: (level, circuit, index)
+    {
+        auto merge = [&](auto level_of, auto order_of) {
+            std::vector<std::pair<long, int2>> all;
+            all.reserve(ntot_);
+            for (int m = 0; m < nc; m++) {
+                const LspPlan &P = circ_[m].plan;
+                const std::vector<int> &ord = order_of(P);
+                for (int k : ord) all.push_back({(long)level_of(P, k), make_int2(m, k)});
+            }
+            std::stable_sort(all.begin(), all.end(),
+                             [](const auto &a, const auto &b) { return a.first < b.first; });
+            std::vector<int2> out(all.size());
+            for (size_t i = 0; i < all.size(); i++) out[i] = all[i].second;
+            return out;
+        };
+        auto tf = merge([](const LspPlan &P, int k) { return P.level[k]; },
+                        [](const LspPlan &P) -> const std::vector<int> & { return P.ftask; });
+        auto tw = merge([](const LspPlan &P, int k) { return P.lvf[k]; },
+                        [](const LspPlan &P) -> const std::vector<int> & { return P.fwd_task; });
+        auto tb = merge([](const LspPlan &P, int k) { return P.lvb[k]; },
+                        [](const LspPlan &P) -> const std::vector<int> & { return P.bwd_task; });
+        d_ftask_ = upload(tf); d_fwtask_ = upload(tw); d_bwtask_ = upload(tb);
+        if (!d_ftask_ || !d_fwtask_ || !d_bwtask_) return LSP_E_ALLOC;
+    }
+    // persistent grids
+    {
+        int cap = opt_.lds_cap > 0 ? opt_.lds_cap : 1024;
+        opt_.lds_cap = cap;
+        const size_t lds = (size_t)cap * 8 * 4;
+        if (lds > 65536) {
+            rc = check(refactor_set_lds<float>(cap), "lds attr");
+            if (!rc) rc = check(refactor_set_lds<double>(cap), "lds attr");
+            if (!rc) rc = check(refactor_set_lds<__half>(cap), "lds attr");
+            if (rc) return rc;
+        }
+        int per_cu = opt_.blocks_per_cu;
+        if (per_cu <= 0) {
+            const size_t lds_cu = 160 * 1024;
+            per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, lds_cu / lds));
+        }
+        long want = (ntot_ + 3) / 4;
+        fblocks_ = (int)std::max(1L, std::min<long>((long)per_cu * cu_count_, want));
+        tblocks_f_ = (int)std::max(1L, std::min<long>(8L * cu_count_, want));
+        tblocks_b_ = tblocks_f_;
+        int maxcol = 0;
+        for (auto &C : circ_) maxcol = std::max(maxcol, C.plan.max_col);
+        scratch_stride_ = maxcol > cap ? maxcol : 1;
+        d_scratch_ = (double *)dalloc((size_t)fblocks_ * 4 * scratch_stride_ * sizeof(double));
+        if (!d_scratch_) return LSP_E_ALLOC;
+    }
+    // vectors + Krylov state
+    {
+        const int mm = std::max(1, opt_.ir_inner);
+        d_b_ = (double *)dalloc(ntot_ * 8); d_x_ = (double *)dalloc(ntot_ * 8);
+        d_r_ = (double *)dalloc(ntot_ * 8); d_w_ = (double *)dalloc(ntot_ * 8);
+        d_z_ = (double *)dalloc(ntot_ * 8); d_t_ = (double *)dalloc(ntot_ * 8);
+        d_ty_ = (double *)dalloc(ntot_ * 8); d_tz_ = (double *)dalloc(ntot_ * 8);
+        d_V_ = (double *)dalloc((size_t)(mm + 1) * ntot_ * 8);
+        std::vector<long> voff(nc);
+        std::vector<int4> chunks;
+        std::vector<int> cbeg(nc + 1, 0);
+        int cr = opt_.chunk_rows > 0 ? opt_.chunk_rows : 512;
+        for (int m = 0; m < nc; m++) {
+            voff[m] = circ_[m].voff;
+            cbeg[m] = (int)chunks.size();
+            for (int r0 = 0; r0 < circ_[m].n; r0 += cr)
+                chunks.push_back(make_int4(m, r0, std::min(circ_[m].n, r0 + cr), 0));
+        }
+        cbeg[nc] = (int)chunks.size();
+        nchunks_ = (int)chunks.size();
+        d_voff_ = upload(voff);
+        d_chunks_ = upload(chunks);
+        d_cbeg_ = upload(cbeg);
+        d_partial_ = (double *)dalloc((size_t)nchunks_ * (mm + 1) * 8);
+        d_H_ = (double *)dalloc((size_t)nc * (mm + 1) * mm * 8);
+        d_cs_ = (double *)dalloc((size_t)nc * mm * 8);
+        d_sn_ = (double *)dalloc((size_t)nc * mm * 8);
+        d_g_ = (double *)dalloc((size_t)nc * (mm + 1) * 8);
+        d_yv_ = (double *)dalloc((size_t)nc * mm * 8);
+        d_beta_ = (double *)dalloc(nc * 8); d_nb_ = (double *)dalloc(nc * 8);
+        d_hn_ = (double *)dalloc(nc * 8);
+        d_hcol_ = (double *)dalloc((size_t)nc * (mm + 1) * 8);
+        d_act_ = (int *)dalloc(nc * 4); d_jfin_ = (int *)dalloc(nc * 4);
+        d_conv_ = (int *)dalloc(nc * 4); d_cycled_ = (int *)dalloc(nc * 4);
+        d_nact_ = (int *)dalloc(16); d_subset_ = (int *)dalloc(nc * 4);
+        if (!d_b_ || !d_x_ || !d_r_ || !d_w_ || !d_z_ || !d_t_ || !d_ty_ || !d_tz_ || !d_V_ ||
+            !d_voff_ || !d_chunks_ || !d_cbeg_ || !d_partial_ || !d_H_ || !d_act_)
+            return LSP_E_ALLOC;
+        if (hipHostMalloc((void **)&h_pin_, 4096, hipHostMallocDefault) != hipSuccess)
+            return LSP_E_ALLOC;
+    }
+    for (auto &e : ev_) {
+        rc = check(hipEventCreate(&e), "event");
+        if (rc) return rc;
+    }
+    // initial SELL values
+    rc = check(launch_gather_cast<double>(nsell_tot_, d_ssrc_, d_ain_, d_sval64_, stream_), "gather");
+    if (!rc) rc = check(launch_gather_cast<float>(nsell_tot_, d_ssrc_, d_ain_, d_sval32_, stream_), "gather");
+    if (!rc) rc = check(hipStreamSynchronize(stream_), "sync");
+    return rc;
+}
+
+int Batch::set_stream(hipStream_t s)
+{
+    if (!s) return LSP_E_ARG;
+    if (own_stream_ && stream_) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipStreamDestroy(stream_);
+    }
+    own_stream_ = false;
+    stream_ = s;
+    return LSP_OK;
+}
+
+int Batch::set_values(const double *vals, bool on_device)
+{
+    if (!vals) return LSP_E_ARG;
+    int rc = check(hipMemcpyAsync(d_ain_, vals, nnz_tot_ * sizeof(double),
+                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                  stream_), "set values");
+    if (rc) return rc;
+    rc = check(launch_gather_cast<double>(nsell_tot_, d_ssrc_, d_ain_, d_sval64_, stream_), "gather");
+    if (!rc) rc = check(launch_gather_cast<float>(nsell_tot_, d_ssrc_, d_ain_, d_sval32_, stream_), "gather");
+    return rc;
+}
+
+KrylovArgs Batch::kargs() const
+{
+    KrylovArgs K{};
+    K.nmat = (int)circ_.size();
+    K.mmax = std::max(1, opt_.ir_inner);
+    K.ntot = ntot_;
+    K.voff = d_voff_;
+    K.chunks = d_chunks_;
+    K.nchunks = nchunks_;
+    K.cbeg = d_cbeg_;
+    K.partial = d_partial_;
+    K.H = d_H_; K.cs = d_cs_; K.sn = d_sn_; K.g = d_g_; K.yv = d_yv_;
+    K.beta = d_beta_; K.nb = d_nb_; K.hn = d_hn_; K.hcol = d_hcol_;
+    K.act = d_act_; K.jfin = d_jfin_; K.conv = d_conv_; K.cycled = d_cycled_;
+    K.inner_tol = opt_.ir_inner_tol;
+    K.outer_tol = opt_.ir_outer_tol;
+    return K;
+}
+
+// ---------------------------------------------------------------------------
+template <typename T>
+int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
+{
+    int rc = ensure_factor_set(fs);
+    if (rc) return rc;
+    FactorLaunch L{};
+    L.mats = fs.fdesc;
+    L.task = d_ftask_;
+    L.active = d_active;
+    L.ntask = (int)ntot_;
+    L.epoch = next_epoch();
+    L.ctr = d_ctr_;
+    L.abort_word = d_abort_;
+    L.scratch = d_scratch_;
+    L.scratch_stride = scratch_stride_;
+    L.cap = opt_.lds_cap;
+    L.total_waves = fblocks_ * 4;
+    L.ptol = opt_.refactor_ptol;
+    rc = check(launch_reset_mstat(d_mstat_, (int)circ_.size(), d_abort_, stream_), "reset");
+    if (rc) return rc;
+    (void)hipEventRecord(ev_[0], stream_);
+    rc = check(launch_refactor<T>(L, fblocks_, stream_), "refactor launch");
+    (void)hipEventRecord(ev_[1], stream_);
+    return rc;
+}
+
+int Batch::refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *subset)
+{
+    const int *d_active = nullptr;
+    int rc;
+    if (subset) {
+        rc = check(hipMemcpyAsync(d_subset_, subset->data(), circ_.size() * sizeof(int),
+                                  hipMemcpyHostToDevice, stream_), "subset");
+        if (rc) return rc;
+        rc = check(hipStreamSynchronize(stream_), "sync");
+        if (rc) return rc;
+        d_active = d_subset_;
+    }
+    if (prec == LSP_PREC_FP32) rc = refactor_t(f32_, d_active);
+    else if (prec == LSP_PREC_FP64) rc = refactor_t(f64_, d_active);
+    else if (prec == LSP_PREC_FP16) rc = refactor_t(f16_, d_active);
+    else return LSP_E_ARG;
+    if (rc) return rc;
+    cur_prec_ = prec;
+    const int nc = (int)circ_.size();
+    std::vector<int> ms(4 * nc);
+    rc = check(hipMemcpyAsync(ms.data(), d_mstat_, ms.size() * 4, hipMemcpyDeviceToHost, stream_), "mstat");
+    if (!rc) rc = check(hipMemcpyAsync(h_pin_, d_abort_, 4, hipMemcpyDeviceToHost, stream_), "abort");
+    if (!rc) rc = check(hipStreamSynchronize(stream_), "refactor sync");
+    if (rc) return rc;
+    (void)hipEventElapsedTime(&t_refactor_, ev_[0], ev_[1]);
+    stats_.refactor_ms += t_refactor_;
+    stats_.refactor_count++;
+    const int abort_code = h_pin_[0];
+    lsp_refactor_report r{};
+    r.first_bad_circuit = -1;
+    r.ms = t_refactor_;
+    for (int m = 0; m < nc; m++) {
+        if (subset && !(*subset)[m]) continue;
+        r.bad_pivots += ms[4 * m];
+        if (ms[4 * m + 2] != 0x7fffffff) r.singular_circuits++;
+        if ((ms[4 * m] || ms[4 * m + 2] != 0x7fffffff || ms[4 * m + 3]) && r.first_bad_circuit < 0)
+            r.first_bad_circuit = m;
+    }
+    r.status = abort_code == kErrSpin ? LSP_E_DEVICE
+             : abort_code == kErrCapacity ? LSP_E_CAPACITY
+             : r.singular_circuits ? LSP_E_SINGULAR : LSP_OK;
+    if (rep) *rep = r;
+    return r.status;
+}
+
+// ---------------------------------------------------------------------------
+template <typename T>
+int Batch::trisolve_t(FactorSet<T> &fs, const double *d_b, double *d_x, const int *d_active)
+{
+    if (!fs.ready) return LSP_E_ARG;
+    TriLaunch Lf{};
+    Lf.mats = fs.tdesc;
+    Lf.task = d_fwtask_;
+    Lf.voff = d_voff_;
+    Lf.active = d_active;
+    Lf.b = d_b;
+    Lf.x = d_x;
+    Lf.y = d_ty_;
+    Lf.z = d_tz_;
+    Lf.ntask = (int)ntot_;
+    Lf.epoch = next_epoch();
+    Lf.ctr = d_ctr_ + 2;
+    Lf.abort_word = d_abort_ + 1;
+    Lf.total_waves = tblocks_f_ * 4;
+    TriLaunch Lb = Lf;
+    Lb.task = d_bwtask_;
+    Lb.epoch = next_epoch();
+    Lb.ctr = d_ctr_ + 4;
+    Lb.total_waves = tblocks_b_ * 4;
+    (void)hipEventRecord(ev_[2], stream_);
+    int rc = check(launch_trisolve_half<T>(Lf, true, tblocks_f_, stream_), "trisolve fwd");
+    (void)hipEventRecord(ev_[4], stream_);
+    if (!rc) rc = check(launch_trisolve_half<T>(Lb, false, tblocks_b_, stream_), "trisolve bwd");
+    (void)hipEventRecord(ev_[3], stream_);
+    tri_pending_ = true;
+    return rc;
+}
+
+int Batch::launch_trisolve_prec(int prec, const double *d_b, double *d_x, const int *active)
+{
+    if (prec == LSP_PREC_FP32) return trisolve_t(f32_, d_b, d_x, active);
+    if (prec == LSP_PREC_FP64) return trisolve_t(f64_, d_b, d_x, active);
+    if (prec == LSP_PREC_FP16) return trisolve_t(f16_, d_b, d_x, active);
+    return LSP_E_ARG;
+}
+
+int Batch::launch_spmv64(const double *d_xv, double *d_y, const double *d_bv, const int *active)
+{
+    return check(launch_sell_spmv<double>(d_sdesc64_, d_stask_, nstask_, (const int64_t *)d_voff_,
+                                          d_xv, d_y, d_bv, active, stream_), "spmv");
+}
+
+int Batch::trisolve(const double *b, double *x, bool on_device)
+{
+    if (cur_prec_ < 0) return LSP_E_ARG;
+    const double *db = b;
+    double *dx = x;
+    int rc;
+    if (!on_device) {
+        rc = check(hipMemcpyAsync(d_b_, b, ntot_ * 8, hipMemcpyHostToDevice, stream_), "b");
+        if (rc) return rc;
+        db = d_b_;
+        dx = d_x_;
+    }
+    rc = check(hipMemsetAsync(d_abort_ + 1, 0, 4, stream_), "abort reset");
+    if (rc) return rc;
+    rc = launch_trisolve_prec(cur_prec_, db, dx, nullptr);
+    if (rc) return rc;
+    if (!on_device) {
+        rc = check(hipMemcpyAsync(x, d_x_, ntot_ * 8, hipMemcpyDeviceToHost, stream_), "x");
+        if (rc) return rc;
+    }
+    rc = check(hipMemcpyAsync(h_pin_, d_abort_ + 1, 4, hipMemcpyDeviceToHost, stream_), "abort");
+    if (!rc) rc = check(hipStreamSynchronize(stream_), "sync");
+    if (rc) return rc;
+    account_trisolve();
+    return h_pin_[0] ? LSP_E_DEVICE : LSP_OK;
+}
+
+int Batch::spmv(const double *x, double *y, bool on_device, int prec)
+{
+    const double *dx = x;
+    double *dy = y;
+    int rc;
+    if (!on_device) {
+        rc = check(hipMemcpyAsync(d_t_, x, ntot_ * 8, hipMemcpyHostToDevice, stream_), "x");
+        if (rc) return rc;
+        dx = d_t_;
+        dy = d_w_;
+    }
+    if (prec == LSP_PREC_FP32)
+        rc = check(launch_sell_spmv<float>(d_sdesc32_, d_stask_, nstask_, (const int64_t *)d_voff_,
+                                           dx, dy, nullptr, nullptr, stream_), "spmv32");
+    else
+        rc = launch_spmv64(dx, dy, nullptr, nullptr);
+    if (rc) return rc;
+    if (!on_device) {
+        rc = check(hipMemcpyAsync(y, d_w_, ntot_ * 8, hipMemcpyDeviceToHost, stream_), "y");
+        if (rc) return rc;
+    }
+    return check(hipStreamSynchronize(stream_), "sync");
+}
+
+// ---------------------------------------------------------------------------
+int Batch::gmres_ir(const double *d_bv, double *d_xv, int prec, const std::vector<char> *subset,
+                    lsp_solve_report *rep)
+{
+    const KrylovArgs K = kargs();
+    const int nc = (int)circ_.size();
+    const int mm = K.mmax;
+    hipStream_t s = stream_;
+    int rc;
+#define LSP_TRY(expr) do { rc = (expr); if (rc) return rc; } while (0)
+#define LSP_HIP(expr, what) LSP_TRY(check((expr), what))
+    // conv = 1 for circuits outside the subset, 0 inside; x = 0 inside
+    {
+        std::vector<int> cv(nc, 0);
+        if (subset) for (int m = 0; m < nc; m++) cv[m] = (*subset)[m] ? 0 : 1;
+        LSP_HIP(hipMemcpyAsync(d_conv_, cv.data(), nc * 4, hipMemcpyHostToDevice, s), "conv");
+        LSP_HIP(hipStreamSynchronize(s), "sync");
+    }
+    LSP_HIP(k_activate(K, s), "activate");
+    LSP_HIP(k_chunk_dots(K, d_bv, d_bv, 1, s), "dots");
+    LSP_HIP(k_reduce(K, 1, 2, 0, d_nb_, s), "reduce");
+    LSP_HIP(k_chunk_fill(K, d_xv, 0.0, s), "zero x");
+    int outer_used = 0, inner_total = 0;
+    for (int outer = 0; outer < opt_.ir_outer; outer++) {
+        LSP_HIP(k_activate(K, s), "activate");
+        LSP_TRY(launch_spmv64(d_xv, d_r_, d_bv, d_act_));
+        LSP_HIP(k_chunk_dots(K, d_r_, d_r_, 1, s), "dots");
+        LSP_HIP(k_reduce(K, 1, 2, 0, d_beta_, s), "reduce");
+        LSP_HIP(k_zero_int(d_nact_, 1, s), "zero");
+        LSP_HIP(k_start_cycle(K, d_nact_, s), "start");
+        LSP_TRY(sync_read_int(d_nact_, h_pin_, 1));
+        if (h_pin_[0] == 0) break;
+        outer_used++;
+        LSP_HIP(k_chunk_scale(K, d_r_, d_V_, d_beta_, s), "scale");
+        for (int j = 0; j < mm; j++) {
+            double *Vj = d_V_ + (long)j * ntot_;
+            LSP_TRY(launch_trisolve_prec(prec, Vj, d_z_, d_act_));
+            LSP_TRY(launch_spmv64(d_z_, d_w_, nullptr, d_act_));
+            for (int pass = 0; pass < 2; pass++) {
+                LSP_HIP(k_chunk_dots(K, d_V_, d_w_, j + 1, s), "dots");
+                LSP_HIP(k_reduce(K, j + 1, 1, j, nullptr, s), "reduce");
+                LSP_HIP(k_chunk_update(K, d_V_, d_w_, j + 1, s), "update");
+            }
+            LSP_HIP(k_chunk_dots(K, d_w_, d_w_, 1, s), "dots");
+            LSP_HIP(k_reduce(K, 1, 2, 0, d_hn_, s), "reduce");
+            LSP_HIP(k_chunk_scale(K, d_w_, d_V_ + (long)(j + 1) * ntot_, d_hn_, s), "scale");
+            LSP_HIP(k_zero_int(d_nact_, 1, s), "zero");
+            LSP_HIP(k_givens(K, j, d_nact_, s), "givens");
+            inner_total++;
+            LSP_TRY(sync_read_int(d_nact_, h_pin_, 1));
+            if (h_pin_[0] == 0) break;
+        }
+        LSP_HIP(k_backsolve(K, s), "backsolve");
+        LSP_HIP(k_chunk_combine(K, d_V_, d_t_, s), "combine");
+        LSP_TRY(launch_trisolve_prec(prec, d_t_, d_z_, d_act_));
+        LSP_HIP(k_chunk_axpy(K, d_xv, d_z_, s), "axpy");
+    }
+    // final true residual of every circuit in the subset
+    {
+        std::vector<int> am(nc, 1);
+        if (subset) for (int m = 0; m < nc; m++) am[m] = (*subset)[m] ? 1 : 0;
+        LSP_HIP(hipMemcpyAsync(d_act_, am.data(), nc * 4, hipMemcpyHostToDevice, s), "act");
+        LSP_TRY(launch_spmv64(d_xv, d_r_, d_bv, d_act_));
+        LSP_HIP(k_chunk_dots(K, d_r_, d_r_, 1, s), "dots");
+        LSP_HIP(k_reduce(K, 1, 2, 0, d_beta_, s), "reduce");
+        std::vector<double> beta(nc), nb(nc);
+        LSP_HIP(hipMemcpyAsync(beta.data(), d_beta_, nc * 8, hipMemcpyDeviceToHost, s), "beta");
+        LSP_HIP(hipMemcpyAsync(nb.data(), d_nb_, nc * 8, hipMemcpyDeviceToHost, s), "nb");
+        LSP_HIP(hipMemcpyAsync(h_pin_, d_abort_ + 1, 4, hipMemcpyDeviceToHost, s), "abort");
+        LSP_HIP(hipStreamSynchronize(s), "sync");
+        account_trisolve();
+        if (h_pin_[0]) return LSP_E_DEVICE;
+        lsp_solve_report r{};
+        r.outer = outer_used;
+        r.inner_total = inner_total;
+        for (int m = 0; m < nc; m++) {
+            if (subset && !(*subset)[m]) continue;
+            double rel = nb[m] > 0 ? beta[m] / nb[m] : beta[m];
+            if (!(rel == rel)) rel = INFINITY;
+            r.max_rel_res = std::max(r.max_rel_res, rel);
+            if (rel <= opt_.ir_outer_tol * 10 || rel <= 1e-9) r.converged++;
+        }
+        r.status = LSP_OK;
+        if (rep) *rep = r;
+    }
+#undef LSP_TRY
+#undef LSP_HIP
+    return LSP_OK;
+}
+
+int Batch::solve(const double *b, double *x, bool on_device, lsp_solve_report *rep)
+{
+    if (cur_prec_ < 0) return LSP_E_ARG;
+    const double *db = b;
+    double *dx = x;
+    int rc;
+    if (!on_device) {
+        rc = check(hipMemcpyAsync(d_b_, b, ntot_ * 8, hipMemcpyHostToDevice, stream_), "b");
+        if (rc) return rc;
+        db = d_b_;
+        dx = d_x_;
+    }
+    rc = check(hipMemsetAsync(d_abort_ + 1, 0, 4, stream_), "abort reset");
+    if (rc) return rc;
+    lsp_solve_report r{};
+    rc = gmres_ir(db, dx, cur_prec_, nullptr, &r);
+    if (rc) return rc;
+    // per-circuit fallback to fp64 factors when IR stalls
+    if (opt_.fallback_fp64 && cur_prec_ != LSP_PREC_FP64 && r.converged < (int)circ_.size()) {
+        // identify stalled circuits from the final residuals
+        const int nc = (int)circ_.size();
+        std::vector<double> beta(nc), nb(nc);
+        (void)hipMemcpy(beta.data(), d_beta_, nc * 8, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(nb.data(), d_nb_, nc * 8, hipMemcpyDeviceToHost);
+        std::vector<int> sub(nc, 0);
+        std::vector<char> subc(nc, 0);
+        int cnt = 0;
+        for (int m = 0; m < nc; m++) {
+            double rel = nb[m] > 0 ? beta[m] / nb[m] : beta[m];
+            if (!(rel <= opt_.ir_outer_tol * 10 || rel <= 1e-9)) { sub[m] = 1; subc[m] = 1; cnt++; }
+        }
+        if (cnt) {
+            const int keep = cur_prec_;
+            lsp_refactor_report rr{};
+            rc = refactor(LSP_PREC_FP64, &rr, &sub);
+            cur_prec_ = keep;
+            if (rc && rc != LSP_E_SINGULAR) return rc;
+            rc = check(hipMemsetAsync(d_abort_ + 1, 0, 4, stream_), "abort reset");
+            if (rc) return rc;
+            lsp_solve_report r2{};
+            rc = gmres_ir(db, dx, LSP_PREC_FP64, &subc, &r2);
+            if (rc) return rc;
+            r.fallback = cnt;
+            r.converged = r.converged + r2.converged;
+            r.outer = std::max(r.outer, r2.outer);
+            r.inner_total += r2.inner_total;
+            // recompute the max residual over all circuits
+            std::vector<double> b2(nc);
+            (void)hipMemcpy(b2.data(), d_beta_, nc * 8, hipMemcpyDeviceToHost);
+            r.max_rel_res = 0;
+            for (int m = 0; m < nc; m++) {
+                double rel = sub[m] ? (nb[m] > 0 ? b2[m] / nb[m] : b2[m])
+                                    : (nb[m] > 0 ? beta[m] / nb[m] : beta[m]);
+                r.max_rel_res = std::max(r.max_rel_res, rel);
+            }
+        }
+    }
+    if (!on_device) {
+        rc = check(hipMemcpyAsync(x, d_x_, ntot_ * 8, hipMemcpyDeviceToHost, stream_), "x");
+        if (!rc) rc = check(hipStreamSynchronize(stream_), "sync");
+        if (rc) return rc;
+    }
+    r.status = r.converged == (int)circ_.size() ? LSP_OK : LSP_E_NOT_CONVERGED;
+    if (rep) *rep = r;
+    return r.status;
+}
+
+// ---------------------------------------------------------------------------
+int Batch::info(int c, lsp_circuit_info *inf) const
+{
+    if (c < 0 || c >= (int)circ_.size() || !inf) return LSP_E_ARG;
+    const CircuitHost &C = circ_[c];
+    inf->n = C.n;
+    inf->nnz = C.nnz;
+    inf->lnnz = C.sym.lnnz;
+    inf->unnz = C.sym.unnz;
+    inf->n_updates = C.plan.n_updates;
+    inf->nlevels = C.plan.nlevels;
+    inf->nlv_fwd = C.plan.nlv_f;
+    inf->nlv_bwd = C.plan.nlv_b;
+    inf->max_col = C.plan.max_col;
+    inf->sell_slots = C.sell.slots;
+    return LSP_OK;
+}
+
+int Batch::export_factors(int c, int *pinv, int *Lp, int *Li, double *Lx, int *Up, int *Ui,
+                          double *Ux)
+{
+    if (c < 0 || c >= (int)circ_.size() || cur_prec_ < 0) return LSP_E_ARG;
+    const CircuitHost &C = circ_[c];
+    const LspPlan &P = C.plan;
+    const long ns = P.cp[C.n];
+    std::vector<double> v(ns);
+    int rc;
+    if (cur_prec_ == LSP_PREC_FP64) {
+        rc = check(hipMemcpy(v.data(), f64_.val + C.o_slot, ns * 8, hipMemcpyDeviceToHost), "export");
+    } else if (cur_prec_ == LSP_PREC_FP32) {
+        std::vector<float> f(ns);
+        rc = check(hipMemcpy(f.data(), f32_.val + C.o_slot, ns * 4, hipMemcpyDeviceToHost), "export");
+        for (long i = 0; i < ns; i++) v[i] = f[i];
+    } else {
+        std::vector<__half> f(ns);
+        rc = check(hipMemcpy(f.data(), f16_.val + C.o_slot, ns * 2, hipMemcpyDeviceToHost), "export");
+        for (long i = 0; i < ns; i++) v[i] = (double)__half2float(f[i]);
+    }
+    if (rc) return rc;
+    if (pinv) std::copy(P.pinv.begin(), P.pinv.end(), pinv);
+    long l = 0, u = 0;
+    for (int k = 0; k < C.n; k++) {
+        if (Lp) Lp[k] = (int)l;
+        if (Up) Up[k] = (int)u;
+        const long c0 = P.cp[k], d = P.dp[k], c1 = P.cp[k + 1];
+        for (long s = c0; s < d; s++) {
+            if (Ui) Ui[u] = P.pos[s];
+            if (Ux) Ux[u] = v[s];
+            u++;
+        }
+        if (Ui) Ui[u] = k;
+        if (Ux) Ux[u] = v[d];
+        u++;
+        if (Li) Li[l] = k;
+        if (Lx) Lx[l] = 1.0;
+        l++;
+        for (long s = d + 1; s < c1; s++) {
+            if (Li) Li[l] = P.pos[s];
+            if (Lx) Lx[l] = v[s];
+            l++;
+        }
+    }
+    if (Lp) Lp[C.n] = (int)l;
+    if (Up) Up[C.n] = (int)u;
+    return LSP_OK;
+}
+
+int Batch::device_vectors(double **b, double **x)
+{
+    if (b) *b = d_b_;
+    if (x) *x = d_x_;
+    return LSP_OK;
+}
+
+int Batch::kernel_stats(lsp_kernel_stats *st, int reset)
+{
+    if (st) *st = stats_;
+    if (reset) stats_ = lsp_kernel_stats{};
+    return LSP_OK;
+}
+
+int Batch::last_times(double *rf, double *fw, double *bw) const
+{
+    if (rf) *rf = t_refactor_;
+    if (fw) *fw = t_fwd_;
+    if (bw) *bw = t_bwd_;
+    return LSP_OK;
+}
+
+template int Batch::ensure_factor_set<float>(FactorSet<float> &);
+template int Batch::ensure_factor_set<double>(FactorSet<double> &);
+template int Batch::ensure_factor_set<__half>(FactorSet<__half> &);
+
+}  // namespace lsp

@@ -1,0 +1,148 @@
+// batch.hpp -- device-resident batch of circuits (one circuit = batch of 1).
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../host/host_internal.h"
+#include "../kernels/launch.hpp"
+#include "luspice.h"
+
+namespace lsp {
+
+// Typed device allocation owned by a Batch.
+struct DevMem {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct CircuitHost {
+    int n = 0;
+    long nnz = 0;
+    std::vector<int> row, col;           // COO pattern (0-based)
+    std::vector<double> val0;            // values used for the symbolic phase
+    std::vector<int> q;
+    LspGpLu sym;                         // CPU symbolic factors (fp64)
+    LspPlan plan;
+    LspSell sell;
+    long voff = 0, aoff = 0;             // vector / value arena offsets
+    // offsets into the concatenated device arrays
+    long o_np1 = 0;                      // n+1 sized arrays
+    long o_n = 0;                        // n sized arrays
+    long o_slot = 0;                     // slot arrays
+    long o_a = 0;                    // A-entry arrays
+    long o_rel = 0;                      // update map
+    long o_l = 0, o_u = 0;               // strict L / U entry arrays
+    long o_sl = 0, o_slp1 = 0, o_ss = 0; // SELL slices, slices+1, slots
+};
+
+template <typename T>
+struct FactorSet {                       // per storage precision
+    T *val = nullptr, *lrow = nullptr, *urow = nullptr, *diag = nullptr;
+    FactorDesc<T> *fdesc = nullptr;      // device
+    TriDesc<T> *tdesc = nullptr;         // device
+    bool ready = false;                  // arrays allocated
+};
+
+class Batch {
+public:
+    ~Batch();
+    int create(const lsp_circuit_in *c, int nc, const lsp_options *opt);
+    int set_stream(hipStream_t s);
+    int set_values(const double *vals, bool on_device);
+    int refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *subset = nullptr);
+    int trisolve(const double *b, double *x, bool on_device);
+    int solve(const double *b, double *x, bool on_device, lsp_solve_report *rep);
+    int spmv(const double *x, double *y, bool on_device, int prec);
+    int info(int c, lsp_circuit_info *inf) const;
+    int export_factors(int c, int *pinv, int *Lp, int *Li, double *Lx, int *Up, int *Ui,
+                       double *Ux);
+    int device_vectors(double **b, double **x);
+    int last_times(double *rf, double *fw, double *bw) const;
+    int kernel_stats(lsp_kernel_stats *st, int reset);
+    long total_n() const { return ntot_; }
+    long total_nnz() const { return nnz_tot_; }
+    int ncircuits() const { return (int)circ_.size(); }
+
+    // device-resident pieces used by the serving loop and the bench
+    int gmres_ir(const double *d_b, double *d_x, int prec, const std::vector<char> *subset,
+                 lsp_solve_report *rep);
+    int launch_trisolve_prec(int prec, const double *d_b, double *d_x, const int *active);
+    int launch_spmv64(const double *d_x, double *d_y, const double *d_b, const int *active);
+    hipStream_t stream() const { return stream_; }
+    int cur_prec() const { return cur_prec_; }
+    std::string last_error;
+
+private:
+    template <typename T> int ensure_factor_set(FactorSet<T> &fs);
+    template <typename T> int refactor_t(FactorSet<T> &fs, const int *d_active);
+    template <typename T> int trisolve_t(FactorSet<T> &fs, const double *d_b, double *d_x,
+                                         const int *d_active);
+    void *dalloc(size_t bytes);
+    template <typename X> X *upload(const std::vector<X> &v);
+    int check(hipError_t e, const char *what);
+    int sync_read_int(const int *d, int *h, int count);
+    int next_epoch();
+    KrylovArgs kargs() const;
+    void account_trisolve();
+
+    lsp_options opt_{};
+    hipStream_t stream_ = nullptr;
+    bool own_stream_ = false;
+    std::vector<CircuitHost> circ_;
+    std::vector<DevMem> mem_;
+    long ntot_ = 0, nnz_tot_ = 0;
+    int cu_count_ = 0;
+    int cur_prec_ = -1;
+
+    // structure (device)
+    int64_t *d_cp_ = nullptr, *d_relp_ = nullptr, *d_soff_ = nullptr;
+    int *d_dp_ = nullptr, *d_pos_ = nullptr, *d_aptr_ = nullptr, *d_asrc_ = nullptr,
+        *d_aslot_ = nullptr, *d_dst_ = nullptr, *d_lrp_ = nullptr, *d_lci_ = nullptr,
+        *d_urp_ = nullptr, *d_uci_ = nullptr, *d_rowat_ = nullptr, *d_q_ = nullptr,
+        *d_done_ = nullptr, *d_fdone_ = nullptr, *d_bdone_ = nullptr, *d_srlen_ = nullptr,
+        *d_scol_ = nullptr, *d_sperm_ = nullptr, *d_ssrc_ = nullptr;
+    uint16_t *d_rel_ = nullptr;
+    double *d_ain_ = nullptr, *d_sval64_ = nullptr;
+    float *d_sval32_ = nullptr;
+    SellDesc<double> *d_sdesc64_ = nullptr;
+    SellDesc<float> *d_sdesc32_ = nullptr;
+    int2 *d_ftask_ = nullptr, *d_fwtask_ = nullptr, *d_bwtask_ = nullptr, *d_stask_ = nullptr;
+    long nslot_tot_ = 0, nl_tot_ = 0, nu_tot_ = 0, nsell_tot_ = 0;
+    int nstask_ = 0;
+    int *d_ctr_ = nullptr;               // [0..1] factor, [2..3] fwd, [4..5] bwd
+    int *d_abort_ = nullptr;             // [0] factor, [1] trisolve
+    int *d_mstat_ = nullptr;             // 4 per circuit
+    double *d_scratch_ = nullptr;
+    long scratch_stride_ = 0;
+    int fblocks_ = 0, tblocks_f_ = 0, tblocks_b_ = 0;
+    FactorSet<float> f32_;
+    FactorSet<double> f64_;
+    FactorSet<__half> f16_;
+    // vectors and Krylov state
+    double *d_b_ = nullptr, *d_x_ = nullptr, *d_r_ = nullptr, *d_w_ = nullptr,
+           *d_z_ = nullptr, *d_t_ = nullptr, *d_ty_ = nullptr, *d_tz_ = nullptr, *d_V_ = nullptr;
+    long *d_voff_ = nullptr;
+    int4 *d_chunks_ = nullptr;
+    int *d_cbeg_ = nullptr;
+    int nchunks_ = 0;
+    double *d_partial_ = nullptr, *d_H_ = nullptr, *d_cs_ = nullptr, *d_sn_ = nullptr,
+           *d_g_ = nullptr, *d_yv_ = nullptr, *d_beta_ = nullptr, *d_nb_ = nullptr,
+           *d_hn_ = nullptr, *d_hcol_ = nullptr;
+    int *d_act_ = nullptr, *d_jfin_ = nullptr, *d_conv_ = nullptr, *d_cycled_ = nullptr,
+        *d_nact_ = nullptr, *d_subset_ = nullptr;
+    int *h_pin_ = nullptr;               // pinned host words for flag reads
+    hipEvent_t ev_[6] = {};
+    float t_refactor_ = 0, t_fwd_ = 0, t_bwd_ = 0;
+    bool tri_pending_ = false;
+    lsp_kernel_stats stats_{};
+};
+
+}  // namespace lsp
+
+struct lsp_batch {
+    lsp::Batch impl;
+};

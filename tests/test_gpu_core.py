@@ -1,0 +1,102 @@
+"""Parity of the HIP path against the CPU oracle (config 0 and small batches).
+
+Tolerances: integer structure (P, pattern) bit-exact; fp64 factors within
+1e-12 relative (summation order differs from the DFS order); fp32 factor
+storage within fp32 rounding of the fp64 factors; SpMV with fp64 values
+bitwise (same per-row order, contraction off); GMRES-IR solutions within
+1e-6 relative L2 of the fp64 oracle x (the north-star pass line).
+"""
+import numpy as np
+import pytest
+
+import luspice_amd as L
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_x(c, q):
+    rc, x, ln, un = O.cpu_lu_ref(c.n, c.row, c.col, c.val, q, c.rhs)
+    assert rc == -1
+    return x
+
+
+def test_refactor_fp64_matches_oracle_factors(c0, c0_q):
+    print("case: c0 refactor fp64", flush=True)
+    B = L.Batch(L.Circuit(c0.n, c0.row, c0.col, c0.val, c0.rhs, q=c0_q))
+    rep = B.refactor("fp64")
+    assert rep["rc"] == 0, rep
+    F = B.export_factors(0)
+    R = O.lu_factor(c0.n, c0.row, c0.col, c0.val, c0_q)
+    assert np.array_equal(F["pinv"], R["pinv"])
+    assert np.array_equal(F["Lp"], R["Lp"]) and np.array_equal(F["Up"], R["Up"])
+    n = c0.n
+    for k in range(n):
+        a, b = slice(R["Lp"][k], R["Lp"][k + 1]), slice(R["Up"][k], R["Up"][k + 1])
+        oi, oj = np.argsort(R["Li"][a], kind="stable"), np.argsort(R["Ui"][b], kind="stable")
+        assert np.array_equal(np.sort(R["Li"][a]), F["Li"][a][np.argsort(F["Li"][a])])
+        assert np.array_equal(np.sort(R["Ui"][b]), F["Ui"][b][np.argsort(F["Ui"][b])])
+        fl = F["Lx"][a][np.argsort(F["Li"][a], kind="stable")]
+        fu = F["Ux"][b][np.argsort(F["Ui"][b], kind="stable")]
+        np.testing.assert_allclose(fl, R["Lx"][a][oi], rtol=1e-10, atol=1e-12 * np.abs(R["Lx"][a]).max())
+        np.testing.assert_allclose(fu, R["Ux"][b][oj], rtol=1e-10, atol=1e-12 * np.abs(R["Ux"][b]).max())
+
+
+def test_trisolve_fp64_direct(c0, c0_q):
+    print("case: c0 trisolve fp64", flush=True)
+    B = L.Batch(L.Circuit(c0.n, c0.row, c0.col, c0.val, c0.rhs, q=c0_q))
+    assert B.refactor("fp64")["rc"] == 0
+    x = B.trisolve(c0.rhs)
+    xo = _oracle_x(c0, c0_q)
+    assert O.rel_l2(x, xo) < 1e-10
+
+
+def test_spmv_fp64_bitwise(c0):
+    print("case: c0 spmv", flush=True)
+    B = L.Batch(c0)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, c0.n)
+    y = B.spmv(x, "fp64")
+    order = np.lexsort((np.arange(c0.nnz), c0.row))
+    rp = np.zeros(c0.n + 1, np.int32)
+    np.add.at(rp, c0.row + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    yo = O.csr_spmv(c0.n, rp, c0.col[order], c0.val[order], x)
+    assert np.array_equal(y, yo)
+
+
+def test_gmres_ir_fp32_factors_c0(c0, c0_q):
+    print("case: c0 gmres-ir fp32", flush=True)
+    B = L.Batch(L.Circuit(c0.n, c0.row, c0.col, c0.val, c0.rhs, q=c0_q))
+    rep = B.refactor("fp32")
+    assert rep["rc"] == 0, rep
+    x, sr = B.solve(c0.rhs)
+    xo = _oracle_x(c0, c0_q)
+    err = O.rel_l2(x, xo)
+    print("rel err", err, sr, flush=True)
+    assert sr["rc"] == 0 and err <= 1e-6
+    # the reference's own mixed path restated on the CPU (fp32 splu factors)
+    F32 = O.lu_factor(c0.n, c0.row, c0.col, c0.val, c0_q)
+    F32 = dict(F32, Lx=F32["Lx"].astype(np.float32), Ux=F32["Ux"].astype(np.float32))
+    xr, info = O.gmres_ir(c0.n, c0.row, c0.col, c0.val, c0.rhs, F32, c0_q)
+    assert O.rel_l2(x, xr) <= 1e-6
+
+
+def test_batch_uneven_same_alone_and_mixed():
+    print("case: batch of 6 uneven circuits", flush=True)
+    cs = [L.mna("sized", n=n, seed=100 + i) for i, n in enumerate([150, 900, 333, 2100, 77, 1500])]
+    B = L.Batch(cs)
+    assert B.refactor("fp32")["rc"] == 0
+    rhs = np.concatenate([c.rhs for c in cs])
+    x, rep = B.solve(rhs)
+    assert rep["rc"] == 0, rep
+    for i, c in enumerate(cs):
+        xi = x[B.voff[i]:B.voff[i + 1]]
+        q = L.colamd(c)
+        xo = _oracle_x(c, q)
+        assert O.rel_l2(xi, xo) <= 1e-6
+        solo = L.Batch(c)
+        assert solo.refactor("fp32")["rc"] == 0
+        xs, r2 = solo.solve(c.rhs)
+        assert r2["rc"] == 0
+        assert O.rel_l2(xi, xs) <= 1e-6
