@@ -57,6 +57,8 @@ typedef struct {
     int chunk_rows;        /* rows per reduction chunk (0 = auto) */
     int sell_sigma;        /* SELL sort window (0 = whole matrix) */
     int fallback_fp64;     /* 1: stalled IR refactors in fp64 and retries */
+    int refactor_blocks;   /* persistent refactor grid (0 = auto) */
+    int trisolve_blocks;   /* persistent trisolve grid (0 = auto) */
 } lsp_options;
 
 typedef struct {
@@ -129,6 +131,8 @@ int  lsp_batch_device_vectors(lsp_batch *b, double **rhs, double **x);
 int  lsp_batch_last_times(const lsp_batch *b, double *refactor_ms,
                           double *fwd_ms, double *bwd_ms);
 
+/* Text of the last error recorded by the batch ("" if none). */
+const char *lsp_batch_last_error(const lsp_batch *b);
 /* Running per-kernel device times since creation (or the last reset). */
 int  lsp_batch_kernel_stats(lsp_batch *b, lsp_kernel_stats *st, int reset);
 

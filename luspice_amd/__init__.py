@@ -196,6 +196,8 @@ class Batch:
         rc = N.lib.lsp_batch_refactor(self._h, prec_code(prec), N.C.byref(rep))
         out = {k: getattr(rep, k) for k, _ in N.RefactorReport._fields_}
         out["rc"] = rc
+        if rc != 0:
+            out["error"] = N.lib.lsp_batch_last_error(self._h).decode()
         return out
 
     def trisolve(self, rhs: np.ndarray) -> np.ndarray:

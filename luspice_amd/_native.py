@@ -37,7 +37,8 @@ class Options(C.Structure):
                 ("ir_outer", C.c_int), ("ir_inner", C.c_int),
                 ("ir_inner_tol", C.c_double), ("ir_outer_tol", C.c_double),
                 ("lds_cap", C.c_int), ("blocks_per_cu", C.c_int), ("chunk_rows", C.c_int),
-                ("sell_sigma", C.c_int), ("fallback_fp64", C.c_int)]
+                ("sell_sigma", C.c_int), ("fallback_fp64", C.c_int),
+                ("refactor_blocks", C.c_int), ("trisolve_blocks", C.c_int)]
 
 
 class RefactorReport(C.Structure):
@@ -88,6 +89,7 @@ _sig("lsp_batch_export_factors", C.c_int, C.c_void_p, C.c_int, c_int_p, c_int_p,
 _sig("lsp_batch_device_vectors", C.c_int, C.c_void_p, C.POINTER(C.c_void_p),
      C.POINTER(C.c_void_p))
 _sig("lsp_batch_last_times", C.c_int, C.c_void_p, c_double_p, c_double_p, c_double_p)
+_sig("lsp_batch_last_error", C.c_char_p, C.c_void_p)
 _sig("lsp_batch_kernel_stats", C.c_int, C.c_void_p, C.POINTER(KernelStats), C.c_int)
 _sig("lsp_colamd", C.c_int, C.c_int, C.c_long, c_int_p, c_int_p, c_int_p)
 _sig("lsp_symbolic_factor", C.c_int, C.c_int, C.c_long, c_int_p, c_int_p, c_double_p, c_int_p,

@@ -32,6 +32,16 @@ __device__ __forceinline__ int ld_relaxed(const int *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Read-modify-write probes are performed at the device coherence point, so a
+// flag written by a wave on another XCD is seen whatever a local cache holds.
+__device__ __forceinline__ int ld_coherent(int *p)
+{
+    return __hip_atomic_fetch_add(p, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void publish(int *p, int v)
+{
+    __hip_atomic_exchange(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ __forceinline__ void st_release(int *p, int v)
 {
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -49,20 +59,52 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-// Spin until *flag == epoch.  Returns false when the wait was abandoned (spin
-// limit hit here or elsewhere).
-__device__ __forceinline__ bool wait_epoch(const int *flag, int epoch, int *abort_word)
+// Wave-collective wait: every lane whose flag pointer is non-null waits until
+// *flag == epoch; returns true once all of them have seen it, false when the
+// wait was abandoned (round limit hit here, or the abort word raised
+// elsewhere).  Must be called by all lanes of the wave.
+//
+// Polling discipline (measured on MI355X): a lane stops reloading once it has
+// seen its flag; the wave polls with one relaxed agent-scope load instruction
+// per round (an acquire load would also invalidate the cache on every poll);
+// rounds back off exponentially up to ~1 us.  Hundreds of waves polling every
+// few hundred cycles with all lanes saturate the memory path the producers
+// need, and the factorization then crawls.  The caller issues one
+// acquire_fence() after a successful wait.
+constexpr int kRoundLimit = 1 << 21;
+
+__device__ __forceinline__ bool wave_wait(const int *flag, int epoch, int *abort_word,
+                                          int *dbg = nullptr, int who2 = -1)
 {
-    int spins = 0;
-    while (ld_acquire(flag) != epoch) {
-        if (ld_relaxed(abort_word) != 0) return false;
-        __builtin_amdgcn_s_sleep(1);
-        if (++spins > kSpinLimit) {
-            st_relaxed(abort_word, kErrSpin);
+    int seen = flag ? -1 : epoch;
+    int ok = flag == nullptr;
+    int backoff = 0;
+    for (int round = 0;; round++) {
+        if (!ok) {
+            seen = ld_relaxed(flag);
+            ok = seen == epoch;
+        }
+        if (__all(ok)) return true;
+        if ((round & 15) == 15 && ld_relaxed(abort_word) != 0) return false;
+        if (round > kRoundLimit) {
+            if (!ok && atomicCAS(abort_word, 0, kErrSpin) == 0 && dbg) {
+                dbg[1] = seen;
+                dbg[2] = epoch;
+                dbg[3] = round;
+                dbg[4] = who2;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            }
             return false;
         }
+        for (int s = 0; s <= backoff; s++) __builtin_amdgcn_s_sleep(4);
+        if (backoff < 7) backoff = 2 * backoff + 1;
     }
-    return true;
+}
+
+// Makes data published before the observed flags visible to this wave.
+__device__ __forceinline__ void acquire_fence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 // wave64 reductions (DPP / permute lowered by the compiler from shuffles)

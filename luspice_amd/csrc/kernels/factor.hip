@@ -30,9 +30,28 @@
 namespace lsp {
 
 
+__device__ __forceinline__ void trace_state(int *ws, int phase, int a, int b)
+{
+    if (ws && lane_id() == 0) {
+        st_relaxed(ws + 1, phase);
+        st_relaxed(ws + 2, a);
+        st_relaxed(ws + 3, b);
+    }
+}
+
+// ws[4..7]: max release-fence cycles, max acquire-fence cycles, columns
+// finished, max column cycles (all in units of 1024 clocks)
+__device__ __forceinline__ void trace_max(int *ws, int slot, long long cycles)
+{
+    if (ws && lane_id() == 0) {
+        const int v = (int)(cycles >> 10);
+        if (v > ws[slot]) ws[slot] = v;
+    }
+}
+
 template <typename T>
 __device__ void factor_column(const FactorDesc<T> &M, int k, double *x,
-                              int epoch, int *abort_word, double ptol)
+                              int epoch, int *abort_word, double ptol, int *dbg, int *ws)
 {
     const int lane = lane_id();
     const int64_t c0 = M.cp[k];
@@ -51,14 +70,27 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, double *x,
     for (int s0 = 0; s0 < dl; s0 += kWave) {
         const int nd = min(kWave, dl - s0);
         int l0 = 0, len = 0;
-        bool ok = true;
+        const int *flag = nullptr;
+        int jj = -1;
         if (lane < nd) {
             const int j = M.pos[c0 + s0 + lane];
-            ok = wait_epoch(&M.done[j], epoch, abort_word);
+            jj = j;
+            flag = &M.done[j];
             l0 = M.dp[j] + 1;
             len = (int)(M.cp[j + 1] - l0);
         }
-        if (!__all(ok)) return;                    // aborted; kernel drains
+        if (ws) {
+            // record the largest dependency of this chunk that is not done yet
+            int pend = (lane < nd && ld_relaxed(flag) != epoch) ? jj : -1;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) pend = max(pend, __shfl_xor(pend, off, kWave));
+            trace_state(ws, 3, k, pend);
+        }
+        if (!wave_wait(flag, epoch, abort_word, dbg, k)) return;   // aborted; kernel drains
+        trace_state(ws, 4, k, s0);
+        const long long ta = ws ? (long long)clock64() : 0;
+        acquire_fence();
+        if (ws) trace_max(ws, 5, (long long)clock64() - ta);
         for (int d = 0; d < nd; d++) {
             const int jl0 = __builtin_amdgcn_readlane(l0, d);
             const int jlen = __builtin_amdgcn_readlane(len, d);
@@ -108,8 +140,14 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, double *x,
         M.val[c0 + i] = v;
         M.lrow[M.dst[c0 + i]] = v;
     }
+    const long long tr = ws ? (long long)clock64() : 0;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (lane == 0) st_release(&M.done[k], epoch);
+    if (lane == 0) publish(&M.done[k], epoch);
+    if (ws) {
+        trace_max(ws, 4, (long long)clock64() - tr);
+        if (lane == 0) ws[6] += 1;
+    }
+    trace_state(ws, 6, k, 0);
 }
 
 template <typename T>
@@ -122,11 +160,19 @@ __global__ void __launch_bounds__(256) refactor_kernel(FactorLaunch L)
     double *xl = lds + (long)wv * L.cap;
     double *xg = L.scratch + ((long)blockIdx.x * waves_per_block + wv) * L.scratch_stride;
     const FactorDesc<T> *mats = (const FactorDesc<T> *)L.mats;
+    const int gw = blockIdx.x * waves_per_block + wv;
+    int *ws = L.trace ? L.trace + L.ntask + 8 * gw : nullptr;
+    if (ws && lane == 0) { ws[4] = 0; ws[5] = 0; ws[6] = 0; ws[7] = 0; }
+    trace_state(ws, 1, -1, -1);
 
     for (;;) {
         int t = 0;
         if (lane == 0) t = atomicAdd(&L.ctr[0], 1);
         t = __shfl(t, 0, kWave);
+        if (ws && lane == 0) {
+            st_relaxed(ws, t);
+            if (t < L.ntask) st_relaxed(L.trace + t, gw);
+        }
         if (t >= L.ntask) break;
         if (ld_relaxed(L.abort_word) != 0) continue;          // drain
         const int2 tk = L.task[t];
@@ -134,14 +180,17 @@ __global__ void __launch_bounds__(256) refactor_kernel(FactorLaunch L)
         const FactorDesc<T> M = mats[tk.x];
         const int k = tk.y;
         const int m = (int)(M.cp[k + 1] - M.cp[k]);
+        const long long tc = ws ? (long long)clock64() : 0;
         if (m <= L.cap) {
-            factor_column<T>(M, k, xl, L.epoch, L.abort_word, L.ptol);
+            factor_column<T>(M, k, xl, L.epoch, L.abort_word, L.ptol, L.dbg, ws);
         } else if (m <= L.scratch_stride) {
-            factor_column<T>(M, k, xg, L.epoch, L.abort_word, L.ptol);
+            factor_column<T>(M, k, xg, L.epoch, L.abort_word, L.ptol, L.dbg, ws);
         } else {
             if (lane == 0) st_relaxed(L.abort_word, kErrCapacity);
         }
+        if (ws) trace_max(ws, 7, (long long)clock64() - tc);
     }
+    trace_state(ws, 7, -1, -1);
     // the last wave out re-arms the queue for the next launch
     if (lane == 0) {
         const int fin = atomicAdd(&L.ctr[1], 1);
@@ -154,9 +203,10 @@ __global__ void __launch_bounds__(256) refactor_kernel(FactorLaunch L)
 
 // Per-matrix status words are reset by a kernel (not a memset) so the whole
 // refactor sequence stays capturable in a HIP graph.
-__global__ void reset_mstat_kernel(int *mstat, int nmat, int *abort_word)
+__global__ void reset_mstat_kernel(int *mstat, int nmat, int *abort_word, int *dbg)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (dbg && i < 8) dbg[i] = -1;
     if (i < nmat) {
         mstat[4 * i + 0] = 0;
         mstat[4 * i + 1] = 0x7fffffff;
@@ -185,12 +235,12 @@ template hipError_t launch_refactor<float>(const FactorLaunch &, int, hipStream_
 template hipError_t launch_refactor<double>(const FactorLaunch &, int, hipStream_t);
 template hipError_t launch_refactor<__half>(const FactorLaunch &, int, hipStream_t);
 
-hipError_t launch_reset_mstat(int *mstat, int nmat, int *abort_word, hipStream_t st)
+hipError_t launch_reset_mstat(int *mstat, int nmat, int *abort_word, int *dbg, hipStream_t st)
 {
     const int threads = 256;
     const int blocks = (nmat + threads - 1) / threads;
     hipLaunchKernelGGL(reset_mstat_kernel, dim3(blocks > 0 ? blocks : 1), dim3(threads), 0, st,
-                       mstat, nmat, abort_word);
+                       mstat, nmat, abort_word, dbg);
     return hipGetLastError();
 }
 

@@ -22,6 +22,8 @@ struct FactorLaunch {
     int cap;                // LDS doubles per wave
     int total_waves;
     double ptol;
+    int *dbg;               // 8 ints: first stuck wait (dep, seen, epoch, spins, column)
+    int *trace;             // optional: [ntask] grabbing wave, [waves*4] wave state
 };
 
 struct TriLaunch {
@@ -67,7 +69,7 @@ struct KrylovArgs {
 // factor.hip
 template <typename T> hipError_t launch_refactor(const FactorLaunch &L, int blocks, hipStream_t st);
 template <typename T> hipError_t refactor_set_lds(int cap);
-hipError_t launch_reset_mstat(int *mstat, int nmat, int *abort_word, hipStream_t st);
+hipError_t launch_reset_mstat(int *mstat, int nmat, int *abort_word, int *dbg, hipStream_t st);
 
 // trisolve.hip
 template <typename T>

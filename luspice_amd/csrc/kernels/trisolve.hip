@@ -47,13 +47,16 @@ __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
         int *flags = kForward ? M.fdone : M.bdone;
         double acc = 0.0;
         bool ok = true;
-        for (int e = rp[i] + lane; e < rp[i + 1]; e += kWave) {
-            const int j = ci[e];
-            const double v = to_f64(rv[e]);
-            ok = ok && wait_epoch(&flags[j], L.epoch, L.abort_word);
-            acc += v * src[j];
+        const int e0 = rp[i], e1 = rp[i + 1];
+        for (int b0 = e0; b0 < e1 && ok; b0 += kWave) {
+            const int e = b0 + lane;
+            const int j = e < e1 ? ci[e] : 0;
+            ok = wave_wait(e < e1 ? &flags[j] : nullptr, L.epoch, L.abort_word);
+            if (!ok) break;
+            acquire_fence();
+            if (e < e1) acc += to_f64(rv[e]) * src[j];
         }
-        if (!__all(ok)) continue;
+        if (!ok) continue;
         acc = wave_sum(acc);
         if (lane == 0) {
             if (kForward) {
@@ -64,7 +67,7 @@ __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
                 L.x[vo + M.q[i]] = zi;
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            st_release(&flags[i], L.epoch);
+            publish(&flags[i], L.epoch);
         }
     }
     if (lane == 0) {

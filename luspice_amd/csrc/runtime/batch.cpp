@@ -15,6 +15,8 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 
@@ -54,6 +56,81 @@ void *Batch::dalloc(size_t bytes)
     if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
     mem_.push_back({p, bytes});
     return p;
+}
+
+void Batch::dfree(void *p)
+{
+    if (!p) return;
+    for (auto it = mem_.begin(); it != mem_.end(); ++it) {
+        if (it->p == p) {
+            (void)hipFree(p);
+            mem_.erase(it);
+            return;
+        }
+    }
+}
+
+// Krylov storage for restart length mm (basis, Hessenberg, rotations).
+int Batch::alloc_krylov(int mm)
+{
+    const int nc = (int)circ_.size();
+    for (void *p : {(void *)d_V_, (void *)d_partial_, (void *)d_H_, (void *)d_cs_, (void *)d_sn_,
+                    (void *)d_g_, (void *)d_yv_, (void *)d_hcol_})
+        dfree(p);
+    d_V_ = (double *)dalloc((size_t)(mm + 1) * ntot_ * 8);
+    d_partial_ = (double *)dalloc((size_t)nchunks_ * (mm + 1) * 8);
+    d_H_ = (double *)dalloc((size_t)nc * (mm + 1) * mm * 8);
+    d_cs_ = (double *)dalloc((size_t)nc * mm * 8);
+    d_sn_ = (double *)dalloc((size_t)nc * mm * 8);
+    d_g_ = (double *)dalloc((size_t)nc * (mm + 1) * 8);
+    d_yv_ = (double *)dalloc((size_t)nc * mm * 8);
+    d_hcol_ = (double *)dalloc((size_t)nc * (mm + 1) * 8);
+    if (!d_V_ || !d_partial_ || !d_H_ || !d_cs_ || !d_sn_ || !d_g_ || !d_yv_ || !d_hcol_)
+        return LSP_E_ALLOC;
+    mm_alloc_ = mm;
+    return LSP_OK;
+}
+
+int Batch::set_ir_params(int outer, int inner, double inner_tol, double outer_tol)
+{
+    if (outer < 1 || inner < 1 || !(inner_tol > 0) || !(outer_tol > 0)) return LSP_E_ARG;
+    if (inner > mm_alloc_) {
+        int rc = alloc_krylov(inner);
+        if (rc) return rc;
+    }
+    opt_.ir_outer = outer;
+    opt_.ir_inner = inner;
+    opt_.ir_inner_tol = inner_tol;
+    opt_.ir_outer_tol = outer_tol;
+    return LSP_OK;
+}
+
+int Batch::diag_stats(int c, double *min_abs, double *max_abs)
+{
+    if (c < 0 || c >= (int)circ_.size() || cur_prec_ < 0) return LSP_E_ARG;
+    const int n = circ_[c].n;
+    std::vector<double> d(n);
+    int rc;
+    if (cur_prec_ == LSP_PREC_FP64) {
+        rc = check(hipMemcpy(d.data(), f64_.diag + circ_[c].o_n, n * 8, hipMemcpyDeviceToHost), "diag");
+    } else if (cur_prec_ == LSP_PREC_FP32) {
+        std::vector<float> f(n);
+        rc = check(hipMemcpy(f.data(), f32_.diag + circ_[c].o_n, n * 4, hipMemcpyDeviceToHost), "diag");
+        for (int i = 0; i < n; i++) d[i] = f[i];
+    } else {
+        std::vector<__half> f(n);
+        rc = check(hipMemcpy(f.data(), f16_.diag + circ_[c].o_n, n * 2, hipMemcpyDeviceToHost), "diag");
+        for (int i = 0; i < n; i++) d[i] = (double)__half2float(f[i]);
+    }
+    if (rc) return rc;
+    double lo = INFINITY, hi = 0;
+    for (double v : d) {
+        lo = std::min(lo, std::fabs(v));
+        hi = std::max(hi, std::fabs(v));
+    }
+    if (min_abs) *min_abs = lo;
+    if (max_abs) *max_abs = hi;
+    return LSP_OK;
 }
 
 template <typename X>
@@ -292,6 +369,7 @@ int Batch::create(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
         d_ctr_ = upload(z6); d_abort_ = upload(z6);
         std::vector<int> ms(4 * nc, 0);
         d_mstat_ = upload(ms);
+        d_dbg_ = upload(z6);
         if (!d_done_ || !d_fdone_ || !d_bdone_ || !d_ctr_ || !d_abort_ || !d_mstat_) return LSP_E_ALLOC;
     }
     d_ain_ = (double *)dalloc(nnz_tot_ * sizeof(double));
@@ -332,7 +410,7 @@ int Batch::create(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
     }
     // merged task lists: (level, circuit, index)
     {
-        auto merge = [&](auto level_of, auto order_of) {
+        auto merge = [&](auto level_of, auto order_of, long &width) {
             std::vector<std::pair<long, int2>> all;
             all.reserve(ntot_);
             for (int m = 0; m < nc; m++) {
@@ -343,15 +421,23 @@ int Batch::create(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
             std::stable_sort(all.begin(), all.end(),
                              [](const auto &a, const auto &b) { return a.first < b.first; });
             std::vector<int2> out(all.size());
-            for (size_t i = 0; i < all.size(); i++) out[i] = all[i].second;
+            width = 0;
+            for (size_t i = 0, run = 0; i < all.size(); i++) {
+                out[i] = all[i].second;
+                run = (i > 0 && all[i].first == all[i - 1].first) ? run + 1 : 1;
+                width = std::max(width, (long)run);
+            }
             return out;
         };
         auto tf = merge([](const LspPlan &P, int k) { return P.level[k]; },
-                        [](const LspPlan &P) -> const std::vector<int> & { return P.ftask; });
+                        [](const LspPlan &P) -> const std::vector<int> & { return P.ftask; },
+                        width_f_);
         auto tw = merge([](const LspPlan &P, int k) { return P.lvf[k]; },
-                        [](const LspPlan &P) -> const std::vector<int> & { return P.fwd_task; });
+                        [](const LspPlan &P) -> const std::vector<int> & { return P.fwd_task; },
+                        width_tf_);
         auto tb = merge([](const LspPlan &P, int k) { return P.lvb[k]; },
-                        [](const LspPlan &P) -> const std::vector<int> & { return P.bwd_task; });
+                        [](const LspPlan &P) -> const std::vector<int> & { return P.bwd_task; },
+                        width_tb_);
         d_ftask_ = upload(tf); d_fwtask_ = upload(tw); d_bwtask_ = upload(tb);
         if (!d_ftask_ || !d_fwtask_ || !d_bwtask_) return LSP_E_ALLOC;
     }
@@ -371,10 +457,17 @@ int Batch::create(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
             const size_t lds_cu = 160 * 1024;
             per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, lds_cu / lds));
         }
-        long want = (ntot_ + 3) / 4;
-        fblocks_ = (int)std::max(1L, std::min<long>((long)per_cu * cu_count_, want));
-        tblocks_f_ = (int)std::max(1L, std::min<long>(8L * cu_count_, want));
-        tblocks_b_ = tblocks_f_;
+        // Waves beyond the widest dependency level only poll; size each
+        // persistent grid to about twice that width (4 waves per block).
+        auto blocks_for = [&](long width, long cap_blocks) {
+            long want = (2 * width + 3) / 4;
+            return (int)std::max(1L, std::min(cap_blocks, want));
+        };
+        fblocks_ = blocks_for(width_f_, (long)per_cu * cu_count_);
+        tblocks_f_ = blocks_for(width_tf_, 8L * cu_count_);
+        tblocks_b_ = blocks_for(width_tb_, 8L * cu_count_);
+        if (opt_.refactor_blocks > 0) fblocks_ = opt_.refactor_blocks;
+        if (opt_.trisolve_blocks > 0) tblocks_f_ = tblocks_b_ = opt_.trisolve_blocks;
         int maxcol = 0;
         for (auto &C : circ_) maxcol = std::max(maxcol, C.plan.max_col);
         scratch_stride_ = maxcol > cap ? maxcol : 1;
@@ -388,7 +481,6 @@ int Batch::create(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
         d_r_ = (double *)dalloc(ntot_ * 8); d_w_ = (double *)dalloc(ntot_ * 8);
         d_z_ = (double *)dalloc(ntot_ * 8); d_t_ = (double *)dalloc(ntot_ * 8);
         d_ty_ = (double *)dalloc(ntot_ * 8); d_tz_ = (double *)dalloc(ntot_ * 8);
-        d_V_ = (double *)dalloc((size_t)(mm + 1) * ntot_ * 8);
         std::vector<long> voff(nc);
         std::vector<int4> chunks;
         std::vector<int> cbeg(nc + 1, 0);
@@ -404,20 +496,15 @@ int Batch::create(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
         d_voff_ = upload(voff);
         d_chunks_ = upload(chunks);
         d_cbeg_ = upload(cbeg);
-        d_partial_ = (double *)dalloc((size_t)nchunks_ * (mm + 1) * 8);
-        d_H_ = (double *)dalloc((size_t)nc * (mm + 1) * mm * 8);
-        d_cs_ = (double *)dalloc((size_t)nc * mm * 8);
-        d_sn_ = (double *)dalloc((size_t)nc * mm * 8);
-        d_g_ = (double *)dalloc((size_t)nc * (mm + 1) * 8);
-        d_yv_ = (double *)dalloc((size_t)nc * mm * 8);
+        rc = alloc_krylov(mm);
+        if (rc) return rc;
         d_beta_ = (double *)dalloc(nc * 8); d_nb_ = (double *)dalloc(nc * 8);
         d_hn_ = (double *)dalloc(nc * 8);
-        d_hcol_ = (double *)dalloc((size_t)nc * (mm + 1) * 8);
         d_act_ = (int *)dalloc(nc * 4); d_jfin_ = (int *)dalloc(nc * 4);
         d_conv_ = (int *)dalloc(nc * 4); d_cycled_ = (int *)dalloc(nc * 4);
         d_nact_ = (int *)dalloc(16); d_subset_ = (int *)dalloc(nc * 4);
-        if (!d_b_ || !d_x_ || !d_r_ || !d_w_ || !d_z_ || !d_t_ || !d_ty_ || !d_tz_ || !d_V_ ||
-            !d_voff_ || !d_chunks_ || !d_cbeg_ || !d_partial_ || !d_H_ || !d_act_)
+        if (!d_b_ || !d_x_ || !d_r_ || !d_w_ || !d_z_ || !d_t_ || !d_ty_ || !d_tz_ ||
+            !d_voff_ || !d_chunks_ || !d_cbeg_ || !d_act_)
             return LSP_E_ALLOC;
         if (hipHostMalloc((void **)&h_pin_, 4096, hipHostMallocDefault) != hipSuccess)
             return LSP_E_ALLOC;
@@ -488,6 +575,7 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
     L.active = d_active;
     L.ntask = (int)ntot_;
     L.epoch = next_epoch();
+    next_epoch_peek_ = L.epoch;
     L.ctr = d_ctr_;
     L.abort_word = d_abort_;
     L.scratch = d_scratch_;
@@ -495,7 +583,17 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
     L.cap = opt_.lds_cap;
     L.total_waves = fblocks_ * 4;
     L.ptol = opt_.refactor_ptol;
-    rc = check(launch_reset_mstat(d_mstat_, (int)circ_.size(), d_abort_, stream_), "reset");
+    L.dbg = d_dbg_;
+    L.trace = nullptr;
+    if (getenv("LSP_TRACE_REFACTOR")) {
+        if (!d_trace_) {
+            trace_len_ = ntot_ + 8L * fblocks_ * 4;
+            d_trace_ = (int *)dalloc(trace_len_ * sizeof(int));
+        }
+        (void)hipMemset(d_trace_, 0xff, trace_len_ * sizeof(int));
+        L.trace = d_trace_;
+    }
+    rc = check(launch_reset_mstat(d_mstat_, (int)circ_.size(), d_abort_, d_dbg_, stream_), "reset");
     if (rc) return rc;
     (void)hipEventRecord(ev_[0], stream_);
     rc = check(launch_refactor<T>(L, fblocks_, stream_), "refactor launch");
@@ -540,6 +638,50 @@ int Batch::refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *
         if (ms[4 * m + 2] != 0x7fffffff) r.singular_circuits++;
         if ((ms[4 * m] || ms[4 * m + 2] != 0x7fffffff || ms[4 * m + 3]) && r.first_bad_circuit < 0)
             r.first_bad_circuit = m;
+    }
+    if (d_trace_ && getenv("LSP_TRACE_REFACTOR")) {
+        std::vector<int> tr(trace_len_);
+        std::vector<int> dn(ntot_);
+        (void)hipMemcpy(tr.data(), d_trace_, trace_len_ * 4, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(dn.data(), d_done_, ntot_ * 4, hipMemcpyDeviceToHost);
+        std::vector<int> tk(ntot_ * 2);
+        (void)hipMemcpy(tk.data(), d_ftask_, ntot_ * 8, hipMemcpyDeviceToHost);
+        std::vector<int> col_task(ntot_, -1);
+        for (long t = 0; t < ntot_; t++) col_task[tk[2 * t + 1]] = (int)t;
+        long grabbed = 0, ndone = 0;
+        for (long t = 0; t < ntot_; t++) grabbed += tr[t] >= 0;
+        for (long k = 0; k < ntot_; k++) ndone += dn[k] == next_epoch_peek_;
+        fprintf(stderr, "[trace] tasks grabbed %ld / %ld, columns done %ld\n", grabbed, ntot_, ndone);
+        const long nw = (long)fblocks_ * 4;
+        int mrel = 0, macq = 0, mcol = 0;
+        long cols = 0;
+        for (long w = 0; w < nw; w++) {
+            const int *s = tr.data() + ntot_ + 8 * w;
+            mrel = std::max(mrel, s[4]); macq = std::max(macq, s[5]);
+            mcol = std::max(mcol, s[7]); cols += s[6];
+        }
+        fprintf(stderr, "[trace] max release fence %d Kclk, max acquire fence %d Kclk, "
+                        "max column %d Kclk, columns finished %ld\n", mrel, macq, mcol, cols);
+        for (long w = 0; w < nw; w++) {
+            const int *s = tr.data() + ntot_ + 8 * w;
+            if (s[1] == 7) continue;
+            fprintf(stderr, "[trace] wave %ld task %d phase %d a %d b %d", w, s[0], s[1], s[2], s[3]);
+            if (s[1] == 3 && s[3] >= 0)
+                fprintf(stderr, " -> dep %d task %d grabbed by wave %d done=%d", s[3],
+                        col_task[s[3]], tr[col_task[s[3]]], dn[s[3]]);
+            fprintf(stderr, "\n");
+        }
+    }
+    if (abort_code == kErrSpin) {
+        int dbg[8] = {0};
+        (void)hipMemcpy(dbg, d_dbg_, sizeof(dbg), hipMemcpyDeviceToHost);
+        int flag_now = -99;
+        if (dbg[0] >= 0 && dbg[0] < ntot_)
+            (void)hipMemcpy(&flag_now, d_done_ + dbg[0], 4, hipMemcpyDeviceToHost);
+        last_error = "refactor wait gave up: column " + std::to_string(dbg[4]) + " waiting on " +
+                     std::to_string(dbg[0]) + " saw " + std::to_string(dbg[1]) + " epoch " +
+                     std::to_string(dbg[2]) + " spins " + std::to_string(dbg[3]) +
+                     "; flag in memory after the launch: " + std::to_string(flag_now);
     }
     r.status = abort_code == kErrSpin ? LSP_E_DEVICE
              : abort_code == kErrCapacity ? LSP_E_CAPACITY
@@ -669,6 +811,7 @@ int Batch::gmres_ir(const double *d_bv, double *d_xv, int prec, const std::vecto
     LSP_HIP(k_reduce(K, 1, 2, 0, d_nb_, s), "reduce");
     LSP_HIP(k_chunk_fill(K, d_xv, 0.0, s), "zero x");
     int outer_used = 0, inner_total = 0;
+    res_hist0_.clear();
     for (int outer = 0; outer < opt_.ir_outer; outer++) {
         LSP_HIP(k_activate(K, s), "activate");
         LSP_TRY(launch_spmv64(d_xv, d_r_, d_bv, d_act_));
@@ -676,7 +819,13 @@ int Batch::gmres_ir(const double *d_bv, double *d_xv, int prec, const std::vecto
         LSP_HIP(k_reduce(K, 1, 2, 0, d_beta_, s), "reduce");
         LSP_HIP(k_zero_int(d_nact_, 1, s), "zero");
         LSP_HIP(k_start_cycle(K, d_nact_, s), "start");
-        LSP_TRY(sync_read_int(d_nact_, h_pin_, 1));
+        {
+            double bn[2];
+            LSP_HIP(hipMemcpyAsync(bn, d_beta_, 8, hipMemcpyDeviceToHost, s), "beta0");
+            LSP_HIP(hipMemcpyAsync(bn + 1, d_nb_, 8, hipMemcpyDeviceToHost, s), "nb0");
+            LSP_TRY(sync_read_int(d_nact_, h_pin_, 1));
+            res_hist0_.push_back(bn[1] > 0 ? bn[0] / bn[1] : bn[0]);
+        }
         if (h_pin_[0] == 0) break;
         outer_used++;
         LSP_HIP(k_chunk_scale(K, d_r_, d_V_, d_beta_, s), "scale");

@@ -63,6 +63,10 @@ public:
     int device_vectors(double **b, double **x);
     int last_times(double *rf, double *fw, double *bw) const;
     int kernel_stats(lsp_kernel_stats *st, int reset);
+    int set_ir_params(int outer, int inner, double inner_tol, double outer_tol);
+    int diag_stats(int c, double *min_abs, double *max_abs);
+    const std::vector<double> &residual_history() const { return res_hist0_; }
+    const CircuitHost &circuit(int c) const { return circ_[c]; }
     long total_n() const { return ntot_; }
     long total_nnz() const { return nnz_tot_; }
     int ncircuits() const { return (int)circ_.size(); }
@@ -82,6 +86,8 @@ private:
     template <typename T> int trisolve_t(FactorSet<T> &fs, const double *d_b, double *d_x,
                                          const int *d_active);
     void *dalloc(size_t bytes);
+    void dfree(void *p);
+    int alloc_krylov(int mm);
     template <typename X> X *upload(const std::vector<X> &v);
     int check(hipError_t e, const char *what);
     int sync_read_int(const int *d, int *h, int count);
@@ -116,9 +122,14 @@ private:
     int *d_ctr_ = nullptr;               // [0..1] factor, [2..3] fwd, [4..5] bwd
     int *d_abort_ = nullptr;             // [0] factor, [1] trisolve
     int *d_mstat_ = nullptr;             // 4 per circuit
+    int *d_dbg_ = nullptr;               // first stuck wait of the refactor
+    int *d_trace_ = nullptr;             // optional refactor trace (LSP_TRACE_REFACTOR)
+    long trace_len_ = 0;
+    int next_epoch_peek_ = 0;
     double *d_scratch_ = nullptr;
     long scratch_stride_ = 0;
     int fblocks_ = 0, tblocks_f_ = 0, tblocks_b_ = 0;
+    long width_f_ = 0, width_tf_ = 0, width_tb_ = 0;   // widest merged level
     FactorSet<float> f32_;
     FactorSet<double> f64_;
     FactorSet<__half> f16_;
@@ -138,6 +149,8 @@ private:
     hipEvent_t ev_[6] = {};
     float t_refactor_ = 0, t_fwd_ = 0, t_bwd_ = 0;
     bool tri_pending_ = false;
+    int mm_alloc_ = 0;
+    std::vector<double> res_hist0_;      // ||r||/||b|| of circuit 0 at each restart
     lsp_kernel_stats stats_{};
 };
 

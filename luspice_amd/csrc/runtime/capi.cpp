@@ -125,6 +125,11 @@ int lsp_batch_last_times(const lsp_batch *b, double *rf, double *fw, double *bw)
     return b ? b->impl.last_times(rf, fw, bw) : LSP_E_ARG;
 }
 
+const char *lsp_batch_last_error(const lsp_batch *b)
+{
+    return b ? b->impl.last_error.c_str() : "";
+}
+
 int lsp_batch_kernel_stats(lsp_batch *b, lsp_kernel_stats *st, int reset)
 {
     return b ? b->impl.kernel_stats(st, reset) : LSP_E_ARG;

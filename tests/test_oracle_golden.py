@@ -1,0 +1,292 @@
+"""CPU tests: the oracle and the product's host-side pieces against the
+fixtures written by the reference's own C (tests/golden/make_golden.py),
+against brute-force dense solves and closed forms, and against invariants.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import luspice_amd as L
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def g0():
+    return dict(np.load(os.path.join(GOLD, "c0.npz")))
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    return dict(np.load(os.path.join(GOLD, "tiny.npz")))
+
+
+KEYS = ("pinv", "Lp", "Li", "Lx", "Up", "Ui", "Ux")
+
+
+def _same(F, g, prefix):
+    for k in KEYS:
+        assert np.array_equal(F[k], g[f"{prefix}_{k}"]), k
+
+
+# --------------------------------------------------------------------------
+# oracle pinned to the reference
+def test_oracle_factors_bitwise_c0(g0):
+    F = O.lu_factor(int(g0["n"]), g0["row"], g0["col"], g0["val"], g0["q"], which="oracle")
+    _same(F, g0, "f64")
+
+
+def test_oracle_cpu_lu_ref_bitwise_c0(g0):
+    rc, x, ln, un = O.cpu_lu_ref(int(g0["n"]), g0["row"], g0["col"], g0["val"], g0["q"], g0["rhs"])
+    assert rc == -1 and ln == g0["lnnz"] and un == g0["unnz"]
+    assert np.array_equal(x, g0["x"])
+
+
+def test_oracle_levels_c0(g0):
+    lv, nl = O.levels(int(g0["n"]), g0["f64_Up"], g0["f64_Ui"])
+    assert nl == g0["nlevels"] and np.array_equal(lv, g0["level"])
+
+
+@pytest.mark.parametrize("case", ["ties", "unsym8", "struct_singular", "num_singular"])
+@pytest.mark.parametrize("qn", ["nat", "rev"])
+def test_oracle_tiny(tiny, case, qn):
+    k = f"{case}_{qn}"
+    n = int(tiny[f"{k}_n"])
+    F = O.lu_factor(n, tiny[f"{k}_row"], tiny[f"{k}_col"], tiny[f"{k}_val"], tiny[f"{k}_q"])
+    assert F["singular"] == tiny[f"{k}_singular"]
+    if F["singular"] == -1:
+        _same(F, tiny, k)
+        rc, x, _, _ = O.cpu_lu_ref(n, tiny[f"{k}_row"], tiny[f"{k}_col"], tiny[f"{k}_val"],
+                                   tiny[f"{k}_q"], tiny[f"{k}_b"])
+        assert np.array_equal(x, tiny[f"{k}_x"])
+        # brute force: dense solve of A x = b
+        A = np.zeros((n, n))
+        np.add.at(A, (tiny[f"{k}_row"], tiny[f"{k}_col"]), tiny[f"{k}_val"])
+        np.testing.assert_allclose(A @ x, tiny[f"{k}_b"], atol=1e-12)
+
+
+def _dense_factors(F, n):
+    Ld = np.zeros((n, n))
+    Ud = np.zeros((n, n))
+    for k in range(n):
+        for e in range(F["Lp"][k], F["Lp"][k + 1]):
+            Ld[F["Li"][e], k] = F["Lx"][e]
+        for e in range(F["Up"][k], F["Up"][k + 1]):
+            Ud[F["Ui"][e], k] = F["Ux"][e]
+    return Ld, Ud
+
+
+def test_oracle_closed_form_PAQ_equals_LU(g0):
+    """P A(:,q) = L U, with L unit lower, U upper, diagonal positions first/last."""
+    n = int(g0["n"])
+    F = {k: g0[f"f64_{k}"] for k in KEYS}
+    Ld, Ud = _dense_factors(F, n)
+    assert np.allclose(np.diag(Ld), 1.0) and np.allclose(np.triu(Ld, 1), 0)
+    assert np.allclose(np.tril(Ud, -1), 0)
+    for k in range(n):
+        assert F["Li"][F["Lp"][k]] == k and F["Ui"][F["Up"][k + 1] - 1] == k
+    A = np.zeros((n, n))
+    np.add.at(A, (g0["row"], g0["col"]), g0["val"])
+    P = np.zeros((n, n))
+    P[F["pinv"], np.arange(n)] = 1.0
+    PAQ = P @ A[:, g0["q"]]
+    assert np.abs(PAQ - Ld @ Ud).max() <= 1e-12 * np.abs(A).max()
+    # x = Q U^-1 L^-1 P b
+    z = np.linalg.solve(Ud, np.linalg.solve(Ld, P @ g0["rhs"]))
+    x = np.empty(n)
+    x[g0["q"]] = z
+    assert np.linalg.norm(x - g0["x"]) <= 1e-12 * np.linalg.norm(g0["x"])
+
+
+def test_oracle_trisolve_f32_closed_form(g0):
+    n = int(g0["n"])
+    F = {k: g0[f"f32_{k}"] for k in KEYS}
+    b = g0["rhs"]
+    x = O.trisolve_f32(n, F, g0["q"], b)
+    Fd = dict(F, Lx=F["Lx"].astype(np.float64), Ux=F["Ux"].astype(np.float64))
+    Ld, Ud = _dense_factors(Fd, n)
+    P = np.zeros((n, n))
+    P[F["pinv"], np.arange(n)] = 1.0
+    z = np.linalg.solve(Ud, np.linalg.solve(Ld, P @ b))
+    xd = np.empty(n)
+    xd[g0["q"]] = z
+    assert np.linalg.norm(x - xd) <= 1e-10 * np.linalg.norm(xd)
+
+
+def test_oracle_gmres_ir_c0(g0):
+    """The reference's mixed path (fp32 splu factors + fp64 GMRES-IR), restated,
+    reaches the fp64 solution to 1e-6 and a 1e-10 residual."""
+    n = int(g0["n"])
+    F = {k: g0[f"f32_{k}"] for k in KEYS}
+    x, info = O.gmres_ir(n, g0["row"], g0["col"], g0["val"], g0["rhs"], F, g0["q"])
+    assert info["rc"] == 0 and info["res"] <= 1e-9
+    assert O.rel_l2(x, g0["x"]) <= 1e-6
+
+
+def test_oracle_gmres_catches_wrong_factor(g0):
+    """A factor with one flipped sign still converges in GMRES (it only
+    preconditions) but its direct solve misses the oracle by far."""
+    n = int(g0["n"])
+    F = {k: g0[f"f32_{k}"].copy() for k in KEYS}
+    F["Ux"][F["Up"][n // 2]] *= -1.0
+    xd = O.trisolve_f32(n, F, g0["q"], g0["rhs"])
+    assert O.rel_l2(xd, g0["x"]) > 1e-6
+
+
+def test_oracle_ruiz_invariants():
+    c = L.mna("c0", seed=0)
+    v, R, Cs = O.ruiz(c.n, c.row, c.col, c.val, sweeps=30)
+    # A_s = diag(R) A diag(C) exactly as applied
+    np.testing.assert_allclose(v, R[c.row] * c.val * Cs[c.col], rtol=1e-12)
+    rmax = np.zeros(c.n)
+    np.maximum.at(rmax, c.row, np.abs(v))
+    cmax = np.zeros(c.n)
+    np.maximum.at(cmax, c.col, np.abs(v))
+    assert np.all(np.abs(rmax - 1) < 1e-3) and np.all(np.abs(cmax - 1) < 1e-3)
+    # solving the scaled system recovers x = C x_s
+    A = c.dense()
+    As = np.zeros_like(A)
+    np.add.at(As, (c.row, c.col), v)
+    xs = np.linalg.solve(As, R * c.rhs)
+    np.testing.assert_allclose(Cs * xs, np.linalg.solve(A, c.rhs), rtol=1e-8, atol=1e-12)
+
+
+def test_oracle_sell_order_rule():
+    rng = np.random.default_rng(2)
+    lens = rng.integers(0, 9, 200)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    order, slens, offs, tot = O.sell_order(200, rp, 32)
+    # descending length; equal lengths in reverse index order (counting sort)
+    ref = sorted(range(200), key=lambda i: (-lens[i], -i))
+    assert list(order) == ref
+    assert tot == sum(32 * max(lens[order[s * 32:(s + 1) * 32]]) for s in range(len(slens)))
+
+
+# --------------------------------------------------------------------------
+# product host pieces pinned to the reference
+def test_product_symbolic_bitwise_c0(g0):
+    c = L.Circuit(int(g0["n"]), g0["row"], g0["col"], g0["val"], g0["rhs"])
+    F = L.symbolic_factor(c, g0["q"])
+    _same(F, g0, "f64")
+    rc, x, ln, un = L.cpu_lu_ref(c, g0["q"])
+    assert rc == -1 and np.array_equal(x, g0["x"]) and ln == g0["lnnz"]
+
+
+@pytest.mark.parametrize("case", ["ties", "unsym8", "struct_singular", "num_singular"])
+def test_product_symbolic_tiny(tiny, case):
+    for qn in ("nat", "rev"):
+        k = f"{case}_{qn}"
+        n = int(tiny[f"{k}_n"])
+        c = L.Circuit(n, tiny[f"{k}_row"], tiny[f"{k}_col"], tiny[f"{k}_val"], tiny[f"{k}_b"])
+        rc, x, _, _ = L.cpu_lu_ref(c, tiny[f"{k}_q"])
+        assert rc == tiny[f"{k}_singular"]
+        if rc == -1:
+            _same(L.symbolic_factor(c, tiny[f"{k}_q"]), tiny, k)
+            assert np.array_equal(x, tiny[f"{k}_x"])
+
+
+def test_product_levels(g0):
+    import ctypes as C
+    from luspice_amd import _native as N
+    n = int(g0["n"])
+    lv = np.empty(n, np.int32)
+    nl = C.c_int()
+    N.lib.dsh_level_compute(n, N.ptr(g0["f64_Up"]), N.ptr(g0["f64_Ui"]), N.ptr(lv), C.byref(nl))
+    assert nl.value == g0["nlevels"] and np.array_equal(lv, g0["level"])
+    hist = np.empty(nl.value, np.int32)
+    N.lib.dsh_level_hist(n, N.ptr(lv), nl.value, N.ptr(hist))
+    assert np.array_equal(hist, np.bincount(lv, minlength=nl.value))
+
+
+def test_product_selector_kat_and_random():
+    import ctypes as C
+    from luspice_amd import _native as N
+    from tests.golden.make_golden import Config, MatrixInfo
+    lib = N.lib
+    lib.dsh_format_select.argtypes = [C.POINTER(MatrixInfo), C.POINTER(Config)]
+    lib.dsh_mp_escalate.restype = C.c_int
+    lib.dsh_mp_name.restype = C.c_char_p
+    lib.dsh_fmt_name.restype = C.c_char_p
+    lib.dsh_matrix_analyze.argtypes = [C.c_int, C.c_long, C.POINTER(C.c_int), C.POINTER(MatrixInfo)]
+    with open(os.path.join(GOLD, "selector.json")) as f:
+        G = json.load(f)
+    for case in G["kat"]:
+        mi, cfg = MatrixInfo(*case["info"]), Config()
+        lib.dsh_format_select(C.byref(mi), C.byref(cfg))
+        if "fmt" in case:
+            assert cfg.spmv_format == case["fmt"], case["name"]
+        assert cfg.mp_path == case["path"], case["name"]
+        if "pre_ruiz" in case:
+            assert cfg.pre_ruiz_hint == case["pre_ruiz"]
+        if "factor_prec" in case:
+            assert cfg.factor_prec == case["factor_prec"]
+    for cur, nxt in G["escalate"].items():
+        assert lib.dsh_mp_escalate(int(cur)) == nxt
+    assert lib.dsh_mp_name(0) == b"P-fast" and lib.dsh_fmt_name(1) == b"SELL-C-sigma32"
+    for r in G["random"]:
+        mi, cfg = MatrixInfo(*r["info"]), Config()
+        lib.dsh_format_select(C.byref(mi), C.byref(cfg))
+        got = {k: getattr(cfg, k) for k, _ in Config._fields_}
+        assert got == r["cfg"]
+    for a in G["analyze"]:
+        rp = np.array(a["rowptr"], np.int32)
+        mi = MatrixInfo()
+        lib.dsh_matrix_analyze(len(rp) - 1, int(rp[-1]), N.ptr(rp), C.byref(mi))
+        got = {k: getattr(mi, k) for k, _ in MatrixInfo._fields_}
+        assert got == pytest.approx(a["out"])
+
+
+# --------------------------------------------------------------------------
+# invariants of the unpinned pieces (COLAMD, generator)
+def test_colamd_invariants():
+    c = L.mna("c0", seed=0)
+    q = L.colamd(c)
+    assert sorted(q) == list(range(c.n))
+    assert np.array_equal(q, L.colamd(c))                     # deterministic
+    Fq = L.symbolic_factor(c, q)
+    Fn = L.symbolic_factor(c, None)
+    assert Fq["Lp"][-1] + Fq["Up"][-1] < 0.5 * (Fn["Lp"][-1] + Fn["Up"][-1])
+    # a column with more than 16 entries is dense for all-zero knobs: last
+    n = 40
+    row, col, val = [], [], []
+    for i in range(n):
+        row.append(i); col.append(i); val.append(4.0)
+        if i + 1 < n:
+            row += [i, i + 1]; col += [i + 1, i]; val += [-1.0, -1.0]
+    for i in range(20):
+        row.append(i); col.append(7); val.append(0.5)
+    d = L.Circuit(n, row, col, val)
+    qd = L.colamd(d)
+    assert qd[-1] == 7 and sorted(qd) == list(range(n))
+
+
+def test_mna_generator_invariants():
+    c = L.mna("c0", seed=0)
+    assert c.n == 1020
+    A = c.dense()
+    assert np.array_equal(A != 0, (A != 0).T)                   # symmetric pattern
+    src_rows = range(1000, 1020)
+    assert all(A[r, r] == 0 for r in src_rows)                  # zero-diagonal rows
+    assert all(np.count_nonzero(A[r]) == 1 for r in src_rows)
+    c2 = L.mna("c0", seed=0)
+    assert np.array_equal(c.val, c2.val) and np.array_equal(c.rhs, c2.rhs)
+    assert not np.array_equal(L.mna("c0", seed=1).val, c.val)
+    # node rows are diagonally dominant (capacitors to ground)
+    nodes = A[:1000, :1000]
+    off = np.abs(nodes).sum(1) - np.abs(np.diag(nodes))
+    assert np.all(np.diag(nodes) > off)
+    s = L.mna("sized", n=777, seed=3)
+    assert s.n == 777
+
+
+def test_perturb_is_seeded_lognormal():
+    c = L.mna("sized", n=300, seed=1)
+    v1, b1 = L.perturb(5, 2, c.val, c.n)
+    v2, b2 = L.perturb(5, 2, c.val, c.n)
+    assert np.array_equal(v1, v2) and np.array_equal(b1, b2)
+    r = np.log(v1 / c.val)
+    assert abs(r.mean()) < 0.02 and abs(r.std() - 0.1) < 0.02
+    assert np.all(np.abs(b1) <= 1)
