@@ -51,6 +51,31 @@ __device__ __forceinline__ void st_relaxed(int *p, int v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Wave-uniform reads for control flow.  A persistent wave's loop must stay
+// converged: if its shared state (the task ticket, the abort word) lives in a
+// VGPR, the compiler treats every branch on it as divergent and may build a
+// loop in which some lanes run on without the others (observed: a ticket
+// taken under `if (lane == 0)` and broadcast with a shuffle let lanes 1..63
+// loop back to the shuffle with lane 0 parked, re-running one task forever).
+// Values read through these helpers sit in SGPRs, so branches on them are
+// scalar and the wave moves as one.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Takes the next ticket from a queue counter: every lane executes the atomic,
+// the first active lane adds 1 and the others 0, and its result is the
+// ticket.  No lane-dependent branch surrounds the atomic.
+__device__ __forceinline__ int wave_ticket(int *ctr)
+{
+    const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+    const int v = atomicAdd(ctr, lane_id() == leader ? 1 : 0);
+    return __builtin_amdgcn_readlane(v, leader);
+}
+
+__device__ __forceinline__ bool aborted(const int *abort_word)
+{
+    return uniform(ld_relaxed(abort_word)) != 0;
+}
+
 // Orders this wave's earlier LDS / global accesses before its later ones as
 // seen by other lanes of the same wave.
 __device__ __forceinline__ void wave_sync()
@@ -85,7 +110,7 @@ __device__ __forceinline__ bool wave_wait(const int *flag, int epoch, int *abort
             ok = seen == epoch;
         }
         if (__all(ok)) return true;
-        if ((round & 15) == 15 && ld_relaxed(abort_word) != 0) return false;
+        if ((round & 15) == 15 && aborted(abort_word)) return false;
         if (round > kRoundLimit) {
             if (!ok && atomicCAS(abort_word, 0, kErrSpin) == 0 && dbg) {
                 dbg[1] = seen;

@@ -166,15 +166,13 @@ __global__ void __launch_bounds__(256) refactor_kernel(FactorLaunch L)
     trace_state(ws, 1, -1, -1);
 
     for (;;) {
-        int t = 0;
-        if (lane == 0) t = atomicAdd(&L.ctr[0], 1);
-        t = __shfl(t, 0, kWave);
+        const int t = wave_ticket(&L.ctr[0]);
         if (ws && lane == 0) {
             st_relaxed(ws, t);
             if (t < L.ntask) st_relaxed(L.trace + t, gw);
         }
         if (t >= L.ntask) break;
-        if (ld_relaxed(L.abort_word) != 0) continue;          // drain
+        if (aborted(L.abort_word)) continue;                  // drain
         const int2 tk = L.task[t];
         if (L.active && !L.active[tk.x]) continue;
         const FactorDesc<T> M = mats[tk.x];

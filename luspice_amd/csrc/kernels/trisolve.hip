@@ -28,11 +28,9 @@ __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
     const int lane = lane_id();
     const TriDesc<T> *mats = (const TriDesc<T> *)L.mats;
     for (;;) {
-        int t = 0;
-        if (lane == 0) t = atomicAdd(&L.ctr[0], 1);
-        t = __shfl(t, 0, kWave);
+        const int t = wave_ticket(&L.ctr[0]);
         if (t >= L.ntask) break;
-        if (ld_relaxed(L.abort_word) != 0) continue;
+        if (aborted(L.abort_word)) continue;
         const int2 tk = L.task[t];
         if (L.active && !L.active[tk.x]) continue;
         const TriDesc<T> M = mats[tk.x];
