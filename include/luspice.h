@@ -59,6 +59,8 @@ typedef struct {
     int fallback_fp64;     /* 1: stalled IR refactors in fp64 and retries */
     int refactor_blocks;   /* persistent refactor grid (0 = auto) */
     int trisolve_blocks;   /* persistent trisolve grid (0 = auto) */
+    int tail_max;          /* largest dense trailing block (0 = 8192, <0 = none) */
+    double tail_density;   /* min pattern density of that block (0 = 0.3) */
 } lsp_options;
 
 typedef struct {
@@ -86,6 +88,8 @@ typedef struct {
     int nlevels, nlv_fwd, nlv_bwd;
     int max_col;
     long sell_slots;
+    int tail_T;            /* order of the dense trailing block (0 = none) */
+    long tail_updates;     /* left-looking updates replaced by the dense block */
 } lsp_circuit_info;
 
 typedef struct {

@@ -38,7 +38,8 @@ class Options(C.Structure):
                 ("ir_inner_tol", C.c_double), ("ir_outer_tol", C.c_double),
                 ("lds_cap", C.c_int), ("blocks_per_cu", C.c_int), ("chunk_rows", C.c_int),
                 ("sell_sigma", C.c_int), ("fallback_fp64", C.c_int),
-                ("refactor_blocks", C.c_int), ("trisolve_blocks", C.c_int)]
+                ("refactor_blocks", C.c_int), ("trisolve_blocks", C.c_int),
+                ("tail_max", C.c_int), ("tail_density", C.c_double)]
 
 
 class RefactorReport(C.Structure):
@@ -55,7 +56,8 @@ class SolveReport(C.Structure):
 class CircuitInfo(C.Structure):
     _fields_ = [("n", C.c_int), ("nnz", C.c_long), ("lnnz", C.c_long), ("unnz", C.c_long),
                 ("n_updates", C.c_long), ("nlevels", C.c_int), ("nlv_fwd", C.c_int),
-                ("nlv_bwd", C.c_int), ("max_col", C.c_int), ("sell_slots", C.c_long)]
+                ("nlv_bwd", C.c_int), ("max_col", C.c_int), ("sell_slots", C.c_long),
+                ("tail_T", C.c_int), ("tail_updates", C.c_long)]
 
 
 class KernelStats(C.Structure):

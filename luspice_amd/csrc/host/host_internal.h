@@ -84,7 +84,24 @@ struct LspPlan {
     // scatter count (update FMAs) and per-column statistics
     int64_t n_updates = 0;
     int64_t n_l = 0, n_u = 0;             // off-diagonal counts
+    // Dense trailing block: positions [tail_k0, n) form a T x T block
+    // (T = n - tail_k0, 0 = none) factored densely after the sparse columns
+    // have been applied to it.  Set by lsp_plan_set_tail.
+    int tail_k0 = 0;
+    int tail_T = 0;
+    std::vector<int> tail_ds;             // T: U slots of tail column k0+c with pos < k0
+    std::vector<int> tail_lcut;           // T: first lci index of tail row k0+r with col >= k0
+    std::vector<int> fkey;                // n: ordering key of column k's refactor task
+    int64_t tail_updates = 0;             // updates moved into the dense block
 };
+
+// Chooses the dense trailing block: the largest T in [min_T, max_T] whose
+// pattern density nnz(block) / T^2 is at least min_density (no tail when none
+// qualifies or max_T <= 0).  Rebuilds ftask (sparse columns by level, tail
+// columns keyed after their sparse dependencies), fwd_task / bwd_task (rows
+// below k0 only, backward levels recomputed with the tail rows solved first)
+// and the tail maps.  Returns T.
+int lsp_plan_set_tail(LspPlan &P, int min_T, int max_T, double min_density);
 
 // Build the plan from A (q-order CSC, values only used for nothing here) and
 // the symbolic factors.  src_index maps each q-CSC entry to the caller's value

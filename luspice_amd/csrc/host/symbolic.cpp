@@ -298,5 +298,89 @@ int lsp_build_plan(int n, const int *Ap, const int *Ai, const int *src_index,
     P.nlv_b = db + 1;
     sort_by_level(P.lvf, P.nlv_f, P.fwd_task);
     sort_by_level(P.lvb, P.nlv_b, P.bwd_task);
+    P.tail_k0 = n;
+    P.tail_T = 0;
+    P.fkey = P.level;
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// dense trailing block
+// ---------------------------------------------------------------------------
+int lsp_plan_set_tail(LspPlan &P, int min_T, int max_T, double min_density)
+{
+    const int n = P.n;
+    // nz(k0) = pattern entries inside [k0, n) x [k0, n), grown one position at a
+    // time: column k0-1 adds its diagonal and L part (all below), row k0-1 adds
+    // its U entries (all to the right).
+    int best = 0;
+    int64_t nz = 0;
+    for (int T = 1; T <= std::min(n, max_T); T++) {
+        const int k = n - T;
+        nz += (P.cp[k + 1] - P.dp[k]) + (P.urp[k + 1] - P.urp[k]);
+        if (T >= min_T && (double)nz >= min_density * (double)T * (double)T) best = T;
+    }
+    const int k0 = n - best;
+    P.tail_T = best;
+    P.tail_k0 = k0;
+    P.tail_ds.assign(best, 0);
+    P.tail_lcut.assign(best, 0);
+    P.tail_updates = 0;
+    P.fkey = P.level;
+    if (best == 0) return 0;
+    for (int c = 0; c < best; c++) {
+        const int k = k0 + c;
+        int ds = 0;
+        int key = 0;
+        for (int64_t s = P.cp[k]; s < P.dp[k] && P.pos[s] < k0; s++) {
+            ds++;
+            key = std::max(key, P.level[P.pos[s]] + 1);
+        }
+        P.tail_ds[c] = ds;
+        P.fkey[k] = key;
+        for (int64_t s = P.cp[k] + ds; s < P.dp[k]; s++) {
+            const int j = P.pos[s];
+            P.tail_updates += P.cp[j + 1] - P.dp[j] - 1;
+        }
+        int e = P.lrp[k];
+        while (e < P.lrp[k + 1] && P.lci[e] < k0) e++;
+        P.tail_lcut[c] = e;
+    }
+    int nkey = 0;
+    for (int k = 0; k < n; k++) nkey = std::max(nkey, P.fkey[k] + 1);
+    sort_by_level(P.fkey, nkey, P.ftask);
+    // forward rows below k0 keep their levels (their L entries are all below k0)
+    std::vector<int> fw;
+    fw.reserve(k0);
+    for (int i : P.fwd_task)
+        if (i < k0) fw.push_back(i);
+    P.fwd_task.swap(fw);
+    int df = 0;
+    for (int i = 0; i < k0; i++) df = std::max(df, P.lvf[i]);
+    P.nlv_f = k0 > 0 ? df + 1 : 0;
+    // backward: tail rows are solved densely first; rows below k0 wait only
+    // on rows below k0
+    int db = 0;
+    for (int i = n - 1; i >= 0; i--) {
+        if (i >= k0) { P.lvb[i] = -1; continue; }
+        int lv = 0;
+        for (int e = P.urp[i]; e < P.urp[i + 1]; e++) {
+            const int c = P.uci[e];
+            if (c < k0) lv = std::max(lv, P.lvb[c] + 1);
+        }
+        P.lvb[i] = lv;
+        db = std::max(db, lv);
+    }
+    P.nlv_b = k0 > 0 ? db + 1 : 0;
+    std::vector<int> bw;
+    bw.reserve(k0);
+    {
+        std::vector<int> cnt(P.nlv_b + 1, 0);
+        for (int i = 0; i < k0; i++) cnt[P.lvb[i] + 1]++;
+        for (int l = 0; l < P.nlv_b; l++) cnt[l + 1] += cnt[l];
+        bw.assign(k0, 0);
+        for (int i = 0; i < k0; i++) bw[cnt[P.lvb[i]]++] = i;
+    }
+    P.bwd_task.swap(bw);
+    return best;
 }

@@ -34,7 +34,14 @@ struct FactorDesc {
     int *done;              // per-column epoch flags
     int *mstat;             // 4 ints: bad pivots, first bad column,
                             // first zero-pivot column, non-finite count
+    // dense trailing block (tail_T = 0: none, tail_k0 = n)
+    const int *tail_ds;     // tail_T: sparse dependencies of tail column c
+    double *tail_S;         // tail_T^2 fp64 work block, column-major
+    unsigned long long *tail_colmax;   // tail_T: max |L(i,c)| as bit patterns
+    T *tail_F;              // tail_T^2 factor copy in storage precision
     int n;
+    int tail_k0;
+    int tail_T;
     int pad_;
 };
 
@@ -49,8 +56,23 @@ struct TriDesc {
     const int *rowat;       // position -> original row (P)
     const int *q;           // position -> original column (Q)
     int *fdone, *bdone;     // per-row epoch flags
+    // dense trailing block: rows [tail_k0, n) are solved with tail_F
+    const T *tail_F;        // tail_T^2, column-major, unit-L below / U on and above
+    const int *tail_lcut;   // tail_T: end of the sparse part of L row tail_k0 + r
+    int *tail_flag;         // ceil(tail_T / 64) block flags (epoch-tagged)
+    int *tail_ctr;          // 2 ints: block ticket counter, exit counter
     int n;
+    int tail_k0;
+    int tail_T;
     int pad_;
+};
+
+// One dense trailing block to factor (fp64 work, fixed pivot order).
+struct DenseJob {
+    double *S;              // T x T, column-major, leading dimension T
+    unsigned long long *colmax;
+    int T;
+    int mat;                // matrix index (descriptor, mstat, active mask)
 };
 
 // SELL-C-sigma SpMV with C = 64 (one wave per slice, one lane per row).

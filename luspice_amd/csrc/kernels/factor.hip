@@ -66,9 +66,14 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, double *x,
     }
     wave_sync();
 
+    // A column of the dense trailing block applies only its dependencies
+    // outside the block (a prefix of its U slots); the block itself is
+    // factored densely afterwards.
+    const bool tail = k >= M.tail_k0;
+    const int ndep = tail ? M.tail_ds[k - M.tail_k0] : dl;
     int64_t u = M.relp[k];
-    for (int s0 = 0; s0 < dl; s0 += kWave) {
-        const int nd = min(kWave, dl - s0);
+    for (int s0 = 0; s0 < ndep; s0 += kWave) {
+        const int nd = min(kWave, ndep - s0);
         int l0 = 0, len = 0;
         const int *flag = nullptr;
         int jj = -1;
@@ -104,6 +109,23 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, double *x,
             u += jlen;
             wave_sync();
         }
+    }
+
+    if (tail) {
+        // U entries above the block are final; the rest of the column is the
+        // Schur-complement column S(:, c) = A22(:, c) - L21 U12(:, c).
+        const int c = k - M.tail_k0;
+        for (int i = lane; i < ndep; i += kWave) {
+            const T v = from_f64<T>(x[i]);
+            M.val[c0 + i] = v;
+            M.urow[M.dst[c0 + i]] = v;
+        }
+        double *Sc = M.tail_S + (size_t)c * (size_t)M.tail_T;
+        for (int r = lane; r < M.tail_T; r += kWave) Sc[r] = 0.0;
+        __builtin_amdgcn_s_waitcnt(0);          // zero stores complete before the scatter
+        for (int s = ndep + lane; s < m; s += kWave) Sc[M.pos[c0 + s] - M.tail_k0] = x[s];
+        if (lane == 0) M.tail_colmax[c] = 0ull;
+        return;
     }
 
     const double piv = x[dl];

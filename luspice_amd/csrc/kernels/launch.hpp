@@ -71,6 +71,17 @@ template <typename T> hipError_t launch_refactor(const FactorLaunch &L, int bloc
 template <typename T> hipError_t refactor_set_lds(int cap);
 hipError_t launch_reset_mstat(int *mstat, int nmat, int *abort_word, int *dbg, hipStream_t st);
 
+// dense trailing blocks (dense.hip)
+hipError_t dense_set_lds();
+hipError_t launch_dense_lu(const DenseJob *jobs, int njobs, int maxT, const int *active, hipStream_t st);
+template <typename T>
+hipError_t launch_dense_finish(const DenseJob *jobs, int njobs, int maxT, const void *mats,
+                               const int *active, double ptol, hipStream_t st);
+template <typename T>
+hipError_t launch_tail_solve(const void *mats, const int *tails, int ntails, int maxT, const long *voff,
+                             const int *active, const double *b, double *y, double *z, double *x,
+                             bool forward, int epoch, int *abort_word, hipStream_t st);
+
 // trisolve.hip
 template <typename T>
 hipError_t launch_trisolve_half(const TriLaunch &L, bool forward, int blocks, hipStream_t st);

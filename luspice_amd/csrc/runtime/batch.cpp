@@ -201,6 +201,11 @@ static int build_circuit(CircuitHost &C, const lsp_circuit_in &in, const lsp_opt
     if (lsp_build_plan(C.n, Ap.data(), Ai.data(), src.data(), C.sym, C.q.data(), C.plan) != 0)
         return LSP_E_STRUCTURE;
     if (C.plan.max_col > 65535) return LSP_E_CAPACITY;
+    {
+        const int tmax = opt.tail_max == 0 ? 8192 : opt.tail_max;
+        const double tden = opt.tail_density > 0 ? opt.tail_density : 0.3;
+        if (tmax > 0) lsp_plan_set_tail(C.plan, 64, tmax, tden);
+    }
     std::vector<int> rp, ci, rsrc;
     lsp_coo_to_csr(C.n, C.nnz, C.row.data(), C.col.data(), rp, ci, rsrc);
     if (lsp_build_sell(C.n, rp.data(), ci.data(), rsrc.data(), 64, opt.sell_sigma, C.sell) != 0)
@@ -216,7 +221,8 @@ int Batch::ensure_factor_set(FactorSet<T> &fs)
     fs.lrow = (T *)dalloc(nl_tot_ * sizeof(T));
     fs.urow = (T *)dalloc(nu_tot_ * sizeof(T));
     fs.diag = (T *)dalloc(ntot_ * sizeof(T));
-    if (!fs.val || !fs.lrow || !fs.urow || !fs.diag) return LSP_E_ALLOC;
+    fs.tailF = (T *)dalloc(nt2_tot_ * sizeof(T));
+    if (!fs.val || !fs.lrow || !fs.urow || !fs.diag || !fs.tailF) return LSP_E_ALLOC;
     const int nc = (int)circ_.size();
     std::vector<FactorDesc<T>> fd(nc);
     std::vector<TriDesc<T>> td(nc);
@@ -240,6 +246,12 @@ int Batch::ensure_factor_set(FactorSet<T> &fs)
         F.done = d_done_ + C.o_n;
         F.mstat = d_mstat_ + 4 * m;
         F.n = C.n;
+        F.tail_ds = d_tail_ds_ + C.o_t;
+        F.tail_S = d_tailS_ + C.o_t2;
+        F.tail_colmax = d_colmax_ + C.o_t;
+        F.tail_F = fs.tailF + C.o_t2;
+        F.tail_k0 = C.plan.tail_k0;
+        F.tail_T = C.plan.tail_T;
         TriDesc<T> &D = td[m];
         D.lrp = d_lrp_ + C.o_np1;
         D.lci = d_lci_ + C.o_l;
@@ -253,6 +265,12 @@ int Batch::ensure_factor_set(FactorSet<T> &fs)
         D.fdone = d_fdone_ + C.o_n;
         D.bdone = d_bdone_ + C.o_n;
         D.n = C.n;
+        D.tail_F = fs.tailF + C.o_t2;
+        D.tail_lcut = d_tail_lcut_ + C.o_t;
+        D.tail_flag = d_tail_flag_ + C.o_tb;
+        D.tail_ctr = d_tail_ctr_ + 2 * m;
+        D.tail_k0 = C.plan.tail_k0;
+        D.tail_T = C.plan.tail_T;
     }
     fs.fdesc = upload(fd);
     fs.tdesc = upload(td);
@@ -299,10 +317,13 @@ int Batch::create(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
 
     // ---- concatenate structures ----
     long o_np1 = 0, o_n = 0, o_slot = 0, o_a = 0, o_rel = 0, o_l = 0, o_u = 0, o_sl = 0,
-         o_slp1 = 0, o_ss = 0;
+         o_slp1 = 0, o_ss = 0, o_t = 0, o_t2 = 0, o_tb = 0;
     for (auto &C : circ_) {
         C.o_np1 = o_np1; C.o_n = o_n; C.o_slot = o_slot; C.o_a = o_a; C.o_rel = o_rel;
         C.o_l = o_l; C.o_u = o_u; C.o_sl = o_sl; C.o_slp1 = o_slp1; C.o_ss = o_ss;
+        C.o_t = o_t; C.o_t2 = o_t2; C.o_tb = o_tb;
+        const long T = C.plan.tail_T;
+        o_t += T; o_t2 += T * T; o_tb += (T + 63) / 64;
         C.voff = o_n;
         C.aoff = o_a;
         o_np1 += C.n + 1; o_n += C.n; o_slot += C.plan.cp[C.n]; o_a += C.nnz;
@@ -311,6 +332,7 @@ int Batch::create(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
     }
     ntot_ = o_n; nnz_tot_ = o_a; nslot_tot_ = o_slot; nl_tot_ = o_l; nu_tot_ = o_u;
     nsell_tot_ = o_ss;
+    nt_tot_ = o_t; nt2_tot_ = o_t2; ntb_tot_ = o_tb;
     {
         std::vector<int64_t> cp, relp, soff;
         std::vector<int> dp, pos, aptr, asrc, aslot, dst, lrp, lci, urp, uci, rowat, q;
@@ -372,6 +394,44 @@ int Batch::create(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
         d_dbg_ = upload(z6);
         if (!d_done_ || !d_fdone_ || !d_bdone_ || !d_ctr_ || !d_abort_ || !d_mstat_) return LSP_E_ALLOC;
     }
+    // dense trailing blocks
+    {
+        std::vector<int> tds, tlc, tails;
+        std::vector<DenseJob> jobs;
+        d_tailS_ = (double *)dalloc(nt2_tot_ * sizeof(double));
+        d_colmax_ = (unsigned long long *)dalloc(nt_tot_ * sizeof(unsigned long long));
+        if (!d_tailS_ || !d_colmax_) return LSP_E_ALLOC;
+        for (int m = 0; m < nc; m++) {
+            const CircuitHost &C = circ_[m];
+            const LspPlan &P = C.plan;
+            tds.insert(tds.end(), P.tail_ds.begin(), P.tail_ds.end());
+            tlc.insert(tlc.end(), P.tail_lcut.begin(), P.tail_lcut.end());
+            if (P.tail_T > 0) {
+                DenseJob J{};
+                J.S = d_tailS_ + C.o_t2;
+                J.colmax = d_colmax_ + C.o_t;
+                J.T = P.tail_T;
+                J.mat = m;
+                jobs.push_back(J);
+                tails.push_back(m);
+                max_tail_T_ = std::max(max_tail_T_, P.tail_T);
+            }
+        }
+        njobs_ = (int)jobs.size();
+        d_tail_ds_ = upload(tds);
+        d_tail_lcut_ = upload(tlc);
+        d_djobs_ = upload(jobs);
+        d_tails_ = upload(tails);
+        std::vector<int> zb(ntb_tot_, 0), zc(2 * nc, 0);
+        d_tail_flag_ = upload(zb);
+        d_tail_ctr_ = upload(zc);
+        if (!d_tail_ds_ || !d_tail_lcut_ || !d_djobs_ || !d_tails_ || !d_tail_flag_ || !d_tail_ctr_)
+            return LSP_E_ALLOC;
+        if (njobs_) {
+            rc = check(dense_set_lds(), "dense lds attr");
+            if (rc) return rc;
+        }
+    }
     d_ain_ = (double *)dalloc(nnz_tot_ * sizeof(double));
     d_sval64_ = (double *)dalloc(nsell_tot_ * sizeof(double));
     d_sval32_ = (float *)dalloc(nsell_tot_ * sizeof(float));
@@ -429,7 +489,7 @@ int Batch::create(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
             }
             return out;
         };
-        auto tf = merge([](const LspPlan &P, int k) { return P.level[k]; },
+        auto tf = merge([](const LspPlan &P, int k) { return P.fkey[k]; },
                         [](const LspPlan &P) -> const std::vector<int> & { return P.ftask; },
                         width_f_);
         auto tw = merge([](const LspPlan &P, int k) { return P.lvf[k]; },
@@ -438,6 +498,8 @@ int Batch::create(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
         auto tb = merge([](const LspPlan &P, int k) { return P.lvb[k]; },
                         [](const LspPlan &P) -> const std::vector<int> & { return P.bwd_task; },
                         width_tb_);
+        nfw_tot_ = (long)tw.size();
+        nbw_tot_ = (long)tb.size();
         d_ftask_ = upload(tf); d_fwtask_ = upload(tw); d_bwtask_ = upload(tb);
         if (!d_ftask_ || !d_fwtask_ || !d_bwtask_) return LSP_E_ALLOC;
     }
@@ -597,6 +659,12 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
     if (rc) return rc;
     (void)hipEventRecord(ev_[0], stream_);
     rc = check(launch_refactor<T>(L, fblocks_, stream_), "refactor launch");
+    if (!rc && njobs_) {
+        rc = check(launch_dense_lu(d_djobs_, njobs_, max_tail_T_, d_active, stream_), "dense lu");
+        if (!rc)
+            rc = check(launch_dense_finish<T>(d_djobs_, njobs_, max_tail_T_, fs.fdesc, d_active,
+                                              opt_.refactor_ptol, stream_), "dense finish");
+    }
     (void)hipEventRecord(ev_[1], stream_);
     return rc;
 }
@@ -704,19 +772,29 @@ int Batch::trisolve_t(FactorSet<T> &fs, const double *d_b, double *d_x, const in
     Lf.x = d_x;
     Lf.y = d_ty_;
     Lf.z = d_tz_;
-    Lf.ntask = (int)ntot_;
+    Lf.ntask = (int)nfw_tot_;
     Lf.epoch = next_epoch();
     Lf.ctr = d_ctr_ + 2;
     Lf.abort_word = d_abort_ + 1;
     Lf.total_waves = tblocks_f_ * 4;
     TriLaunch Lb = Lf;
     Lb.task = d_bwtask_;
+    Lb.ntask = (int)nbw_tot_;
     Lb.epoch = next_epoch();
     Lb.ctr = d_ctr_ + 4;
     Lb.total_waves = tblocks_b_ * 4;
+    const int tail_epoch = next_epoch();
     (void)hipEventRecord(ev_[2], stream_);
     int rc = check(launch_trisolve_half<T>(Lf, true, tblocks_f_, stream_), "trisolve fwd");
+    if (!rc && njobs_)
+        rc = check(launch_tail_solve<T>(fs.tdesc, d_tails_, njobs_, max_tail_T_, d_voff_, d_active, d_b,
+                                        d_ty_, d_tz_, d_x, true, tail_epoch, d_abort_ + 1, stream_),
+                   "tail fwd");
     (void)hipEventRecord(ev_[4], stream_);
+    if (!rc && njobs_)
+        rc = check(launch_tail_solve<T>(fs.tdesc, d_tails_, njobs_, max_tail_T_, d_voff_, d_active, d_b,
+                                        d_ty_, d_tz_, d_x, false, Lb.epoch, d_abort_ + 1, stream_),
+                   "tail bwd");
     if (!rc) rc = check(launch_trisolve_half<T>(Lb, false, tblocks_b_, stream_), "trisolve bwd");
     (void)hipEventRecord(ev_[3], stream_);
     tri_pending_ = true;
@@ -967,6 +1045,8 @@ int Batch::info(int c, lsp_circuit_info *inf) const
     inf->nlv_bwd = C.plan.nlv_b;
     inf->max_col = C.plan.max_col;
     inf->sell_slots = C.sell.slots;
+    inf->tail_T = C.plan.tail_T;
+    inf->tail_updates = C.plan.tail_updates;
     return LSP_OK;
 }
 

@@ -37,11 +37,13 @@ struct CircuitHost {
     long o_rel = 0;                      // update map
     long o_l = 0, o_u = 0;               // strict L / U entry arrays
     long o_sl = 0, o_slp1 = 0, o_ss = 0; // SELL slices, slices+1, slots
+    long o_t = 0, o_t2 = 0, o_tb = 0;    // dense block: T, T^2, 64-row blocks
 };
 
 template <typename T>
 struct FactorSet {                       // per storage precision
     T *val = nullptr, *lrow = nullptr, *urow = nullptr, *diag = nullptr;
+    T *tailF = nullptr;                  // dense trailing blocks, storage precision
     FactorDesc<T> *fdesc = nullptr;      // device
     TriDesc<T> *tdesc = nullptr;         // device
     bool ready = false;                  // arrays allocated
@@ -130,6 +132,15 @@ private:
     long scratch_stride_ = 0;
     int fblocks_ = 0, tblocks_f_ = 0, tblocks_b_ = 0;
     long width_f_ = 0, width_tf_ = 0, width_tb_ = 0;   // widest merged level
+    // dense trailing blocks
+    int *d_tail_ds_ = nullptr, *d_tail_lcut_ = nullptr, *d_tail_flag_ = nullptr,
+        *d_tail_ctr_ = nullptr, *d_tails_ = nullptr;
+    double *d_tailS_ = nullptr;
+    unsigned long long *d_colmax_ = nullptr;
+    DenseJob *d_djobs_ = nullptr;
+    int njobs_ = 0, max_tail_T_ = 0;
+    long nt_tot_ = 0, nt2_tot_ = 0, ntb_tot_ = 0;
+    long nfw_tot_ = 0, nbw_tot_ = 0;     // row tasks of the sparse solves
     FactorSet<float> f32_;
     FactorSet<double> f64_;
     FactorSet<__half> f16_;
