@@ -16,6 +16,7 @@
 #ifndef LUSPICE_H
 #define LUSPICE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -106,6 +107,17 @@ int  lsp_options_default(lsp_options *opt);
 
 int  lsp_batch_create(const lsp_circuit_in *circuits, int ncircuits,
                       const lsp_options *opt, lsp_batch **out);
+/* Two-phase creation for callers that admit work by device memory:
+ * lsp_batch_prepare runs the host symbolic phase only (no device calls),
+ * lsp_batch_estimate_bytes gives the device memory the batch will hold
+ * (an upper estimate including fp32 and fp64 factor sets), and
+ * lsp_batch_upload allocates and uploads.  lsp_batch_create = both. */
+int  lsp_batch_prepare(const lsp_circuit_in *circuits, int ncircuits,
+                       const lsp_options *opt, lsp_batch **out);
+int  lsp_batch_estimate_bytes(const lsp_batch *b, size_t *bytes);
+int  lsp_batch_upload(lsp_batch *b);
+/* Device memory currently held by the batch. */
+int  lsp_batch_device_bytes(const lsp_batch *b, size_t *bytes);
 void lsp_batch_destroy(lsp_batch *b);
 int  lsp_batch_set_stream(lsp_batch *b, void *hip_stream);
 int  lsp_batch_info(const lsp_batch *b, int circuit, lsp_circuit_info *info);
@@ -131,6 +143,11 @@ int  lsp_batch_export_factors(lsp_batch *b, int circuit, int *pinv,
                               int *Up, int *Ui, double *Ux);
 /* Device pointers of the vector arenas (length total_n) for zero-copy use. */
 int  lsp_batch_device_vectors(lsp_batch *b, double **rhs, double **x);
+/* Copies total_n doubles into the resident right-hand-side arena / out of
+ * the resident solution arena, ordered on the batch's stream (src / dst may
+ * be host or device memory; the call returns when the copy is complete). */
+int  lsp_batch_set_rhs(lsp_batch *b, const double *rhs);
+int  lsp_batch_get_x(lsp_batch *b, double *x);
 /* Device time of the last refactor / trisolve launches (ms, hipEvent). */
 int  lsp_batch_last_times(const lsp_batch *b, double *refactor_ms,
                           double *fwd_ms, double *bwd_ms);

@@ -143,7 +143,9 @@ class Batch:
     sequence, L/U pattern, level sets, SELL layout) and uploads everything;
     refactor()/solve() then run on the GPU."""
 
-    def __init__(self, circuits, **opts):
+    def __init__(self, circuits, upload=True, **opts):
+        """upload=False runs only the host symbolic phase; call upload() to
+        allocate and upload (estimate_bytes() tells how much it will take)."""
         if isinstance(circuits, Circuit):
             circuits = [circuits]
         self.circuits = list(circuits)
@@ -157,14 +159,30 @@ class Batch:
             arr[i].q = N.ptr(c.q)
         self.opts = options(**opts)
         h = N.C.c_void_p()
-        rc = N.lib.lsp_batch_create(arr, len(self.circuits), N.C.byref(self.opts), N.C.byref(h))
-        check(rc, "lsp_batch_create")
+        rc = N.lib.lsp_batch_prepare(arr, len(self.circuits), N.C.byref(self.opts), N.C.byref(h))
+        check(rc, "lsp_batch_prepare")
         self._h = h
-        tn, tz = N.C.c_long(), N.C.c_long()
-        N.lib.lsp_batch_sizes(h, N.C.byref(tn), N.C.byref(tz))
-        self.total_n, self.total_nnz = tn.value, tz.value
+        self.total_n = sum(c.n for c in self.circuits)
+        self.total_nnz = sum(c.nnz for c in self.circuits)
+        self.uploaded = False
+        if upload:
+            self.upload()
         self.voff = np.cumsum([0] + [c.n for c in self.circuits])
         self.aoff = np.cumsum([0] + [c.nnz for c in self.circuits])
+
+    def upload(self):
+        check(N.lib.lsp_batch_upload(self._h), "lsp_batch_upload")
+        self.uploaded = True
+
+    def estimate_bytes(self) -> int:
+        v = N.C.c_size_t()
+        check(N.lib.lsp_batch_estimate_bytes(self._h, N.C.byref(v)), "estimate_bytes")
+        return v.value
+
+    def device_bytes(self) -> int:
+        v = N.C.c_size_t()
+        check(N.lib.lsp_batch_device_bytes(self._h, N.C.byref(v)), "device_bytes")
+        return v.value
 
     def close(self):
         if getattr(self, "_h", None):
@@ -245,6 +263,21 @@ class Batch:
             self._h, i, N.ptr(out["pinv"]), N.ptr(out["Lp"]), N.ptr(out["Li"]),
             N.ptr(out["Lx"]), N.ptr(out["Up"]), N.ptr(out["Ui"]), N.ptr(out["Ux"])), "export")
         return out
+
+    def set_rhs(self, src):
+        """Copy total_n doubles into the resident RHS arena: src is a numpy
+        array or a device address (int)."""
+        if isinstance(src, np.ndarray):
+            src = np.ascontiguousarray(src, np.float64)
+            assert src.size == self.total_n
+            check(N.lib.lsp_batch_set_rhs(self._h, src.ctypes.data_as(N.C.c_void_p)), "set_rhs")
+        else:
+            check(N.lib.lsp_batch_set_rhs(self._h, N.C.c_void_p(int(src))), "set_rhs")
+
+    def get_x(self) -> np.ndarray:
+        x = np.empty(self.total_n)
+        check(N.lib.lsp_batch_get_x(self._h, x.ctypes.data_as(N.C.c_void_p)), "get_x")
+        return x
 
     def device_vectors(self):
         b, x = N.C.c_void_p(), N.C.c_void_p()

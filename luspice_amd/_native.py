@@ -77,6 +77,11 @@ _sig("lsp_options_default", C.c_int, C.POINTER(Options))
 _sig("lsp_batch_create", C.c_int, C.POINTER(CircuitIn), C.c_int, C.POINTER(Options),
      C.POINTER(C.c_void_p))
 _sig("lsp_batch_destroy", None, C.c_void_p)
+_sig("lsp_batch_prepare", C.c_int, C.POINTER(CircuitIn), C.c_int, C.POINTER(Options),
+     C.POINTER(C.c_void_p))
+_sig("lsp_batch_estimate_bytes", C.c_int, C.c_void_p, C.POINTER(C.c_size_t))
+_sig("lsp_batch_upload", C.c_int, C.c_void_p)
+_sig("lsp_batch_device_bytes", C.c_int, C.c_void_p, C.POINTER(C.c_size_t))
 _sig("lsp_batch_set_stream", C.c_int, C.c_void_p, C.c_void_p)
 _sig("lsp_batch_info", C.c_int, C.c_void_p, C.c_int, C.POINTER(CircuitInfo))
 _sig("lsp_batch_sizes", C.c_int, C.c_void_p, c_long_p, c_long_p)
@@ -90,6 +95,8 @@ _sig("lsp_batch_export_factors", C.c_int, C.c_void_p, C.c_int, c_int_p, c_int_p,
      c_double_p, c_int_p, c_int_p, c_double_p)
 _sig("lsp_batch_device_vectors", C.c_int, C.c_void_p, C.POINTER(C.c_void_p),
      C.POINTER(C.c_void_p))
+_sig("lsp_batch_set_rhs", C.c_int, C.c_void_p, C.c_void_p)
+_sig("lsp_batch_get_x", C.c_int, C.c_void_p, C.c_void_p)
 _sig("lsp_batch_last_times", C.c_int, C.c_void_p, c_double_p, c_double_p, c_double_p)
 _sig("lsp_batch_last_error", C.c_char_p, C.c_void_p)
 _sig("lsp_batch_kernel_stats", C.c_int, C.c_void_p, C.POINTER(KernelStats), C.c_int)

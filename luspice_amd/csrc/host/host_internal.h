@@ -64,7 +64,7 @@ struct LspPlan {
     // left-looking update map: for column k, for each U slot (dep j,
     // ascending), for each L slot t of column j: local slot of pos[t] in k
     std::vector<int64_t> relp;            // n+1
-    std::vector<int> rel;                 // total updates
+    std::vector<uint16_t> rel;            // local slots (max_col <= 65535)
     int max_col = 0;                      // max column pattern length
     // dependency levels of the factorization (U pattern, level.c semantics)
     std::vector<int> level;               // n

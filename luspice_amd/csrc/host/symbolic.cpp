@@ -169,6 +169,7 @@ int lsp_build_plan(int n, const int *Ap, const int *Ai, const int *src_index,
         P.cp[k + 1] = P.cp[k] + nu + 1 + nl;
         P.max_col = std::max(P.max_col, nu + 1 + nl);
     }
+    if (P.max_col > 65535) return -9;            // local slots are 16-bit
     const int64_t nslots = P.cp[n];
     P.pos.assign(nslots, 0);
     for (int k = 0; k < n; k++) {
@@ -233,7 +234,7 @@ int lsp_build_plan(int n, const int *Ap, const int *Ai, const int *src_index,
             for (int64_t t = P.dp[j] + 1; t < P.cp[j + 1]; t++) {
                 int ps = P.pos[t];
                 if (owner[ps] != k) return -8;     // fill outside the pattern
-                P.rel[u++] = where[ps];
+                P.rel[u++] = (uint16_t)where[ps];
             }
         }
     }
@@ -345,6 +346,28 @@ int lsp_plan_set_tail(LspPlan &P, int min_T, int max_T, double min_density)
         int e = P.lrp[k];
         while (e < P.lrp[k + 1] && P.lci[e] < k0) e++;
         P.tail_lcut[c] = e;
+    }
+    // the update map of a block column keeps only its dependencies outside
+    // the block (a prefix of the column's entries)
+    {
+        std::vector<int64_t> nrelp(n + 1, 0);
+        for (int k = 0; k < n; k++) {
+            int64_t cnt = P.relp[k + 1] - P.relp[k];
+            if (k >= k0) {
+                cnt = 0;
+                for (int s = 0; s < P.tail_ds[k - k0]; s++) {
+                    const int j = P.pos[P.cp[k] + s];
+                    cnt += P.cp[j + 1] - P.dp[j] - 1;
+                }
+            }
+            nrelp[k + 1] = nrelp[k] + cnt;
+        }
+        std::vector<uint16_t> nrel((size_t)nrelp[n]);
+        for (int k = 0; k < n; k++)
+            std::copy(P.rel.begin() + P.relp[k], P.rel.begin() + P.relp[k] + (nrelp[k + 1] - nrelp[k]),
+                      nrel.begin() + nrelp[k]);
+        P.rel.swap(nrel);
+        P.relp.swap(nrelp);
     }
     int nkey = 0;
     for (int k = 0; k < n; k++) nkey = std::max(nkey, P.fkey[k] + 1);

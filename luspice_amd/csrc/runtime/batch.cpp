@@ -281,20 +281,15 @@ int Batch::ensure_factor_set(FactorSet<T> &fs)
 
 int Batch::create(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
 {
-    if (!cin || nc <= 0) return LSP_E_ARG;
-    if (opt) opt_ = *opt; else lsp_options_default(&opt_);
-    int dev = 0;
-    hipDeviceProp_t prop;
-    int rc = check(hipGetDevice(&dev), "hipGetDevice");
-    if (rc) return rc;
-    rc = check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
-    if (rc) return rc;
-    cu_count_ = prop.multiProcessorCount;
-    rc = check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream");
-    if (rc) return rc;
-    own_stream_ = true;
+    int rc = prepare(cin, nc, opt);
+    return rc ? rc : upload_all();
+}
 
-    // ---- host symbolic phase, circuits in parallel ----
+// Host phase: symbolic analysis of every circuit (no device calls).
+int Batch::prepare(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
+{
+    if (!cin || nc <= 0 || !circ_.empty()) return LSP_E_ARG;
+    if (opt) opt_ = *opt; else lsp_options_default(&opt_);
     circ_.resize(nc);
     std::vector<int> rcs(nc, 0);
     {
@@ -311,9 +306,63 @@ int Batch::create(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
     for (int m = 0; m < nc; m++) {
         if (rcs[m] != LSP_OK) {
             last_error = "circuit " + std::to_string(m) + ": symbolic phase failed";
+            circ_.clear();
             return rcs[m];
         }
     }
+    return LSP_OK;
+}
+
+// Device bytes upload_all() and the first fp32 + fp64 refactors will hold
+// (an upper estimate used for admission; device_bytes() reports the actual).
+size_t Batch::estimate_device_bytes() const
+{
+    double n = 0, nnz = 0, slots = 0, rel = 0, nlu = 0, sell = 0, slices = 0, T = 0, T2 = 0;
+    double maxcol = 0;
+    const double nc = (double)circ_.size();
+    for (const auto &C : circ_) {
+        const LspPlan &P = C.plan;
+        n += C.n; nnz += C.nnz; slots += (double)P.cp[C.n]; rel += (double)P.relp[C.n];
+        nlu += (double)(P.n_l + P.n_u); sell += (double)C.sell.slots; slices += C.sell.nslices;
+        T += P.tail_T; T2 += (double)P.tail_T * P.tail_T;
+        maxcol = std::max(maxcol, (double)P.max_col);
+    }
+    const double m = std::max(1, opt_.ir_inner);
+    double b = 0;
+    b += 16 * (n + nc) + 4 * n + 8 * slots + 4 * (n + nc) + 8 * nnz;     // cp relp dp pos dst aptr asrc aslot
+    b += 8 * (n + nc) + 4 * nlu + 8 * n + 2 * rel;                        // lrp urp lci uci rowat q rel
+    b += 8 * (slices + nc) + 8 * n + 8 * sell;                            // SELL structure
+    b += 12 * n + 8 * nnz + 12 * sell;                                    // flags, values
+    b += 8 * T2 + 16 * T;                                                  // dense block work
+    b += 24 * n + 8 * slices;                                             // task lists
+    b += 64 * n + 8 * (m + 1) * n;                                        // vectors, Krylov basis
+    b += (4 + 8) * (slots + nlu + n + T2);                                 // fp32 + fp64 factor sets
+    if (maxcol > std::max(1, opt_.lds_cap)) b += 256.0 * 8 * 4 * maxcol * 8;   // global column scratch
+    return (size_t)(1.05 * b) + (1u << 20);
+}
+
+size_t Batch::device_bytes() const
+{
+    size_t s = 0;
+    for (const auto &mm : mem_) s += mm.bytes;
+    return s;
+}
+
+// Device phase: concatenates the circuits' structures and uploads them.
+int Batch::upload_all()
+{
+    if (circ_.empty() || stream_) return LSP_E_ARG;
+    const int nc = (int)circ_.size();
+    int dev = 0;
+    hipDeviceProp_t prop;
+    int rc = check(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    rc = check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    if (rc) return rc;
+    cu_count_ = prop.multiProcessorCount;
+    rc = check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream");
+    if (rc) return rc;
+    own_stream_ = true;
 
     // ---- concatenate structures ----
     long o_np1 = 0, o_n = 0, o_slot = 0, o_a = 0, o_rel = 0, o_l = 0, o_u = 0, o_sl = 0,
@@ -327,7 +376,7 @@ int Batch::create(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
         C.voff = o_n;
         C.aoff = o_a;
         o_np1 += C.n + 1; o_n += C.n; o_slot += C.plan.cp[C.n]; o_a += C.nnz;
-        o_rel += C.plan.n_updates; o_l += C.plan.n_l; o_u += C.plan.n_u;
+        o_rel += C.plan.relp[C.n]; o_l += C.plan.n_l; o_u += C.plan.n_u;
         o_sl += C.sell.nslices; o_slp1 += C.sell.nslices + 1; o_ss += C.sell.slots;
     }
     ntot_ = o_n; nnz_tot_ = o_a; nslot_tot_ = o_slot; nl_tot_ = o_l; nu_tot_ = o_u;
@@ -355,7 +404,7 @@ int Batch::create(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
             uci.insert(uci.end(), P.uci.begin(), P.uci.end());
             rowat.insert(rowat.end(), P.rowat.begin(), P.rowat.end());
             q.insert(q.end(), P.q.begin(), P.q.end());
-            for (int r : P.rel) rel.push_back((uint16_t)r);
+            rel.insert(rel.end(), P.rel.begin(), P.rel.end());
             const LspSell &S = C.sell;
             soff.insert(soff.end(), S.off.begin(), S.off.end());
             std::vector<int> rl(C.n);
@@ -1104,6 +1153,14 @@ int Batch::device_vectors(double **b, double **x)
     if (b) *b = d_b_;
     if (x) *x = d_x_;
     return LSP_OK;
+}
+
+int Batch::copy_vector(const double *p, bool to_rhs)
+{
+    int rc = to_rhs ? check(hipMemcpyAsync(d_b_, p, ntot_ * 8, hipMemcpyDefault, stream_), "rhs")
+                    : check(hipMemcpyAsync((void *)p, d_x_, ntot_ * 8, hipMemcpyDefault, stream_), "x");
+    if (!rc) rc = check(hipStreamSynchronize(stream_), "sync");
+    return rc;
 }
 
 int Batch::kernel_stats(lsp_kernel_stats *st, int reset)

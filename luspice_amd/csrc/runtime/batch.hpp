@@ -53,6 +53,10 @@ class Batch {
 public:
     ~Batch();
     int create(const lsp_circuit_in *c, int nc, const lsp_options *opt);
+    int prepare(const lsp_circuit_in *c, int nc, const lsp_options *opt);   // host symbolic only
+    int upload_all();                                                         // device phase
+    size_t estimate_device_bytes() const;
+    size_t device_bytes() const;
     int set_stream(hipStream_t s);
     int set_values(const double *vals, bool on_device);
     int refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *subset = nullptr);
@@ -63,6 +67,7 @@ public:
     int export_factors(int c, int *pinv, int *Lp, int *Li, double *Lx, int *Up, int *Ui,
                        double *Ux);
     int device_vectors(double **b, double **x);
+    int copy_vector(const double *p, bool to_rhs);
     int last_times(double *rf, double *fw, double *bw) const;
     int kernel_stats(lsp_kernel_stats *st, int reset);
     int set_ir_params(int outer, int inner, double inner_tol, double outer_tol);

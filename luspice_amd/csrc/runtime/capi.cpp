@@ -120,6 +120,51 @@ int lsp_batch_device_vectors(lsp_batch *b, double **rhs, double **x)
     return b ? b->impl.device_vectors(rhs, x) : LSP_E_ARG;
 }
 
+int lsp_batch_set_rhs(lsp_batch *b, const double *rhs)
+{
+    return b && rhs ? b->impl.copy_vector(rhs, true) : LSP_E_ARG;
+}
+
+int lsp_batch_get_x(lsp_batch *b, double *x)
+{
+    return b && x ? b->impl.copy_vector(x, false) : LSP_E_ARG;
+}
+
+int lsp_batch_prepare(const lsp_circuit_in *circuits, int ncircuits, const lsp_options *opt,
+                      lsp_batch **out)
+{
+    if (!out) return LSP_E_ARG;
+    *out = nullptr;
+    lsp_batch *b = new (std::nothrow) lsp_batch();
+    if (!b) return LSP_E_ALLOC;
+    int rc = b->impl.prepare(circuits, ncircuits, opt);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return LSP_OK;
+}
+
+int lsp_batch_estimate_bytes(const lsp_batch *b, size_t *bytes)
+{
+    if (!b || !bytes) return LSP_E_ARG;
+    *bytes = b->impl.estimate_device_bytes();
+    return LSP_OK;
+}
+
+int lsp_batch_upload(lsp_batch *b)
+{
+    return b ? b->impl.upload_all() : LSP_E_ARG;
+}
+
+int lsp_batch_device_bytes(const lsp_batch *b, size_t *bytes)
+{
+    if (!b || !bytes) return LSP_E_ARG;
+    *bytes = b->impl.device_bytes();
+    return LSP_OK;
+}
+
 int lsp_batch_last_times(const lsp_batch *b, double *rf, double *fw, double *bw)
 {
     return b ? b->impl.last_times(rf, fw, bw) : LSP_E_ARG;

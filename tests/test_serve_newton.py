@@ -1,0 +1,89 @@
+"""Batched Newton driver and the serving loop.
+
+GPU tests: every result is compared with the fp64 oracle (rel err <= 1e-6),
+a request's per-step solutions are the same alone and in a concurrent mix,
+and admission never holds more estimated device memory than the budget.
+CPU test: the request generator is seeded and Poisson-like.
+"""
+import numpy as np
+import pytest
+
+import luspice_amd as L
+from luspice_amd import serve as S
+from luspice_amd import workloads as W
+from oracle import oracle as O
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_request_generator_seeded():
+    a = S.make_requests(12, rate=500.0, nmin=200, nmax=2000, seed=4)
+    b = S.make_requests(12, rate=500.0, nmin=200, nmax=2000, seed=4)
+    assert [r.circuit.n for r in a] == [r.circuit.n for r in b]
+    assert [r.arrival for r in a] == [r.arrival for r in b]
+    assert all(200 <= r.circuit.n <= 2000 for r in a)
+    assert all(5 <= r.steps <= 50 for r in a)
+    assert np.all(np.diff([r.arrival for r in a]) >= 0) and a[0].arrival == 0.0
+    va, ba = a[3].step_inputs(2)
+    vb, bb = b[3].step_inputs(2)
+    assert np.array_equal(va, vb) and np.array_equal(ba, bb)
+
+
+@pytest.mark.gpu
+def test_newton_batched_matches_oracle():
+    print("case: 24-circuit Newton batch, 3 steps", flush=True)
+    circ = W.newton_mix(24, 100, 3000, seed=3)
+    NB = W.NewtonBatched(circ, 3, seed=3)
+    aoff = np.cumsum([0] + [c.nnz for c in circ])
+    voff = np.cumsum([0] + [c.n for c in circ])
+    for s in range(3):
+        rr, sr = NB.run_step(s)
+        assert rr["rc"] == 0 and sr["rc"] == 0, (rr, sr)
+        x = NB.solution()
+        vals, rhs = NB.host[s]
+        for i, c in enumerate(circ):
+            rc, xr, _, _ = O.cpu_lu_ref(c.n, c.row, c.col, vals[aoff[i]:aoff[i + 1]], c.q,
+                                         rhs[voff[i]:voff[i + 1]])
+            assert _rel(x[voff[i]:voff[i + 1]], xr) <= 1e-6, (s, i, c.n)
+
+
+@pytest.mark.gpu
+def test_serve_same_alone_and_mixed():
+    print("case: 6 concurrent requests vs alone", flush=True)
+    reqs = S.make_requests(6, rate=1e6, nmin=200, nmax=3000, smin=3, smax=4, seed=4)
+    srv = S.Server(8 << 30, slots=6, keep_results=True)
+    st = srv.run(reqs)
+    srv.close()
+    assert st["errors"] == 0 and st["solves"] == sum(r.steps for r in reqs), st
+    for r in reqs:
+        alone = S.Request(r.rid, 0.0, r.circuit, r.steps, r.seed)
+        s1 = S.Server(8 << 30, slots=1, keep_results=True)
+        s1.run([alone])
+        s1.close()
+        assert len(alone.results) == r.steps
+        for a, b in zip(r.results, alone.results):
+            assert _rel(a, b) <= 1e-12
+        vals, rhs = r.step_inputs(r.steps - 1)
+        c = r.circuit
+        rc, xr, _, _ = O.cpu_lu_ref(c.n, c.row, c.col, vals, c.q, rhs)
+        assert _rel(r.results[-1], xr) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_serve_admission_respects_budget():
+    print("case: admission under a tight HBM budget", flush=True)
+    reqs = S.make_requests(5, rate=1e6, nmin=1000, nmax=2000, smin=2, smax=3, seed=5)
+    est = []
+    for r in reqs:
+        c = r.circuit
+        B = L.Batch(L.Circuit(c.n, c.row, c.col, c.val, c.rhs, q=c.q), upload=False)
+        est.append(B.estimate_bytes())
+        B.close()
+    budget = int(max(est) * 1.5)              # never room for two of the larger ones
+    srv = S.Server(budget, slots=5)
+    st = srv.run(reqs)
+    srv.close()
+    assert st["errors"] == 0 and st["solves"] == sum(r.steps for r in reqs)
+    assert st["peak_admitted_bytes"] <= budget
