@@ -72,7 +72,6 @@ def main():
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     import luspice_amd as L
     from oracle import oracle as O
 
@@ -87,14 +86,9 @@ def main():
     inf = B.info(0)
     log(f"[bench] symbolic+plan+upload {time.time() - t0:.1f}s  lnnz={inf['lnnz']} "
         f"unnz={inf['unnz']} updates={inf['n_updates']} levels={inf['nlevels']}")
-    dev_b, dev_x = B.device_vectors()
-    rhs_t = torch.from_numpy(c.rhs).cuda()
-    # stage the RHS into the resident arena (device to device)
-    import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
-    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-    hip.hipMemcpy(ctypes.c_void_p(dev_b), ctypes.c_void_p(rhs_t.data_ptr()), c.n * 8, 3)
-    torch.cuda.synchronize()
+    # the right-hand side lives in the batch's resident arena for the whole run
+    B.set_rhs(c.rhs)
+    L.device_sync()
 
     # ---- reference fp64 solution (CPU baseline leg doubles as the checker) ----
     cpu = None
@@ -128,15 +122,14 @@ def main():
 
     def measure(prec):
         run_steps(prec, args.warmup)
-        torch.cuda.synchronize()
+        L.device_sync()
         B.kernel_stats(reset=True)
         t1 = time.perf_counter()
         lat, reps = run_steps(prec, args.steps)
-        torch.cuda.synchronize()
+        L.device_sync()
         wall = time.perf_counter() - t1
         st = B.kernel_stats()
-        x = np.empty(c.n)
-        hip.hipMemcpy(ctypes.c_void_p(x.ctypes.data), ctypes.c_void_p(dev_x), c.n * 8, 2)
+        x = B.get_x()
         err = float(np.linalg.norm(x - xref) / np.linalg.norm(xref)) if xref is not None else None
         return dict(wall=wall, lat=lat, reps=reps, stats=st, err=err)
 

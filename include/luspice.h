@@ -118,6 +118,15 @@ int  lsp_batch_estimate_bytes(const lsp_batch *b, size_t *bytes);
 int  lsp_batch_upload(lsp_batch *b);
 /* Device memory currently held by the batch. */
 int  lsp_batch_device_bytes(const lsp_batch *b, size_t *bytes);
+
+/* Plain device-memory helpers for callers without their own HIP runtime
+ * binding (staging inputs in HBM): allocate, copy (any direction,
+ * synchronous), synchronise the device, query free / total memory. */
+int  lsp_dev_alloc(void **ptr, size_t bytes);
+int  lsp_dev_free(void *ptr);
+int  lsp_dev_copy(void *dst, const void *src, size_t bytes);
+int  lsp_dev_sync(void);
+int  lsp_dev_mem_info(size_t *free_bytes, size_t *total_bytes);
 void lsp_batch_destroy(lsp_batch *b);
 int  lsp_batch_set_stream(lsp_batch *b, void *hip_stream);
 int  lsp_batch_info(const lsp_batch *b, int circuit, lsp_circuit_info *info);

@@ -82,6 +82,46 @@ def perturb(seed: int, step: int, val0: np.ndarray, n: int, sigma: float = 0.1):
     return val, rhs
 
 
+class DeviceArray:
+    """A numpy array copied into device memory (owned; freed on close)."""
+
+    def __init__(self, host: np.ndarray):
+        self.host_dtype = host.dtype
+        self.size = host.size
+        self.nbytes = host.nbytes
+        p = N.C.c_void_p()
+        check(N.lib.lsp_dev_alloc(N.C.byref(p), self.nbytes), "dev_alloc")
+        self.ptr = p.value
+        src = np.ascontiguousarray(host)
+        check(N.lib.lsp_dev_copy(N.C.c_void_p(self.ptr), src.ctypes.data_as(N.C.c_void_p), self.nbytes),
+              "dev_copy")
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.size, self.host_dtype)
+        check(N.lib.lsp_dev_copy(out.ctypes.data_as(N.C.c_void_p), N.C.c_void_p(self.ptr), self.nbytes),
+              "dev_copy")
+        return out
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            N.lib.lsp_dev_free(N.C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+
+def device_sync():
+    check(N.lib.lsp_dev_sync(), "dev_sync")
+
+
+def device_memory() -> tuple[int, int]:
+    """(free, total) device bytes."""
+    f, t = N.C.c_size_t(), N.C.c_size_t()
+    check(N.lib.lsp_dev_mem_info(N.C.byref(f), N.C.byref(t)), "mem_info")
+    return f.value, t.value
+
+
 def colamd(c: Circuit) -> np.ndarray:
     q = np.empty(c.n, np.int32)
     check(N.lib.lsp_colamd(c.n, c.nnz, N.ptr(c.row), N.ptr(c.col), N.ptr(q)), "colamd")

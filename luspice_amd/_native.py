@@ -98,6 +98,11 @@ _sig("lsp_batch_device_vectors", C.c_int, C.c_void_p, C.POINTER(C.c_void_p),
 _sig("lsp_batch_set_rhs", C.c_int, C.c_void_p, C.c_void_p)
 _sig("lsp_batch_get_x", C.c_int, C.c_void_p, C.c_void_p)
 _sig("lsp_batch_last_times", C.c_int, C.c_void_p, c_double_p, c_double_p, c_double_p)
+_sig("lsp_dev_alloc", C.c_int, C.POINTER(C.c_void_p), C.c_size_t)
+_sig("lsp_dev_free", C.c_int, C.c_void_p)
+_sig("lsp_dev_copy", C.c_int, C.c_void_p, C.c_void_p, C.c_size_t)
+_sig("lsp_dev_sync", C.c_int)
+_sig("lsp_dev_mem_info", C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t))
 _sig("lsp_batch_last_error", C.c_char_p, C.c_void_p)
 _sig("lsp_batch_kernel_stats", C.c_int, C.c_void_p, C.POINTER(KernelStats), C.c_int)
 _sig("lsp_colamd", C.c_int, C.c_int, C.c_long, c_int_p, c_int_p, c_int_p)

@@ -38,7 +38,7 @@ def main(argv=None):
     ap.add_argument("--check", type=int, default=-1, help="circuits checked vs the CPU LU (-1 = all)")
     args = ap.parse_args(argv)
 
-    import torch
+    from . import device_sync
     from . import workloads as W
 
     def log(m):
@@ -55,7 +55,7 @@ def main(argv=None):
     log(f"[newton] batch built {time.time() - t0:.1f}s, dense tails {sum(t > 0 for t in tails)}, max T {max(tails)}")
     for s in range(args.warmup):
         NB.run_step(s)
-    torch.cuda.synchronize()
+    device_sync()
     lat = []
     reps = []
     t1 = time.perf_counter()
@@ -64,7 +64,7 @@ def main(argv=None):
         rr, sr = NB.run_step(s)
         lat.append((time.perf_counter() - ts) * 1e3)
         reps.append((rr, sr))
-    torch.cuda.synchronize()
+    device_sync()
     wall_b = time.perf_counter() - t1
     x_last = NB.solution()
     units_b = len(circ) * args.steps / wall_b
@@ -93,11 +93,11 @@ def main(argv=None):
     NP = W.NewtonPlain(circ, args.plain_steps + 1, host_inputs=NB.host[:args.plain_steps + 1])
     log(f"[newton] plain batches built {time.time() - t0:.1f}s")
     NP.run_step(0)
-    torch.cuda.synchronize()
+    device_sync()
     t1 = time.perf_counter()
     for s in range(1, args.plain_steps + 1):
         xp, preps = NP.run_step(s)
-    torch.cuda.synchronize()
+    device_sync()
     wall_p = time.perf_counter() - t1
     units_p = len(circ) * args.plain_steps / wall_p
     log(f"[newton] plain fp64: {units_p:.1f} units/s")

@@ -14,10 +14,11 @@
  *                       0 < kappa_ruiz <= 1e6
  *   dsh_mp_escalate     P-fast -> P-safe -> P-hard -> P-sing (terminal)
  *
- * On this device the SpMV kernel is SELL-C-64; the sigma reported in
- * dsh_config stays 32 so that callers reading the field see the reference's
- * value.  The autotuner (luspice_amd/autotune.py) picks the actual C and
- * sigma.
+ * On this device the SpMV kernel is SELL-C-64 (C = one wave64, one lane per
+ * row); the sigma reported in dsh_config stays 32 so that callers reading the
+ * field see the reference's value.  The autotuner (luspice_amd/autotune.py)
+ * picks the sorting window sigma, the persistent grid sizes and the dense
+ * trailing-block threshold per workload.
  */
 #include <stdlib.h>
 #include <string.h>

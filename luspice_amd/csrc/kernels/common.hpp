@@ -71,6 +71,15 @@ __device__ __forceinline__ int wave_ticket(int *ctr)
     return __builtin_amdgcn_readlane(v, leader);
 }
 
+// Epochs live in device counters so a launch needs no host-chosen argument and
+// a captured graph can be replayed: every wave reads the counter when it
+// starts, and the last wave out (after all others have read it) advances it.
+__device__ __forceinline__ int read_epoch(const int *counter) { return uniform(ld_relaxed(counter)); }
+__device__ __forceinline__ void advance_epoch(int *counter, int epoch)
+{
+    st_relaxed(counter, epoch < 0x3fffffff ? epoch + 1 : 1);
+}
+
 __device__ __forceinline__ bool aborted(const int *abort_word)
 {
     return uniform(ld_relaxed(abort_word)) != 0;

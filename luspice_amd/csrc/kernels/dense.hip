@@ -254,8 +254,8 @@ __global__ void __launch_bounds__(256) tail_fwd_gather_kernel(const void *mats, 
 template <typename T, bool kLower>
 __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const int *tails,
                                                         const long *voff, const int *active,
-                                                        double *y, double *z, double *x, int epoch,
-                                                        int *abort_word)
+                                                        double *y, double *z, double *x,
+                                                        const int *bwd_epoch, int *abort_word)
 {
     const int mi = tails[blockIdx.y];
     const TriDesc<T> M = ((const TriDesc<T> *)mats)[mi];
@@ -266,6 +266,8 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
     const bool live = !(active && !active[mi]);
     const int nblk = (M.tail_T + kNb - 1) / kNb;
     if (live) {
+        const int epoch = read_epoch(&M.tail_ctr[2]);            // block flags
+        const int bep = kLower ? 0 : read_epoch(bwd_epoch);     // row flags of the sparse solve
         if (threadIdx.x == 0) s_ticket = atomicAdd(&M.tail_ctr[0], 1);
         __syncthreads();
         const int t = uniform(s_ticket);
@@ -321,7 +323,7 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
                     }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                if (!kLower && lane < bi) st_relaxed(&M.bdone[k0 + r0 + lane], epoch);
+                if (!kLower && lane < bi) st_relaxed(&M.bdone[k0 + r0 + lane], bep);
                 if (lane == 0) publish(&M.tail_flag[ib], epoch);
             }
         }
@@ -331,6 +333,7 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
             if (fin == (int)gridDim.x - 1) {
                 st_relaxed(&M.tail_ctr[0], 0);
                 st_relaxed(&M.tail_ctr[1], 0);
+                advance_epoch(&M.tail_ctr[2], epoch);
             }
         }
     }
@@ -374,28 +377,28 @@ template hipError_t launch_dense_finish<__half>(const DenseJob *, int, int, cons
 template <typename T>
 hipError_t launch_tail_solve(const void *mats, const int *tails, int ntails, int maxT, const long *voff,
                              const int *active, const double *b, double *y, double *z, double *x,
-                             bool forward, int epoch, int *abort_word, hipStream_t st)
+                             bool forward, const int *bwd_epoch, int *abort_word, hipStream_t st)
 {
     const int nblk = (maxT + kNb - 1) / kNb;
     if (forward) {
         hipLaunchKernelGGL(tail_fwd_gather_kernel<T>, dim3((maxT + 3) / 4, ntails), dim3(256), 0, st, mats,
                            tails, voff, active, b, y);
         hipLaunchKernelGGL((tail_trsv_kernel<T, true>), dim3(nblk, ntails), dim3(256), 0, st, mats, tails,
-                           voff, active, y, z, x, epoch, abort_word);
+                           voff, active, y, z, x, bwd_epoch, abort_word);
     } else {
         hipLaunchKernelGGL((tail_trsv_kernel<T, false>), dim3(nblk, ntails), dim3(256), 0, st, mats, tails,
-                           voff, active, y, z, x, epoch, abort_word);
+                           voff, active, y, z, x, bwd_epoch, abort_word);
     }
     return hipGetLastError();
 }
 template hipError_t launch_tail_solve<float>(const void *, const int *, int, int, const long *, const int *,
-                                             const double *, double *, double *, double *, bool, int, int *,
+                                             const double *, double *, double *, double *, bool, const int *, int *,
                                              hipStream_t);
 template hipError_t launch_tail_solve<double>(const void *, const int *, int, int, const long *, const int *,
-                                              const double *, double *, double *, double *, bool, int, int *,
+                                              const double *, double *, double *, double *, bool, const int *, int *,
                                               hipStream_t);
 template hipError_t launch_tail_solve<__half>(const void *, const int *, int, int, const long *, const int *,
-                                              const double *, double *, double *, double *, bool, int, int *,
+                                              const double *, double *, double *, double *, bool, const int *, int *,
                                               hipStream_t);
 
 }  // namespace lsp

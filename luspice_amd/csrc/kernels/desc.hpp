@@ -60,7 +60,7 @@ struct TriDesc {
     const T *tail_F;        // tail_T^2, column-major, unit-L below / U on and above
     const int *tail_lcut;   // tail_T: end of the sparse part of L row tail_k0 + r
     int *tail_flag;         // ceil(tail_T / 64) block flags (epoch-tagged)
-    int *tail_ctr;          // 2 ints: block ticket counter, exit counter
+    int *tail_ctr;          // 4 ints: block ticket counter, exit counter, epoch, pad
     int n;
     int tail_k0;
     int tail_T;

@@ -186,6 +186,7 @@ __global__ void __launch_bounds__(256) refactor_kernel(FactorLaunch L)
     int *ws = L.trace ? L.trace + L.ntask + 8 * gw : nullptr;
     if (ws && lane == 0) { ws[4] = 0; ws[5] = 0; ws[6] = 0; ws[7] = 0; }
     trace_state(ws, 1, -1, -1);
+    const int epoch = read_epoch(L.epoch);
 
     for (;;) {
         const int t = wave_ticket(&L.ctr[0]);
@@ -202,21 +203,22 @@ __global__ void __launch_bounds__(256) refactor_kernel(FactorLaunch L)
         const int m = (int)(M.cp[k + 1] - M.cp[k]);
         const long long tc = ws ? (long long)clock64() : 0;
         if (m <= L.cap) {
-            factor_column<T>(M, k, xl, L.epoch, L.abort_word, L.ptol, L.dbg, ws);
+            factor_column<T>(M, k, xl, epoch, L.abort_word, L.ptol, L.dbg, ws);
         } else if (m <= L.scratch_stride) {
-            factor_column<T>(M, k, xg, L.epoch, L.abort_word, L.ptol, L.dbg, ws);
+            factor_column<T>(M, k, xg, epoch, L.abort_word, L.ptol, L.dbg, ws);
         } else {
             if (lane == 0) st_relaxed(L.abort_word, kErrCapacity);
         }
         if (ws) trace_max(ws, 7, (long long)clock64() - tc);
     }
     trace_state(ws, 7, -1, -1);
-    // the last wave out re-arms the queue for the next launch
+    // the last wave out re-arms the queue and advances the epoch
     if (lane == 0) {
         const int fin = atomicAdd(&L.ctr[1], 1);
         if (fin == L.total_waves - 1) {
             st_relaxed(&L.ctr[0], 0);
             st_relaxed(&L.ctr[1], 0);
+            advance_epoch(L.epoch, epoch);
         }
     }
 }

@@ -14,7 +14,7 @@ struct FactorLaunch {
     const int2 *task;       // (matrix, column) in level order
     const int *active;      // per-matrix mask (nullptr = all)
     int ntask;
-    int epoch;
+    int *epoch;             // device epoch counter: read at start, bumped by the last wave
     int *ctr;               // [0] queue head, [1] waves finished
     int *abort_word;
     double *scratch;        // per-wave global fallback
@@ -35,7 +35,7 @@ struct TriLaunch {
     double *x;              // solutions (original column order)
     double *y, *z;          // work vectors (position order)
     int ntask;
-    int epoch;
+    int *epoch;             // device epoch counter (see FactorLaunch)
     int *ctr;
     int *abort_word;
     int total_waves;
@@ -80,7 +80,7 @@ hipError_t launch_dense_finish(const DenseJob *jobs, int njobs, int maxT, const 
 template <typename T>
 hipError_t launch_tail_solve(const void *mats, const int *tails, int ntails, int maxT, const long *voff,
                              const int *active, const double *b, double *y, double *z, double *x,
-                             bool forward, int epoch, int *abort_word, hipStream_t st);
+                             bool forward, const int *bwd_epoch, int *abort_word, hipStream_t st);
 
 // trisolve.hip
 template <typename T>

@@ -27,6 +27,7 @@ __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
 {
     const int lane = lane_id();
     const TriDesc<T> *mats = (const TriDesc<T> *)L.mats;
+    const int epoch = read_epoch(L.epoch);
     for (;;) {
         const int t = wave_ticket(&L.ctr[0]);
         if (t >= L.ntask) break;
@@ -49,7 +50,7 @@ __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
         for (int b0 = e0; b0 < e1 && ok; b0 += kWave) {
             const int e = b0 + lane;
             const int j = e < e1 ? ci[e] : 0;
-            ok = wave_wait(e < e1 ? &flags[j] : nullptr, L.epoch, L.abort_word);
+            ok = wave_wait(e < e1 ? &flags[j] : nullptr, epoch, L.abort_word);
             if (!ok) break;
             acquire_fence();
             if (e < e1) acc += to_f64(rv[e]) * src[j];
@@ -65,7 +66,7 @@ __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
                 L.x[vo + M.q[i]] = zi;
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            publish(&flags[i], L.epoch);
+            publish(&flags[i], epoch);
         }
     }
     if (lane == 0) {
@@ -73,6 +74,7 @@ __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
         if (fin == L.total_waves - 1) {
             st_relaxed(&L.ctr[0], 0);
             st_relaxed(&L.ctr[1], 0);
+            advance_epoch(L.epoch, epoch);
         }
     }
 }

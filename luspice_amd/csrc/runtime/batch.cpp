@@ -24,14 +24,6 @@
 
 namespace lsp {
 
-static std::atomic<int> g_epoch{0};
-
-int Batch::next_epoch()
-{
-    int e = ++g_epoch;
-    if (e <= 0) { g_epoch = 1; e = 1; }
-    return e;
-}
 
 Batch::~Batch()
 {
@@ -268,7 +260,7 @@ int Batch::ensure_factor_set(FactorSet<T> &fs)
         D.tail_F = fs.tailF + C.o_t2;
         D.tail_lcut = d_tail_lcut_ + C.o_t;
         D.tail_flag = d_tail_flag_ + C.o_tb;
-        D.tail_ctr = d_tail_ctr_ + 2 * m;
+        D.tail_ctr = d_tail_ctr_ + 4 * m;
         D.tail_k0 = C.plan.tail_k0;
         D.tail_T = C.plan.tail_T;
     }
@@ -438,10 +430,13 @@ int Batch::upload_all()
         d_done_ = upload(zeros); d_fdone_ = upload(zeros); d_bdone_ = upload(zeros);
         std::vector<int> z6(8, 0);
         d_ctr_ = upload(z6); d_abort_ = upload(z6);
+        std::vector<int> e1(4, 1);                 // epochs start at 1, flags at 0
+        d_epoch_ = upload(e1);
         std::vector<int> ms(4 * nc, 0);
         d_mstat_ = upload(ms);
         d_dbg_ = upload(z6);
-        if (!d_done_ || !d_fdone_ || !d_bdone_ || !d_ctr_ || !d_abort_ || !d_mstat_) return LSP_E_ALLOC;
+        if (!d_done_ || !d_fdone_ || !d_bdone_ || !d_ctr_ || !d_abort_ || !d_mstat_ || !d_epoch_)
+            return LSP_E_ALLOC;
     }
     // dense trailing blocks
     {
@@ -471,7 +466,8 @@ int Batch::upload_all()
         d_tail_lcut_ = upload(tlc);
         d_djobs_ = upload(jobs);
         d_tails_ = upload(tails);
-        std::vector<int> zb(ntb_tot_, 0), zc(2 * nc, 0);
+        std::vector<int> zb(ntb_tot_, 0), zc(4 * nc, 0);
+        for (int m = 0; m < nc; m++) zc[4 * m + 2] = 1;          // block-flag epoch
         d_tail_flag_ = upload(zb);
         d_tail_ctr_ = upload(zc);
         if (!d_tail_ds_ || !d_tail_lcut_ || !d_djobs_ || !d_tails_ || !d_tail_flag_ || !d_tail_ctr_)
@@ -685,8 +681,7 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
     L.task = d_ftask_;
     L.active = d_active;
     L.ntask = (int)ntot_;
-    L.epoch = next_epoch();
-    next_epoch_peek_ = L.epoch;
+    L.epoch = d_epoch_;
     L.ctr = d_ctr_;
     L.abort_word = d_abort_;
     L.scratch = d_scratch_;
@@ -702,6 +697,7 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
             d_trace_ = (int *)dalloc(trace_len_ * sizeof(int));
         }
         (void)hipMemset(d_trace_, 0xff, trace_len_ * sizeof(int));
+        (void)hipMemcpy(&next_epoch_peek_, d_epoch_, 4, hipMemcpyDeviceToHost);
         L.trace = d_trace_;
     }
     rc = check(launch_reset_mstat(d_mstat_, (int)circ_.size(), d_abort_, d_dbg_, stream_), "reset");
@@ -822,27 +818,26 @@ int Batch::trisolve_t(FactorSet<T> &fs, const double *d_b, double *d_x, const in
     Lf.y = d_ty_;
     Lf.z = d_tz_;
     Lf.ntask = (int)nfw_tot_;
-    Lf.epoch = next_epoch();
+    Lf.epoch = d_epoch_ + 1;
     Lf.ctr = d_ctr_ + 2;
     Lf.abort_word = d_abort_ + 1;
     Lf.total_waves = tblocks_f_ * 4;
     TriLaunch Lb = Lf;
     Lb.task = d_bwtask_;
     Lb.ntask = (int)nbw_tot_;
-    Lb.epoch = next_epoch();
+    Lb.epoch = d_epoch_ + 2;
     Lb.ctr = d_ctr_ + 4;
     Lb.total_waves = tblocks_b_ * 4;
-    const int tail_epoch = next_epoch();
     (void)hipEventRecord(ev_[2], stream_);
     int rc = check(launch_trisolve_half<T>(Lf, true, tblocks_f_, stream_), "trisolve fwd");
     if (!rc && njobs_)
         rc = check(launch_tail_solve<T>(fs.tdesc, d_tails_, njobs_, max_tail_T_, d_voff_, d_active, d_b,
-                                        d_ty_, d_tz_, d_x, true, tail_epoch, d_abort_ + 1, stream_),
+                                        d_ty_, d_tz_, d_x, true, d_epoch_ + 2, d_abort_ + 1, stream_),
                    "tail fwd");
     (void)hipEventRecord(ev_[4], stream_);
     if (!rc && njobs_)
         rc = check(launch_tail_solve<T>(fs.tdesc, d_tails_, njobs_, max_tail_T_, d_voff_, d_active, d_b,
-                                        d_ty_, d_tz_, d_x, false, Lb.epoch, d_abort_ + 1, stream_),
+                                        d_ty_, d_tz_, d_x, false, d_epoch_ + 2, d_abort_ + 1, stream_),
                    "tail bwd");
     if (!rc) rc = check(launch_trisolve_half<T>(Lb, false, tblocks_b_, stream_), "trisolve bwd");
     (void)hipEventRecord(ev_[3], stream_);
@@ -876,7 +871,7 @@ int Batch::trisolve(const double *b, double *x, bool on_device)
         db = d_b_;
         dx = d_x_;
     }
-    rc = check(hipMemsetAsync(d_abort_ + 1, 0, 4, stream_), "abort reset");
+    rc = check(k_zero_int(d_abort_ + 1, 1, stream_), "abort reset");
     if (rc) return rc;
     rc = launch_trisolve_prec(cur_prec_, db, dx, nullptr);
     if (rc) return rc;
@@ -1024,7 +1019,7 @@ int Batch::solve(const double *b, double *x, bool on_device, lsp_solve_report *r
         db = d_b_;
         dx = d_x_;
     }
-    rc = check(hipMemsetAsync(d_abort_ + 1, 0, 4, stream_), "abort reset");
+    rc = check(k_zero_int(d_abort_ + 1, 1, stream_), "abort reset");
     if (rc) return rc;
     lsp_solve_report r{};
     rc = gmres_ir(db, dx, cur_prec_, nullptr, &r);
@@ -1049,7 +1044,7 @@ int Batch::solve(const double *b, double *x, bool on_device, lsp_solve_report *r
             rc = refactor(LSP_PREC_FP64, &rr, &sub);
             cur_prec_ = keep;
             if (rc && rc != LSP_E_SINGULAR) return rc;
-            rc = check(hipMemsetAsync(d_abort_ + 1, 0, 4, stream_), "abort reset");
+            rc = check(k_zero_int(d_abort_ + 1, 1, stream_), "abort reset");
             if (rc) return rc;
             lsp_solve_report r2{};
             rc = gmres_ir(db, dx, LSP_PREC_FP64, &subc, &r2);

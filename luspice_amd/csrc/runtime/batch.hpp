@@ -98,7 +98,6 @@ private:
     template <typename X> X *upload(const std::vector<X> &v);
     int check(hipError_t e, const char *what);
     int sync_read_int(const int *d, int *h, int count);
-    int next_epoch();
     KrylovArgs kargs() const;
     void account_trisolve();
 
@@ -127,6 +126,7 @@ private:
     long nslot_tot_ = 0, nl_tot_ = 0, nu_tot_ = 0, nsell_tot_ = 0;
     int nstask_ = 0;
     int *d_ctr_ = nullptr;               // [0..1] factor, [2..3] fwd, [4..5] bwd
+    int *d_epoch_ = nullptr;             // epoch counters: [0] factor, [1] fwd, [2] bwd
     int *d_abort_ = nullptr;             // [0] factor, [1] trisolve
     int *d_mstat_ = nullptr;             // 4 per circuit
     int *d_dbg_ = nullptr;               // first stuck wait of the refactor

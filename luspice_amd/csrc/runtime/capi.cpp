@@ -165,6 +165,38 @@ int lsp_batch_device_bytes(const lsp_batch *b, size_t *bytes)
     return LSP_OK;
 }
 
+int lsp_dev_alloc(void **ptr, size_t bytes)
+{
+    if (!ptr) return LSP_E_ARG;
+    *ptr = nullptr;
+    return hipMalloc(ptr, bytes ? bytes : 16) == hipSuccess ? LSP_OK : LSP_E_ALLOC;
+}
+
+int lsp_dev_free(void *ptr)
+{
+    return hipFree(ptr) == hipSuccess ? LSP_OK : LSP_E_HIP;
+}
+
+int lsp_dev_copy(void *dst, const void *src, size_t bytes)
+{
+    if (!dst || !src) return LSP_E_ARG;
+    return hipMemcpy(dst, src, bytes, hipMemcpyDefault) == hipSuccess ? LSP_OK : LSP_E_HIP;
+}
+
+int lsp_dev_sync(void)
+{
+    return hipDeviceSynchronize() == hipSuccess ? LSP_OK : LSP_E_HIP;
+}
+
+int lsp_dev_mem_info(size_t *free_bytes, size_t *total_bytes)
+{
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) return LSP_E_HIP;
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return LSP_OK;
+}
+
 int lsp_batch_last_times(const lsp_batch *b, double *rf, double *fw, double *bw)
 {
     return b ? b->impl.last_times(rf, fw, bw) : LSP_E_ARG;

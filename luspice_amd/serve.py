@@ -204,14 +204,14 @@ def main(argv=None):
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--seed", type=int, default=4)
     args = ap.parse_args(argv)
-    import torch
+    from . import device_sync
 
     for rate in args.rate:
         reqs = make_requests(args.requests, rate, nmax=args.nmax, seed=args.seed)
         srv = Server(int(args.budget_gb * 2**30), slots=args.slots, keep_results=True)
         st = srv.run(reqs)
         srv.close()
-        torch.cuda.synchronize()
+        device_sync()
         st["rate"] = rate
         st["rel_err_max_sample"] = check_against_cpu(reqs)
         print(json.dumps(st), flush=True)

@@ -21,7 +21,7 @@ import math
 
 import numpy as np
 
-from . import Batch, Circuit, colamd, mna, perturb
+from . import Batch, Circuit, DeviceArray, colamd, mna, perturb
 
 
 def newton_mix(ncirc: int = 512, nmin: int = 100, nmax: int = 5000, seed: int = 3,
@@ -50,21 +50,17 @@ class NewtonBatched:
     """All circuits in one batch; step inputs resident in HBM."""
 
     def __init__(self, circuits, steps: int, seed: int = 3, sigma: float = 0.1, **opts):
-        import torch
-
         self.circuits = circuits
         self.B = Batch(circuits, **opts)
         self.steps = steps
         self.host = [step_inputs(circuits, s, seed, sigma) for s in range(steps)]
-        self.dev_vals = [torch.from_numpy(v).cuda() for v, _ in self.host]
-        self.dev_rhs = [torch.from_numpy(b).cuda() for _, b in self.host]
-        self.db, self.dx = self.B.device_vectors()
-        self._torch = torch
+        self.dev_vals = [DeviceArray(v) for v, _ in self.host]
+        self.dev_rhs = [DeviceArray(b) for _, b in self.host]
 
     def run_step(self, s: int, prec: str = "fp32"):
         B = self.B
-        B.set_values_device(self.dev_vals[s].data_ptr())
-        B.set_rhs(self.dev_rhs[s].data_ptr())
+        B.set_values_device(self.dev_vals[s].ptr)
+        B.set_rhs(self.dev_rhs[s].ptr)
         rr = B.refactor(prec)
         sr = B.solve_device()
         return rr, sr

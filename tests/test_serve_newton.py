@@ -87,3 +87,20 @@ def test_serve_admission_respects_budget():
     srv.close()
     assert st["errors"] == 0 and st["solves"] == sum(r.steps for r in reqs)
     assert st["peak_admitted_bytes"] <= budget
+
+
+@pytest.mark.gpu
+def test_autotune_small_space(c0, c0_q):
+    print("case: autotune on c0, reduced space", flush=True)
+    from luspice_amd import autotune
+    circ = L.Circuit(c0.n, c0.row, c0.col, c0.val, c0.rhs, q=c0_q)
+    best, table = autotune.tune([circ], steps=1, cache=None,
+                                space={"sell_sigma": [0, 256], "tail_density": [0.3, 0.6]})
+    # the defaults, then each listed value of each knob (starting from the best so far)
+    assert len(table) == 5 and all(t is not None for _, t in table)
+    assert set(best) <= {"sell_sigma", "tail_density"}
+    B = L.Batch(circ, **best)
+    B.refactor("fp32")
+    x, rep = B.solve(c0.rhs)
+    rc, xr, _, _ = O.cpu_lu_ref(c0.n, c0.row, c0.col, c0.val, c0_q, c0.rhs)
+    assert _rel(x, xr) <= 1e-6
