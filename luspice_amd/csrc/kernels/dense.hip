@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(64) dense_diag_kernel(const DenseJob *jobs, co
     const size_t ld = (size_t)J.T;
     double *Sp = J.S + (size_t)p * ld + p;
     for (int j = 0; j < b; j++) d[j * kNb + lane] = lane < b ? Sp[j * ld + lane] : 0.0;
-    wave_sync();
+    wave_fence();
     for (int c = 0; c < b; c++) {
         double l = 0.0;
         if (lane > c && lane < b) {
@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(64) dense_diag_kernel(const DenseJob *jobs, co
         }
         const double am = wave_max(fabs(l));
         if (lane == 0) atomicMax(&J.colmax[p + c], dbits(am));
-        wave_sync();
+        wave_fence();
     }
     if (lane < b)
         for (int j = 0; j < b; j++) Sp[j * ld + lane] = d[j * kNb + lane];
@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(64) dense_trsm_kernel(const DenseJob *jobs, co
         const int r = p + b + t * kNb + lane;
         const bool ok = r < J.T;
         for (int j = 0; j < b; j++) w[j * kNb + lane] = ok ? J.S[(size_t)(p + j) * ld + r] : 0.0;
-        wave_sync();
+        wave_fence();
         for (int c = 0; c < b; c++) {
             const double xc = w[c * kNb + lane] / d[c * kNb + c];
             w[c * kNb + lane] = xc;
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(64) dense_trsm_kernel(const DenseJob *jobs, co
         const bool ok = col < J.T;
         double *Sc = J.S + (size_t)(ok ? col : p) * ld + p;
         for (int i = 0; i < b; i++) w[i * kNb + lane] = ok ? Sc[i] : 0.0;
-        wave_sync();
+        wave_fence();
         for (int r = 0; r < b; r++) {
             const double yr = w[r * kNb + lane];
             for (int i = r + 1; i < b; i++) w[i * kNb + lane] -= d[r * kNb + i] * yr;
@@ -286,13 +286,14 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
                 const int jb = kLower ? s : nblk - 1 - s;
                 ok = wave_wait(lane == 0 ? &M.tail_flag[jb] : nullptr, epoch, abort_word);
                 if (!ok) break;
-                acquire_fence();
                 const int c0 = jb * kNb;
                 const int bj = min(kNb, M.tail_T - c0);
-                if (lane < bi) {
-                    const T *Fc = M.tail_F + (size_t)c0 * ld + r0 + lane;
-                    for (int c = 0; c < bj; c++)
-                        acc += to_f64(Fc[(size_t)c * ld]) * src[vo + k0 + c0 + c];
+                // the block's 64 solved values, one per lane, then broadcast
+                const double sv = lane < bj ? ld_coherent_val(&src[vo + k0 + c0 + lane]) : 0.0;
+                const T *Fc = M.tail_F + (size_t)c0 * ld + r0 + min(lane, bi - 1);
+                for (int c = 0; c < bj; c++) {
+                    const double yc = __shfl(sv, c, kWave);        // whole wave active
+                    if (lane < bi) acc += to_f64(Fc[(size_t)c * ld]) * yc;
                 }
             }
             red[wv][lane] = acc;
@@ -308,7 +309,7 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
                         const double yc = __shfl(tr, c, kWave);
                         if (lane > c && lane < bi) tr -= to_f64(Fd[(size_t)c * ld + lane]) * yc;
                     }
-                    if (lane < bi) y[vo + k0 + r0 + lane] = tr;
+                    if (lane < bi) st_coherent(&y[vo + k0 + r0 + lane], tr);
                 } else {
                     for (int c = bi - 1; c >= 0; c--) {
                         double zc = __shfl(tr, c, kWave);
@@ -318,13 +319,13 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
                     }
                     if (lane < bi) {
                         const int row = k0 + r0 + lane;
-                        z[vo + row] = tr;
+                        st_coherent(&z[vo + row], tr);
                         x[vo + M.q[row]] = tr;
                     }
                 }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                store_complete();
                 if (!kLower && lane < bi) st_relaxed(&M.bdone[k0 + r0 + lane], bep);
-                if (lane == 0) publish(&M.tail_flag[ib], epoch);
+                if (lane == 0) st_relaxed(&M.tail_flag[ib], epoch);
             }
         }
         // the last workgroup of this matrix re-arms its ticket counter

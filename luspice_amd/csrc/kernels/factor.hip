@@ -21,37 +21,34 @@
 //     atomics touch x;
 //   * the pivot test (|pivot| >= ptol * max |L(:,k)|, in fp64) uses a wave
 //     max-reduction;
-//   * the finished column is stored in column order (read by later columns)
-//     and scattered into row-wise arrays for the trisolve, then published
-//     with a release store of the launch epoch.
+//   * dependencies are applied with a one-dependency software pipeline (the
+//     next L column's values and slots load while the current one updates x);
+//   * the finished column's L part is stored with agent-coherent stores and
+//     the column's flag is published once they are acknowledged, so no L2
+//     write-back / invalidate is needed on either side.
 #include "common.hpp"
 #include "launch.hpp"
 
 namespace lsp {
 
 
-__device__ __forceinline__ void trace_state(int *ws, int phase, int a, int b)
+// Applies dependency d of the current chunk: x[rel] -= L(:, j) * x[s_d].
+// The first 64 entries of the dependency's L column arrive prefetched in
+// (tg, lv); longer columns load the rest here.
+template <typename T>
+__device__ __forceinline__ void apply_dep(const FactorDesc<T> &M, double *x, int64_t ub, int jl0, int jlen,
+                                          double xj, int tg, double lv)
 {
-    if (ws && lane_id() == 0) {
-        st_relaxed(ws + 1, phase);
-        st_relaxed(ws + 2, a);
-        st_relaxed(ws + 3, b);
-    }
-}
-
-// ws[4..7]: max release-fence cycles, max acquire-fence cycles, columns
-// finished, max column cycles (all in units of 1024 clocks)
-__device__ __forceinline__ void trace_max(int *ws, int slot, long long cycles)
-{
-    if (ws && lane_id() == 0) {
-        const int v = (int)(cycles >> 10);
-        if (v > ws[slot]) ws[slot] = v;
-    }
+    const int lane = lane_id();
+    if (xj == 0.0) return;
+    if (lane < jlen) x[tg] -= lv * xj;
+    for (int e = lane + kWave; e < jlen; e += kWave)
+        x[M.rel[ub + e]] -= to_f64(ld_coherent_val(&M.val[jl0 + e])) * xj;
 }
 
 template <typename T>
 __device__ void factor_column(const FactorDesc<T> &M, int k, double *x,
-                              int epoch, int *abort_word, double ptol, int *dbg, int *ws)
+                              int epoch, int *abort_word, double ptol, int *dbg)
 {
     const int lane = lane_id();
     const int64_t c0 = M.cp[k];
@@ -59,12 +56,12 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, double *x,
     const int dl = M.dp[k] - (int)c0;          // local slot of the diagonal
 
     for (int i = lane; i < m; i += kWave) x[i] = 0.0;
-    wave_sync();
+    wave_fence();
     for (int p = M.aptr[k] + lane; p < M.aptr[k + 1]; p += kWave) {
         int s = M.aslot[p];
         if (s >= 0) x[s] = M.ain[M.asrc[p]];
     }
-    wave_sync();
+    wave_fence();
 
     // A column of the dense trailing block applies only its dependencies
     // outside the block (a prefix of its U slots); the block itself is
@@ -76,39 +73,45 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, double *x,
         const int nd = min(kWave, ndep - s0);
         int l0 = 0, len = 0;
         const int *flag = nullptr;
-        int jj = -1;
         if (lane < nd) {
             const int j = M.pos[c0 + s0 + lane];
-            jj = j;
             flag = &M.done[j];
             l0 = M.dp[j] + 1;
             len = (int)(M.cp[j + 1] - l0);
         }
-        if (ws) {
-            // record the largest dependency of this chunk that is not done yet
-            int pend = (lane < nd && ld_relaxed(flag) != epoch) ? jj : -1;
+        // offsets of each dependency's update-map entries within the chunk
+        int uoff = len;
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) pend = max(pend, __shfl_xor(pend, off, kWave));
-            trace_state(ws, 3, k, pend);
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int v = __shfl_up(uoff, o, kWave);
+            if (lane >= o) uoff += v;
         }
+        const int chunk_len = __shfl(uoff, kWave - 1, kWave);
+        uoff -= len;
         if (!wave_wait(flag, epoch, abort_word, dbg, k)) return;   // aborted; kernel drains
-        trace_state(ws, 4, k, s0);
-        const long long ta = ws ? (long long)clock64() : 0;
-        acquire_fence();
-        if (ws) trace_max(ws, 5, (long long)clock64() - ta);
+        // software pipeline: the L values and target slots of dependency d+1
+        // are loaded while dependency d updates x
+        int jl0 = __builtin_amdgcn_readlane(l0, 0), jlen = __builtin_amdgcn_readlane(len, 0);
+        int64_t ub = u + __builtin_amdgcn_readlane(uoff, 0);
+        int tg = 0;
+        double lv = 0.0;
+        if (lane < jlen) { tg = M.rel[ub + lane]; lv = to_f64(ld_coherent_val(&M.val[jl0 + lane])); }
         for (int d = 0; d < nd; d++) {
-            const int jl0 = __builtin_amdgcn_readlane(l0, d);
-            const int jlen = __builtin_amdgcn_readlane(len, d);
-            const double xj = x[s0 + d];
-            if (xj != 0.0) {
-                for (int e = lane; e < jlen; e += kWave) {
-                    const int tgt = M.rel[u + e];
-                    x[tgt] -= to_f64(M.val[jl0 + e]) * xj;
-                }
+            int njl0 = 0, njlen = 0, ntg = 0;
+            int64_t nub = 0;
+            double nlv = 0.0;
+            if (d + 1 < nd) {
+                njl0 = __builtin_amdgcn_readlane(l0, d + 1);
+                njlen = __builtin_amdgcn_readlane(len, d + 1);
+                nub = u + __builtin_amdgcn_readlane(uoff, d + 1);
+                if (lane < njlen) { ntg = M.rel[nub + lane]; nlv = to_f64(ld_coherent_val(&M.val[njl0 + lane])); }
             }
-            u += jlen;
-            wave_sync();
+            const double xj = x[s0 + d];
+            apply_dep(M, x, ub, jl0, jlen, xj, tg, lv);
+            wave_fence();
+            jl0 = njl0; jlen = njlen; ub = nub; tg = ntg; lv = nlv;
         }
+        u += chunk_len;
     }
 
     if (tail) {
@@ -122,7 +125,7 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, double *x,
         }
         double *Sc = M.tail_S + (size_t)c * (size_t)M.tail_T;
         for (int r = lane; r < M.tail_T; r += kWave) Sc[r] = 0.0;
-        __builtin_amdgcn_s_waitcnt(0);          // zero stores complete before the scatter
+        wave_fence();                           // same-address stores of one wave stay ordered
         for (int s = ndep + lane; s < m; s += kWave) Sc[M.pos[c0 + s] - M.tail_k0] = x[s];
         if (lane == 0) M.tail_colmax[c] = 0ull;
         return;
@@ -147,6 +150,9 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, double *x,
         }
         if (nonfinite) atomicAdd(&M.mstat[3], 1);
     }
+    // L(:, k) is read by later columns of this launch: coherent stores, then
+    // the flag once they are acknowledged.  U, the diagonal and the row-wise
+    // copies are read only after the launch.
     for (int i = lane; i < dl; i += kWave) {
         const T v = from_f64<T>(x[i]);
         M.val[c0 + i] = v;
@@ -159,41 +165,26 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, double *x,
     }
     for (int i = dl + 1 + lane; i < m; i += kWave) {
         const T v = from_f64<T>(x[i] / piv);
-        M.val[c0 + i] = v;
+        st_coherent(&M.val[c0 + i], v);
         M.lrow[M.dst[c0 + i]] = v;
     }
-    const long long tr = ws ? (long long)clock64() : 0;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (lane == 0) publish(&M.done[k], epoch);
-    if (ws) {
-        trace_max(ws, 4, (long long)clock64() - tr);
-        if (lane == 0) ws[6] += 1;
-    }
-    trace_state(ws, 6, k, 0);
+    store_complete();
+    if (lane == 0) st_relaxed(&M.done[k], epoch);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256) refactor_kernel(FactorLaunch L)
 {
     extern __shared__ double lds[];
-    const int lane = lane_id();
     const int wv = threadIdx.x >> 6;
     const int waves_per_block = blockDim.x >> 6;
     double *xl = lds + (long)wv * L.cap;
     double *xg = L.scratch + ((long)blockIdx.x * waves_per_block + wv) * L.scratch_stride;
     const FactorDesc<T> *mats = (const FactorDesc<T> *)L.mats;
-    const int gw = blockIdx.x * waves_per_block + wv;
-    int *ws = L.trace ? L.trace + L.ntask + 8 * gw : nullptr;
-    if (ws && lane == 0) { ws[4] = 0; ws[5] = 0; ws[6] = 0; ws[7] = 0; }
-    trace_state(ws, 1, -1, -1);
     const int epoch = read_epoch(L.epoch);
 
     for (;;) {
         const int t = wave_ticket(&L.ctr[0]);
-        if (ws && lane == 0) {
-            st_relaxed(ws, t);
-            if (t < L.ntask) st_relaxed(L.trace + t, gw);
-        }
         if (t >= L.ntask) break;
         if (aborted(L.abort_word)) continue;                  // drain
         const int2 tk = L.task[t];
@@ -201,19 +192,16 @@ __global__ void __launch_bounds__(256) refactor_kernel(FactorLaunch L)
         const FactorDesc<T> M = mats[tk.x];
         const int k = tk.y;
         const int m = (int)(M.cp[k + 1] - M.cp[k]);
-        const long long tc = ws ? (long long)clock64() : 0;
         if (m <= L.cap) {
-            factor_column<T>(M, k, xl, epoch, L.abort_word, L.ptol, L.dbg, ws);
+            factor_column<T>(M, k, xl, epoch, L.abort_word, L.ptol, L.dbg);
         } else if (m <= L.scratch_stride) {
-            factor_column<T>(M, k, xg, epoch, L.abort_word, L.ptol, L.dbg, ws);
+            factor_column<T>(M, k, xg, epoch, L.abort_word, L.ptol, L.dbg);
         } else {
-            if (lane == 0) st_relaxed(L.abort_word, kErrCapacity);
+            if (lane_id() == 0) st_relaxed(L.abort_word, kErrCapacity);
         }
-        if (ws) trace_max(ws, 7, (long long)clock64() - tc);
     }
-    trace_state(ws, 7, -1, -1);
     // the last wave out re-arms the queue and advances the epoch
-    if (lane == 0) {
+    if (lane_id() == 0) {
         const int fin = atomicAdd(&L.ctr[1], 1);
         if (fin == L.total_waves - 1) {
             st_relaxed(&L.ctr[0], 0);

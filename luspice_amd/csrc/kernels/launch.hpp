@@ -23,7 +23,6 @@ struct FactorLaunch {
     int total_waves;
     double ptol;
     int *dbg;               // 8 ints: first stuck wait (dep, seen, epoch, spins, column)
-    int *trace;             // optional: [ntask] grabbing wave, [waves*4] wave state
 };
 
 struct TriLaunch {

@@ -15,7 +15,9 @@
 //
 // One wave per row: lanes stride over the row's entries (coalesced index and
 // value loads), wait on their own dependency flags in parallel, and the wave
-// sums in fp64 with a DPP reduction.
+// sums in fp64 with a DPP reduction.  Solved values are stored agent-coherent
+// and flagged once acknowledged; readers load them agent-coherent, so no L2
+// write-back or invalidate is needed (see common.hpp).
 #include "common.hpp"
 #include "launch.hpp"
 
@@ -52,21 +54,20 @@ __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
             const int j = e < e1 ? ci[e] : 0;
             ok = wave_wait(e < e1 ? &flags[j] : nullptr, epoch, L.abort_word);
             if (!ok) break;
-            acquire_fence();
-            if (e < e1) acc += to_f64(rv[e]) * src[j];
+            if (e < e1) acc += to_f64(rv[e]) * ld_coherent_val(&src[j]);
         }
         if (!ok) continue;
         acc = wave_sum(acc);
         if (lane == 0) {
             if (kForward) {
-                y[i] = L.b[vo + M.rowat[i]] - acc;
+                st_coherent(&y[i], L.b[vo + M.rowat[i]] - acc);
             } else {
                 const double zi = (y[i] - acc) / to_f64(M.diag[i]);
-                z[i] = zi;
+                st_coherent(&z[i], zi);
                 L.x[vo + M.q[i]] = zi;
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            publish(&flags[i], epoch);
+            store_complete();
+            st_relaxed(&flags[i], epoch);
         }
     }
     if (lane == 0) {

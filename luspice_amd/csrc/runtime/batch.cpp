@@ -690,16 +690,6 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
     L.total_waves = fblocks_ * 4;
     L.ptol = opt_.refactor_ptol;
     L.dbg = d_dbg_;
-    L.trace = nullptr;
-    if (getenv("LSP_TRACE_REFACTOR")) {
-        if (!d_trace_) {
-            trace_len_ = ntot_ + 8L * fblocks_ * 4;
-            d_trace_ = (int *)dalloc(trace_len_ * sizeof(int));
-        }
-        (void)hipMemset(d_trace_, 0xff, trace_len_ * sizeof(int));
-        (void)hipMemcpy(&next_epoch_peek_, d_epoch_, 4, hipMemcpyDeviceToHost);
-        L.trace = d_trace_;
-    }
     rc = check(launch_reset_mstat(d_mstat_, (int)circ_.size(), d_abort_, d_dbg_, stream_), "reset");
     if (rc) return rc;
     (void)hipEventRecord(ev_[0], stream_);
@@ -751,39 +741,6 @@ int Batch::refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *
         if (ms[4 * m + 2] != 0x7fffffff) r.singular_circuits++;
         if ((ms[4 * m] || ms[4 * m + 2] != 0x7fffffff || ms[4 * m + 3]) && r.first_bad_circuit < 0)
             r.first_bad_circuit = m;
-    }
-    if (d_trace_ && getenv("LSP_TRACE_REFACTOR")) {
-        std::vector<int> tr(trace_len_);
-        std::vector<int> dn(ntot_);
-        (void)hipMemcpy(tr.data(), d_trace_, trace_len_ * 4, hipMemcpyDeviceToHost);
-        (void)hipMemcpy(dn.data(), d_done_, ntot_ * 4, hipMemcpyDeviceToHost);
-        std::vector<int> tk(ntot_ * 2);
-        (void)hipMemcpy(tk.data(), d_ftask_, ntot_ * 8, hipMemcpyDeviceToHost);
-        std::vector<int> col_task(ntot_, -1);
-        for (long t = 0; t < ntot_; t++) col_task[tk[2 * t + 1]] = (int)t;
-        long grabbed = 0, ndone = 0;
-        for (long t = 0; t < ntot_; t++) grabbed += tr[t] >= 0;
-        for (long k = 0; k < ntot_; k++) ndone += dn[k] == next_epoch_peek_;
-        fprintf(stderr, "[trace] tasks grabbed %ld / %ld, columns done %ld\n", grabbed, ntot_, ndone);
-        const long nw = (long)fblocks_ * 4;
-        int mrel = 0, macq = 0, mcol = 0;
-        long cols = 0;
-        for (long w = 0; w < nw; w++) {
-            const int *s = tr.data() + ntot_ + 8 * w;
-            mrel = std::max(mrel, s[4]); macq = std::max(macq, s[5]);
-            mcol = std::max(mcol, s[7]); cols += s[6];
-        }
-        fprintf(stderr, "[trace] max release fence %d Kclk, max acquire fence %d Kclk, "
-                        "max column %d Kclk, columns finished %ld\n", mrel, macq, mcol, cols);
-        for (long w = 0; w < nw; w++) {
-            const int *s = tr.data() + ntot_ + 8 * w;
-            if (s[1] == 7) continue;
-            fprintf(stderr, "[trace] wave %ld task %d phase %d a %d b %d", w, s[0], s[1], s[2], s[3]);
-            if (s[1] == 3 && s[3] >= 0)
-                fprintf(stderr, " -> dep %d task %d grabbed by wave %d done=%d", s[3],
-                        col_task[s[3]], tr[col_task[s[3]]], dn[s[3]]);
-            fprintf(stderr, "\n");
-        }
     }
     if (abort_code == kErrSpin) {
         int dbg[8] = {0};

@@ -130,8 +130,6 @@ private:
     int *d_abort_ = nullptr;             // [0] factor, [1] trisolve
     int *d_mstat_ = nullptr;             // 4 per circuit
     int *d_dbg_ = nullptr;               // first stuck wait of the refactor
-    int *d_trace_ = nullptr;             // optional refactor trace (LSP_TRACE_REFACTOR)
-    long trace_len_ = 0;
     int next_epoch_peek_ = 0;
     double *d_scratch_ = nullptr;
     long scratch_stride_ = 0;
