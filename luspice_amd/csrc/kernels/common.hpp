@@ -19,6 +19,19 @@
 
 namespace lsp {
 
+// Address-space qualified pointers.  Descriptor pointers loaded from memory
+// are generic to the compiler, which then emits FLAT accesses; a FLAT load is
+// counted on both the vector-memory and the LDS counters, so any wait for an
+// LDS result also drains every load in flight.  Hot loops therefore access
+// device arrays through LSP_GLOBAL pointers and LDS through LSP_LDS pointers.
+#define LSP_GLOBAL __attribute__((address_space(1)))
+#define LSP_LDS __attribute__((address_space(3)))
+template <typename T> __device__ __forceinline__ LSP_GLOBAL T *gp(T *p) { return (LSP_GLOBAL T *)p; }
+template <typename T> __device__ __forceinline__ const LSP_GLOBAL T *gp(const T *p)
+{
+    return (const LSP_GLOBAL T *)p;
+}
+
 constexpr int kWave = 64;
 constexpr int kSpinLimit = 1 << 22;   // ~ seconds with s_sleep(1) between polls
 
@@ -26,11 +39,11 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
 __device__ __forceinline__ int ld_acquire(const int *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(gp(p), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ int ld_relaxed(const int *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // Read-modify-write probes are performed at the device coherence point, so a
 // flag written by a wave on another XCD is seen whatever a local cache holds.
@@ -48,7 +61,7 @@ __device__ __forceinline__ void st_release(int *p, int v)
 }
 __device__ __forceinline__ void st_relaxed(int *p, int v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Wave-uniform reads for control flow.  A persistent wave's loop must stay
@@ -67,7 +80,8 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 __device__ __forceinline__ int wave_ticket(int *ctr)
 {
     const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
-    const int v = atomicAdd(ctr, lane_id() == leader ? 1 : 0);
+    const int v = __hip_atomic_fetch_add(gp(ctr), lane_id() == leader ? 1 : 0, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
     return __builtin_amdgcn_readlane(v, leader);
 }
 
@@ -102,21 +116,28 @@ __device__ __forceinline__ void store_complete() { __builtin_amdgcn_s_waitcnt(0)
 
 template <typename T> __device__ __forceinline__ void st_coherent(T *p, T v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <> __device__ __forceinline__ void st_coherent<__half>(__half *p, __half v)
 {
-    __hip_atomic_store((unsigned short *)p, __half_as_ushort(v), __ATOMIC_RELAXED,
+    __hip_atomic_store(gp((unsigned short *)p), __half_as_ushort(v), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
 }
+// Plain (non-coherent) global store of a factor value.
+template <typename T> __device__ __forceinline__ void st_g(T *p, T v) { *gp(p) = v; }
+template <> __device__ __forceinline__ void st_g<__half>(__half *p, __half v)
+{
+    *gp((unsigned short *)p) = __half_as_ushort(v);
+}
+
 template <typename T> __device__ __forceinline__ T ld_coherent_val(const T *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <> __device__ __forceinline__ __half ld_coherent_val<__half>(const __half *p)
 {
     return __ushort_as_half(
-        __hip_atomic_load((const unsigned short *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        __hip_atomic_load(gp((const unsigned short *)p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
 // Orders this wave's accesses in the compiler only: the memory operations of

@@ -44,34 +44,65 @@ __device__ __forceinline__ unsigned long long dbits(double v)
     return (unsigned long long)__double_as_longlong(v);
 }
 
+// Broadcast of lane `src`'s value to the wave (src wave-uniform).
+__device__ __forceinline__ double bcast(double v, int src)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffLL), src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 // ---------------------------------------------------------------------------
+// Diagonal block LU, one wave: lane r keeps row r of the 64x64 block in
+// registers; row c reaches the other lanes by readlane.  A partial last block
+// is padded with identity rows/columns, so every loop is fully unrolled.
 __global__ void __launch_bounds__(64) dense_diag_kernel(const DenseJob *jobs, const int *active, int p)
 {
     const DenseJob J = jobs[blockIdx.x];
     if (p >= J.T || (active && !active[J.mat])) return;
-    __shared__ double d[kNb * kNb];                    // column-major d[j*64 + r]
     const int lane = threadIdx.x;
     const int b = min(kNb, J.T - p);
     const size_t ld = (size_t)J.T;
     double *Sp = J.S + (size_t)p * ld + p;
-    for (int j = 0; j < b; j++) d[j * kNb + lane] = lane < b ? Sp[j * ld + lane] : 0.0;
-    wave_fence();
-    for (int c = 0; c < b; c++) {
-        double l = 0.0;
-        if (lane > c && lane < b) {
-            l = d[c * kNb + lane] / d[c * kNb + c];
-            d[c * kNb + lane] = l;
-            for (int j = c + 1; j < b; j++) d[j * kNb + lane] -= l * d[j * kNb + c];
-        }
-        const double am = wave_max(fabs(l));
-        if (lane == 0) atomicMax(&J.colmax[p + c], dbits(am));
-        wave_fence();
+    const int rr = min(lane, b - 1);
+    double a[kNb];
+#pragma unroll
+    for (int j = 0; j < kNb; j++) {
+        const double v = Sp[(size_t)min(j, b - 1) * ld + rr];
+        a[j] = (j < b && lane < b) ? v : (j == lane ? 1.0 : 0.0);
     }
-    if (lane < b)
-        for (int j = 0; j < b; j++) Sp[j * ld + lane] = d[j * kNb + lane];
+#pragma unroll
+    for (int c = 0; c < kNb; c++) {
+        const double l = a[c] / bcast(a[c], c);
+        if (lane > c) a[c] = l;
+#pragma unroll
+        for (int j = c + 1; j < kNb; j++) {
+            const double u = bcast(a[j], c);
+            if (lane > c) a[j] -= l * u;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < kNb; c++) {
+        const double am = wave_max(lane > c && lane < b ? fabs(a[c]) : 0.0);
+        if (lane == 0 && c < b) atomicMax(&J.colmax[p + c], dbits(am));
+    }
+    if (lane < b) {
+#pragma unroll
+        for (int j = 0; j < kNb; j++)
+            if (j < b) Sp[(size_t)j * ld + lane] = a[j];
+    }
 }
 
 // ---------------------------------------------------------------------------
+// L21 = A21 U11^-1 (64-row tiles) and U12 = L11^-1 A12 (64-column tiles), one
+// wave per tile.  The factored diagonal block sits in LDS (read as
+// broadcasts); each lane's row (L21) or column (U12) is a padded LDS column,
+// so every LDS access is conflict-free, and both tiles are moved between HBM
+// and LDS with coalesced accesses (lanes along the rows of a column).
+constexpr int kWLd = kNb + 1;                          // padded LDS stride of a tile
+constexpr size_t kTrsmLds = sizeof(double) * (kNb * kNb + kNb * kWLd);
+
 __global__ void __launch_bounds__(64) dense_trsm_kernel(const DenseJob *jobs, const int *active, int p)
 {
     const DenseJob J = jobs[blockIdx.y];
@@ -82,42 +113,64 @@ __global__ void __launch_bounds__(64) dense_trsm_kernel(const DenseJob *jobs, co
     const int ntile = (rem + kNb - 1) / kNb;
     const int t = blockIdx.x;
     if (t >= 2 * ntile) return;
-    __shared__ double d[kNb * kNb];                    // factored diagonal block
-    __shared__ double w[kNb * kNb];                    // lane-private vectors w[j*64 + lane]
+    extern __shared__ double lds[];
+    double *__restrict__ d = lds;                      // d[j*64 + i] = D(i, j), identity-padded
+    double *__restrict__ w = lds + kNb * kNb;          // tile, element (row i, col c) at w[c*65 + i]
     const int lane = threadIdx.x;
     const size_t ld = (size_t)J.T;
     const double *Dp = J.S + (size_t)p * ld + p;
-    for (int j = 0; j < b; j++) d[j * kNb + lane] = lane < b ? Dp[j * ld + lane] : 0.0;
+    // every load loop is unrolled so its 64 loads are in flight together
+    {
+        const int rr = min(lane, b - 1);
+#pragma unroll
+        for (int j = 0; j < kNb; j++) {
+            const double v = Dp[(size_t)min(j, b - 1) * ld + rr];
+            d[j * kNb + lane] = (j < b && lane < b) ? v : (j == lane ? 1.0 : 0.0);
+        }
+    }
     if (t < ntile) {
-        // one row of L21 per lane: x U11 = a
+        // rows r0 + lane of L21: x U11 = a; w[c*65 + lane] = x_c of this lane's row
         const int r = p + b + t * kNb + lane;
         const bool ok = r < J.T;
-        for (int j = 0; j < b; j++) w[j * kNb + lane] = ok ? J.S[(size_t)(p + j) * ld + r] : 0.0;
+        const int rc = ok ? r : J.T - 1;
+#pragma unroll
+        for (int c = 0; c < kNb; c++) {
+            const double v = J.S[(size_t)(p + min(c, b - 1)) * ld + rc];
+            w[c * kWLd + lane] = (ok && c < b) ? v : 0.0;
+        }
         wave_fence();
         for (int c = 0; c < b; c++) {
-            const double xc = w[c * kNb + lane] / d[c * kNb + c];
-            w[c * kNb + lane] = xc;
-            for (int j = c + 1; j < b; j++) w[j * kNb + lane] -= xc * d[j * kNb + c];
+            const double xc = w[c * kWLd + lane] / d[c * kNb + c];
+            w[c * kWLd + lane] = xc;
+#pragma unroll 8
+            for (int j = c + 1; j < b; j++) w[j * kWLd + lane] -= xc * d[j * kNb + c];
         }
         for (int c = 0; c < b; c++) {
-            const double v = w[c * kNb + lane];
+            const double v = w[c * kWLd + lane];
             if (ok) J.S[(size_t)(p + c) * ld + r] = v;
             const double am = wave_max(ok ? fabs(v) : 0.0);
             if (lane == 0) atomicMax(&J.colmax[p + c], dbits(am));
         }
     } else {
-        // one column of U12 per lane: L11 y = a (unit lower)
-        const int col = p + b + (t - ntile) * kNb + lane;
-        const bool ok = col < J.T;
-        double *Sc = J.S + (size_t)(ok ? col : p) * ld + p;
-        for (int i = 0; i < b; i++) w[i * kNb + lane] = ok ? Sc[i] : 0.0;
+        // columns c0 + lane of U12: L11 y = a; tile element (row i, column lane)
+        // lives at w[i*65 + lane] (transposed on the way in and out)
+        const int c0 = p + b + (t - ntile) * kNb;
+        const int ncol = min(kNb, J.T - c0);
+        const int rl = min(lane, b - 1);
+#pragma unroll
+        for (int cc = 0; cc < kNb; cc++) {
+            const double v = J.S[(size_t)(c0 + min(cc, ncol - 1)) * ld + p + rl];
+            w[lane * kWLd + cc] = lane < b ? v : 0.0;
+        }
         wave_fence();
         for (int r = 0; r < b; r++) {
-            const double yr = w[r * kNb + lane];
-            for (int i = r + 1; i < b; i++) w[i * kNb + lane] -= d[r * kNb + i] * yr;
+            const double yr = w[r * kWLd + lane];
+#pragma unroll 8
+            for (int i = r + 1; i < b; i++) w[i * kWLd + lane] -= d[r * kNb + i] * yr;
         }
-        if (ok)
-            for (int i = 0; i < b; i++) Sc[i] = w[i * kNb + lane];
+        wave_fence();
+        if (lane < b)
+            for (int cc = 0; cc < ncol; cc++) J.S[(size_t)(c0 + cc) * ld + p + lane] = w[lane * kWLd + cc];
     }
 }
 
@@ -250,7 +303,11 @@ __global__ void __launch_bounds__(256) tail_fwd_gather_kernel(const void *mats, 
 
 // Dense triangular solve of the block rows (see the file comment).
 //   lower: y_T <- L22^-1 y_T (unit diagonal), in place
-//   upper: z_T <- U22^-1 y_T, x[q] = z, bdone[row] = epoch
+//   upper: z_T <- U22^-1 y_T, x[q] = z, bdone[row] = epoch of the sparse solve
+// Per contributing block the 64 x 64 factor slab (one row per lane, in
+// registers) is loaded before its flag is awaited, so the wait overlaps the
+// loads; the solved values arrive with one coherent load and are broadcast by
+// readlane.  The diagonal block is loaded in bulk before the 64-step solve.
 template <typename T, bool kLower>
 __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const int *tails,
                                                         const long *voff, const int *active,
@@ -279,43 +336,51 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
             const long vo = voff[mi];
             const int k0 = M.tail_k0;
             const double *src = kLower ? y : z;
-            // contributions of the other blocks, wave wv takes every 4th one
+            const int rr = r0 + min(lane, bi - 1);               // clamped row of this lane
             double acc = 0.0;
             bool ok = true;
             for (int s = wv; s < t && ok; s += 4) {
                 const int jb = kLower ? s : nblk - 1 - s;
-                ok = wave_wait(lane == 0 ? &M.tail_flag[jb] : nullptr, epoch, abort_word);
-                if (!ok) break;
                 const int c0 = jb * kNb;
                 const int bj = min(kNb, M.tail_T - c0);
-                // the block's 64 solved values, one per lane, then broadcast
+                T fv[kNb];
+#pragma unroll
+                for (int c = 0; c < kNb; c++) fv[c] = M.tail_F[(size_t)(c0 + min(c, bj - 1)) * ld + rr];
+                ok = wave_wait(lane == 0 ? &M.tail_flag[jb] : nullptr, epoch, abort_word);
+                if (!ok) break;
                 const double sv = lane < bj ? ld_coherent_val(&src[vo + k0 + c0 + lane]) : 0.0;
-                const T *Fc = M.tail_F + (size_t)c0 * ld + r0 + min(lane, bi - 1);
-                for (int c = 0; c < bj; c++) {
-                    const double yc = __shfl(sv, c, kWave);        // whole wave active
-                    if (lane < bi) acc += to_f64(Fc[(size_t)c * ld]) * yc;
+#pragma unroll
+                for (int c = 0; c < kNb; c++) {
+                    const double yc = bcast(sv, c);              // zero beyond bj
+                    acc += to_f64(fv[c]) * yc;
                 }
             }
-            red[wv][lane] = acc;
+            red[wv][lane] = lane < bi ? acc : 0.0;
             __syncthreads();
             const bool all_ok = __syncthreads_and(ok);
             if (all_ok && wv == 0) {
+                // row lane of the diagonal block: dg[c] = F(r0 + lane, r0 + c)
+                T dg[kNb];
+#pragma unroll
+                for (int c = 0; c < kNb; c++) dg[c] = M.tail_F[(size_t)(r0 + min(c, bi - 1)) * ld + rr];
                 double tr = 0.0;
                 if (lane < bi)
                     tr = y[vo + k0 + r0 + lane] - (red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane]);
-                const T *Fd = M.tail_F + (size_t)r0 * ld + r0;   // diagonal block
                 if (kLower) {
-                    for (int c = 0; c < bi; c++) {
-                        const double yc = __shfl(tr, c, kWave);
-                        if (lane > c && lane < bi) tr -= to_f64(Fd[(size_t)c * ld + lane]) * yc;
+#pragma unroll
+                    for (int c = 0; c < kNb; c++) {
+                        const double yc = bcast(tr, c);
+                        if (lane > c && c < bi) tr -= to_f64(dg[c]) * yc;
                     }
                     if (lane < bi) st_coherent(&y[vo + k0 + r0 + lane], tr);
                 } else {
-                    for (int c = bi - 1; c >= 0; c--) {
-                        double zc = __shfl(tr, c, kWave);
-                        zc = zc / to_f64(Fd[(size_t)c * ld + c]);
-                        if (lane == c) tr = zc;
-                        if (lane < c) tr -= to_f64(Fd[(size_t)c * ld + lane]) * zc;
+#pragma unroll
+                    for (int c = kNb - 1; c >= 0; c--) {
+                        if (c < bi) {
+                            const double zc = bcast(tr, c) / bcast(to_f64(dg[c]), c);
+                            if (lane == c) tr = zc;
+                            if (lane < c) tr -= to_f64(dg[c]) * zc;
+                        }
                     }
                     if (lane < bi) {
                         const int row = k0 + r0 + lane;
@@ -345,8 +410,11 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
 // ---------------------------------------------------------------------------
 hipError_t dense_set_lds()
 {
-    return hipFuncSetAttribute((const void *)dense_gemm_kernel,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGemmLds);
+    hipError_t e = hipFuncSetAttribute((const void *)dense_gemm_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGemmLds);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void *)dense_trsm_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kTrsmLds);
 }
 
 hipError_t launch_dense_lu(const DenseJob *jobs, int njobs, int maxT, const int *active, hipStream_t st)
@@ -356,7 +424,7 @@ hipError_t launch_dense_lu(const DenseJob *jobs, int njobs, int maxT, const int 
         hipLaunchKernelGGL(dense_diag_kernel, dim3(njobs), dim3(64), 0, st, jobs, active, p);
         if (rem > 0) {
             const int nt = (rem + kNb - 1) / kNb;
-            hipLaunchKernelGGL(dense_trsm_kernel, dim3(2 * nt, njobs), dim3(64), 0, st, jobs, active, p);
+            hipLaunchKernelGGL(dense_trsm_kernel, dim3(2 * nt, njobs), dim3(64), kTrsmLds, st, jobs, active, p);
             hipLaunchKernelGGL(dense_gemm_kernel, dim3(nt * nt, njobs), dim3(256), kGemmLds, st, jobs,
                                active, p);
         }

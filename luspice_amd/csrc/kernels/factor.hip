@@ -32,34 +32,35 @@
 namespace lsp {
 
 
-// Applies dependency d of the current chunk: x[rel] -= L(:, j) * x[s_d].
-// The first 64 entries of the dependency's L column arrive prefetched in
-// (tg, lv); longer columns load the rest here.
-template <typename T>
-__device__ __forceinline__ void apply_dep(const FactorDesc<T> &M, double *x, int64_t ub, int jl0, int jlen,
-                                          double xj, int tg, double lv)
-{
-    const int lane = lane_id();
-    if (xj == 0.0) return;
-    if (lane < jlen) x[tg] -= lv * xj;
-    for (int e = lane + kWave; e < jlen; e += kWave)
-        x[M.rel[ub + e]] -= to_f64(ld_coherent_val(&M.val[jl0 + e])) * xj;
-}
+constexpr int kPre = 8;                 // dependencies in flight ahead of the update
 
-template <typename T>
-__device__ void factor_column(const FactorDesc<T> &M, int k, double *x,
-                              int epoch, int *abort_word, double ptol, int *dbg)
+// One column of the left-looking refactorization.  XP is the work vector's
+// pointer type: an LSP_LDS pointer when the column fits the wave's LDS slice,
+// an LSP_GLOBAL scratch pointer otherwise.
+template <typename T, typename XP>
+__device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
+                              int epoch, int *abort_word, double ptol, int *dbg, long long *waited)
 {
     const int lane = lane_id();
-    const int64_t c0 = M.cp[k];
-    const int m = (int)(M.cp[k + 1] - c0);
-    const int dl = M.dp[k] - (int)c0;          // local slot of the diagonal
+    const LSP_GLOBAL int64_t *cp = gp(M.cp);
+    const LSP_GLOBAL int *dp = gp(M.dp);
+    const LSP_GLOBAL int *pos = gp(M.pos);
+    const LSP_GLOBAL uint16_t *rel = gp(M.rel);
+    const int64_t c0 = cp[k];
+    const int m = (int)(cp[k + 1] - c0);
+    const int dl = dp[k] - (int)c0;            // local slot of the diagonal
 
     for (int i = lane; i < m; i += kWave) x[i] = 0.0;
     wave_fence();
-    for (int p = M.aptr[k] + lane; p < M.aptr[k + 1]; p += kWave) {
-        int s = M.aslot[p];
-        if (s >= 0) x[s] = M.ain[M.asrc[p]];
+    {
+        const LSP_GLOBAL int *aslot = gp(M.aslot);
+        const LSP_GLOBAL int *asrc = gp(M.asrc);
+        const LSP_GLOBAL double *ain = gp(M.ain);
+        const int a1 = gp(M.aptr)[k + 1];
+        for (int p = gp(M.aptr)[k] + lane; p < a1; p += kWave) {
+            const int s = aslot[p];
+            if (s >= 0) x[s] = ain[asrc[p]];
+        }
     }
     wave_fence();
 
@@ -67,17 +68,17 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, double *x,
     // outside the block (a prefix of its U slots); the block itself is
     // factored densely afterwards.
     const bool tail = k >= M.tail_k0;
-    const int ndep = tail ? M.tail_ds[k - M.tail_k0] : dl;
-    int64_t u = M.relp[k];
+    const int ndep = tail ? gp(M.tail_ds)[k - M.tail_k0] : dl;
+    int64_t u = gp(M.relp)[k];
     for (int s0 = 0; s0 < ndep; s0 += kWave) {
         const int nd = min(kWave, ndep - s0);
         int l0 = 0, len = 0;
         const int *flag = nullptr;
         if (lane < nd) {
-            const int j = M.pos[c0 + s0 + lane];
+            const int j = pos[c0 + s0 + lane];
             flag = &M.done[j];
-            l0 = M.dp[j] + 1;
-            len = (int)(M.cp[j + 1] - l0);
+            l0 = dp[j] + 1;
+            len = (int)(cp[j + 1] - l0);
         }
         // offsets of each dependency's update-map entries within the chunk
         int uoff = len;
@@ -88,28 +89,48 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, double *x,
         }
         const int chunk_len = __shfl(uoff, kWave - 1, kWave);
         uoff -= len;
+        const long long tw = waited ? (long long)clock64() : 0;
         if (!wave_wait(flag, epoch, abort_word, dbg, k)) return;   // aborted; kernel drains
-        // software pipeline: the L values and target slots of dependency d+1
-        // are loaded while dependency d updates x
-        int jl0 = __builtin_amdgcn_readlane(l0, 0), jlen = __builtin_amdgcn_readlane(len, 0);
-        int64_t ub = u + __builtin_amdgcn_readlane(uoff, 0);
-        int tg = 0;
-        double lv = 0.0;
-        if (lane < jlen) { tg = M.rel[ub + lane]; lv = to_f64(ld_coherent_val(&M.val[jl0 + lane])); }
-        for (int d = 0; d < nd; d++) {
-            int njl0 = 0, njlen = 0, ntg = 0;
-            int64_t nub = 0;
-            double nlv = 0.0;
-            if (d + 1 < nd) {
-                njl0 = __builtin_amdgcn_readlane(l0, d + 1);
-                njlen = __builtin_amdgcn_readlane(len, d + 1);
-                nub = u + __builtin_amdgcn_readlane(uoff, d + 1);
-                if (lane < njlen) { ntg = M.rel[nub + lane]; nlv = to_f64(ld_coherent_val(&M.val[njl0 + lane])); }
+        if (waited) *waited += (long long)clock64() - tw;
+        // software pipeline: the first 64 L values and target slots of the
+        // next kPre dependencies are in flight while dependency d updates x
+        int ptg[kPre];
+        T plv[kPre];
+#pragma unroll
+        for (int r = 0; r < kPre; r++) {
+            ptg[r] = 0;
+            plv[r] = T(0);
+            if (r < nd) {
+                const int pl0 = __builtin_amdgcn_readlane(l0, r), plen = __builtin_amdgcn_readlane(len, r);
+                const int64_t pub = u + __builtin_amdgcn_readlane(uoff, r);
+                if (lane < plen) { ptg[r] = rel[pub + lane]; plv[r] = ld_coherent_val(M.val + pl0 + lane); }
             }
-            const double xj = x[s0 + d];
-            apply_dep(M, x, ub, jl0, jlen, xj, tg, lv);
-            wave_fence();
-            jl0 = njl0; jlen = njlen; ub = nub; tg = ntg; lv = nlv;
+        }
+        for (int d0 = 0; d0 < nd; d0 += kPre) {
+#pragma unroll
+            for (int r = 0; r < kPre; r++) {
+                const int d = d0 + r;
+                if (d < nd) {
+                    const int jl0 = __builtin_amdgcn_readlane(l0, d), jlen = __builtin_amdgcn_readlane(len, d);
+                    const int64_t ub = u + __builtin_amdgcn_readlane(uoff, d);
+                    const int tg = ptg[r];
+                    const double lv = to_f64(plv[r]);
+                    const int dn = d + kPre;                  // refill this ring slot
+                    if (dn < nd) {
+                        const int nl0 = __builtin_amdgcn_readlane(l0, dn), nlen = __builtin_amdgcn_readlane(len, dn);
+                        const int64_t nub = u + __builtin_amdgcn_readlane(uoff, dn);
+                        if (lane < nlen) { ptg[r] = rel[nub + lane]; plv[r] = ld_coherent_val(M.val + nl0 + lane); }
+                    }
+                    // the multiplier and the first targets are read together:
+                    // one LDS round trip per dependency on the sequential chain
+                    const double xt = x[tg];
+                    const double xj = x[s0 + d];
+                    if (lane < jlen) x[tg] = xt - lv * xj;
+                    for (int e = lane + kWave; e < jlen; e += kWave)
+                        x[rel[ub + e]] -= to_f64(ld_coherent_val(M.val + jl0 + e)) * xj;
+                    wave_fence();
+                }
+            }
         }
         u += chunk_len;
     }
@@ -120,14 +141,14 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, double *x,
         const int c = k - M.tail_k0;
         for (int i = lane; i < ndep; i += kWave) {
             const T v = from_f64<T>(x[i]);
-            M.val[c0 + i] = v;
-            M.urow[M.dst[c0 + i]] = v;
+            st_g(&M.val[c0 + i], v);
+            st_g(&M.urow[gp(M.dst)[c0 + i]], v);
         }
-        double *Sc = M.tail_S + (size_t)c * (size_t)M.tail_T;
+        LSP_GLOBAL double *Sc = gp(M.tail_S) + (size_t)c * (size_t)M.tail_T;
         for (int r = lane; r < M.tail_T; r += kWave) Sc[r] = 0.0;
         wave_fence();                           // same-address stores of one wave stay ordered
-        for (int s = ndep + lane; s < m; s += kWave) Sc[M.pos[c0 + s] - M.tail_k0] = x[s];
-        if (lane == 0) M.tail_colmax[c] = 0ull;
+        for (int s = ndep + lane; s < m; s += kWave) Sc[pos[c0 + s] - M.tail_k0] = x[s];
+        if (lane == 0) gp(M.tail_colmax)[c] = 0ull;
         return;
     }
 
@@ -153,20 +174,21 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, double *x,
     // L(:, k) is read by later columns of this launch: coherent stores, then
     // the flag once they are acknowledged.  U, the diagonal and the row-wise
     // copies are read only after the launch.
+    const LSP_GLOBAL int *dst = gp(M.dst);
     for (int i = lane; i < dl; i += kWave) {
         const T v = from_f64<T>(x[i]);
-        M.val[c0 + i] = v;
-        M.urow[M.dst[c0 + i]] = v;
+        st_g(&M.val[c0 + i], v);
+        st_g(&M.urow[dst[c0 + i]], v);
     }
     if (lane == 0) {
         const T v = from_f64<T>(piv);
-        M.val[c0 + dl] = v;
-        M.diag[k] = v;
+        st_g(&M.val[c0 + dl], v);
+        st_g(&M.diag[k], v);
     }
     for (int i = dl + 1 + lane; i < m; i += kWave) {
         const T v = from_f64<T>(x[i] / piv);
         st_coherent(&M.val[c0 + i], v);
-        M.lrow[M.dst[c0 + i]] = v;
+        st_g(&M.lrow[dst[c0 + i]], v);
     }
     store_complete();
     if (lane == 0) st_relaxed(&M.done[k], epoch);
@@ -178,8 +200,8 @@ __global__ void __launch_bounds__(256) refactor_kernel(FactorLaunch L)
     extern __shared__ double lds[];
     const int wv = threadIdx.x >> 6;
     const int waves_per_block = blockDim.x >> 6;
-    double *xl = lds + (long)wv * L.cap;
-    double *xg = L.scratch + ((long)blockIdx.x * waves_per_block + wv) * L.scratch_stride;
+    LSP_LDS double *xl = (LSP_LDS double *)lds + (long)wv * L.cap;
+    LSP_GLOBAL double *xg = gp(L.scratch) + ((long)blockIdx.x * waves_per_block + wv) * L.scratch_stride;
     const FactorDesc<T> *mats = (const FactorDesc<T> *)L.mats;
     const int epoch = read_epoch(L.epoch);
 
@@ -192,12 +214,22 @@ __global__ void __launch_bounds__(256) refactor_kernel(FactorLaunch L)
         const FactorDesc<T> M = mats[tk.x];
         const int k = tk.y;
         const int m = (int)(M.cp[k + 1] - M.cp[k]);
+        long long waited = 0;
+        long long *wp = L.timeline ? &waited : nullptr;
+        const long long t0 = L.timeline ? (long long)wall_clock64() : 0;
         if (m <= L.cap) {
-            factor_column<T>(M, k, xl, epoch, L.abort_word, L.ptol, L.dbg);
+            factor_column<T>(M, k, xl, epoch, L.abort_word, L.ptol, L.dbg, wp);
         } else if (m <= L.scratch_stride) {
-            factor_column<T>(M, k, xg, epoch, L.abort_word, L.ptol, L.dbg);
+            factor_column<T>(M, k, xg, epoch, L.abort_word, L.ptol, L.dbg, wp);
         } else {
             if (lane_id() == 0) st_relaxed(L.abort_word, kErrCapacity);
+        }
+        if (L.timeline && lane_id() == 0) {
+            long long *r = L.timeline + 4L * t;
+            r[0] = t0;
+            r[1] = (long long)wall_clock64();
+            r[2] = waited;
+            r[3] = k;
         }
     }
     // the last wave out re-arms the queue and advances the epoch

@@ -23,6 +23,8 @@ struct FactorLaunch {
     int total_waves;
     double ptol;
     int *dbg;               // 8 ints: first stuck wait (dep, seen, epoch, spins, column)
+    long long *timeline;    // optional, 4 per task: start, end (100 MHz wall clock),
+                            // shader clocks spent waiting on dependencies, column
 };
 
 struct TriLaunch {

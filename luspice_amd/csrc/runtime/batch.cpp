@@ -550,7 +550,12 @@ int Batch::upload_all()
     }
     // persistent grids
     {
+        // LDS doubles per refactor wave: the requested cap, raised to hold the
+        // longest column when that fits 4 waves in the 160 KB of a CU
+        int maxcol_all = 0;
+        for (auto &C : circ_) maxcol_all = std::max(maxcol_all, C.plan.max_col);
         int cap = opt_.lds_cap > 0 ? opt_.lds_cap : 1024;
+        if (maxcol_all > cap && maxcol_all <= 4096) cap = (maxcol_all + 255) / 256 * 256;
         opt_.lds_cap = cap;
         const size_t lds = (size_t)cap * 8 * 4;
         if (lds > 65536) {
@@ -682,6 +687,12 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
     L.active = d_active;
     L.ntask = (int)ntot_;
     L.epoch = d_epoch_;
+    L.timeline = nullptr;
+    const char *tl_path = getenv("LSP_REFACTOR_TIMELINE");   // diagnostic dump
+    if (tl_path) {
+        if (!d_timeline_) d_timeline_ = (long long *)dalloc((size_t)ntot_ * 4 * sizeof(long long));
+        L.timeline = d_timeline_;
+    }
     L.ctr = d_ctr_;
     L.abort_word = d_abort_;
     L.scratch = d_scratch_;
@@ -741,6 +752,18 @@ int Batch::refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *
         if (ms[4 * m + 2] != 0x7fffffff) r.singular_circuits++;
         if ((ms[4 * m] || ms[4 * m + 2] != 0x7fffffff || ms[4 * m + 3]) && r.first_bad_circuit < 0)
             r.first_bad_circuit = m;
+    }
+    const char *tl_path = getenv("LSP_REFACTOR_TIMELINE");
+    if (tl_path && d_timeline_) {
+        std::vector<long long> tl((size_t)ntot_ * 4);
+        (void)hipMemcpy(tl.data(), d_timeline_, tl.size() * sizeof(long long), hipMemcpyDeviceToHost);
+        std::vector<int2> tk(ntot_);
+        (void)hipMemcpy(tk.data(), d_ftask_, ntot_ * sizeof(int2), hipMemcpyDeviceToHost);
+        if (FILE *f = fopen(tl_path, "wb")) {
+            fwrite(tl.data(), sizeof(long long), tl.size(), f);
+            fwrite(tk.data(), sizeof(int2), tk.size(), f);
+            fclose(f);
+        }
     }
     if (abort_code == kErrSpin) {
         int dbg[8] = {0};
