@@ -61,10 +61,18 @@ struct LspPlan {
     std::vector<int> aptr;                // n+1 (q-order CSC column pointers)
     std::vector<int> asrc;                // nnz_a: index into the caller's value array
     std::vector<int> aslot;               // nnz_a: local slot, -1 = overwritten duplicate
-    // left-looking update map: for column k, for each U slot (dep j,
-    // ascending), for each L slot t of column j: local slot of pos[t] in k
+    // Supernodes of L among positions [0, tail_k0): j and j+1 share one when
+    // L(:, j+1) = L(:, j) minus row j+1.  snend[j] = one past the last column
+    // of j's supernode (j+1 for a single column).
+    std::vector<int> snend;               // n
+    // Segment update maps.  The dependencies of column k that lie in one
+    // supernode D form a run of consecutive U slots (positions j_s .. e-1 with
+    // e = min(snend, k)); every dependency's L rows are the run's later
+    // positions followed by L(:, e-1).  For each run, in slot order, the map
+    // holds the local slot in k of every row of L(:, e-1).
     std::vector<int64_t> relp;            // n+1
     std::vector<uint16_t> rel;            // local slots (max_col <= 65535)
+    int64_t n_segments = 0;               // runs over all columns
     int max_col = 0;                      // max column pattern length
     // dependency levels of the factorization (U pattern, level.c semantics)
     std::vector<int> level;               // n
@@ -102,6 +110,12 @@ struct LspPlan {
 // below k0 only, backward levels recomputed with the tail rows solved first)
 // and the tail maps.  Returns T.
 int lsp_plan_set_tail(LspPlan &P, int min_T, int max_T, double min_density);
+
+// Supernodes over positions [0, tail_k0) and the per-segment update maps
+// (called by lsp_build_plan and lsp_plan_set_tail).  Returns 0, or <0 when a
+// run does not end at its supernode boundary or a row falls outside the
+// target pattern (both mean the symbolic pattern is not closed under fill).
+int lsp_plan_build_segments(LspPlan &P);
 
 // Build the plan from A (q-order CSC, values only used for nothing here) and
 // the symbolic factors.  src_index maps each q-CSC entry to the caller's value

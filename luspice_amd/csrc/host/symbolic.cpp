@@ -196,19 +196,14 @@ int lsp_build_plan(int n, const int *Ap, const int *Ai, const int *src_index,
     P.aptr.assign(Ap, Ap + n + 1);
     P.asrc.assign(P.nnz_a, 0);
     P.aslot.assign(P.nnz_a, -1);
-    std::vector<int64_t> last_writer;
-    // count updates first
-    P.relp.assign(n + 1, 0);
-    for (int k = 0; k < n; k++) {
-        int64_t cnt = 0;
+    // left-looking update count (statistic; the map itself is built per
+    // supernode segment by lsp_plan_build_segments)
+    P.n_updates = 0;
+    for (int k = 0; k < n; k++)
         for (int64_t s = P.cp[k]; s < P.dp[k]; s++) {
-            int j = P.pos[s];
-            cnt += P.cp[j + 1] - P.dp[j] - 1;
+            const int j = P.pos[s];
+            P.n_updates += P.cp[j + 1] - P.dp[j] - 1;
         }
-        P.relp[k + 1] = P.relp[k] + cnt;
-    }
-    P.n_updates = P.relp[n];
-    P.rel.assign(P.n_updates, 0);
     std::vector<int> slot_writer;
     for (int k = 0; k < n; k++) {
         for (int64_t s = P.cp[k]; s < P.cp[k + 1]; s++) {
@@ -227,15 +222,6 @@ int lsp_build_plan(int n, const int *Ap, const int *Ai, const int *src_index,
             slot_writer[loc] = p;
             P.aslot[p] = loc;
             P.asrc[p] = src_index ? src_index[p] : p;
-        }
-        int64_t u = P.relp[k];
-        for (int64_t s = P.cp[k]; s < P.dp[k]; s++) {
-            int j = P.pos[s];
-            for (int64_t t = P.dp[j] + 1; t < P.cp[j + 1]; t++) {
-                int ps = P.pos[t];
-                if (owner[ps] != k) return -8;     // fill outside the pattern
-                P.rel[u++] = (uint16_t)where[ps];
-            }
         }
     }
     // factor levels (src/level.c:20-40 on the U pattern)
@@ -302,6 +288,58 @@ int lsp_build_plan(int n, const int *Ap, const int *Ai, const int *src_index,
     P.tail_k0 = n;
     P.tail_T = 0;
     P.fkey = P.level;
+    return lsp_plan_build_segments(P);
+}
+
+// ---------------------------------------------------------------------------
+// supernode segments
+// ---------------------------------------------------------------------------
+int lsp_plan_build_segments(LspPlan &P)
+{
+    const int n = P.n;
+    const int k0 = P.tail_k0;
+    auto llen = [&](int j) { return (int)(P.cp[j + 1] - P.dp[j] - 1); };
+    // j joins j-1's supernode when L(:, j-1) = {j} + L(:, j)
+    P.snend.assign(n, 0);
+    for (int j = n - 1; j >= 0; j--) {
+        bool next_joins = false;
+        const int c = j + 1;
+        if (c < k0 && llen(j) == llen(c) + 1 && llen(j) > 0 && P.pos[P.dp[j] + 1] == c) {
+            next_joins = true;
+            const int64_t a = P.dp[j] + 2, b = P.dp[c] + 1;
+            for (int t = 0; t < llen(c); t++)
+                if (P.pos[a + t] != P.pos[b + t]) { next_joins = false; break; }
+        }
+        P.snend[j] = next_joins ? P.snend[c] : j + 1;
+    }
+    // runs and their maps
+    std::vector<int> where(n, -1), owner(n, -1);
+    P.relp.assign(n + 1, 0);
+    P.rel.clear();
+    P.n_segments = 0;
+    for (int k = 0; k < n; k++) {
+        const int64_t c0 = P.cp[k];
+        for (int64_t s = c0; s < P.cp[k + 1]; s++) {
+            where[P.pos[s]] = (int)(s - c0);
+            owner[P.pos[s]] = k;
+        }
+        const int ndep = k < k0 ? P.dp[k] - (int)c0 : P.tail_ds[k - k0];
+        for (int t = 0; t < ndep; t++) {
+            const int j = P.pos[c0 + t];
+            const int e = std::min(P.snend[j], k);
+            const bool last = t + 1 == ndep || P.pos[c0 + t + 1] != j + 1 ||
+                              std::min(P.snend[P.pos[c0 + t + 1]], k) != e;
+            if (!last) continue;
+            if (j != e - 1) return -10;            // run stops inside its supernode
+            P.n_segments++;
+            for (int64_t s = P.dp[j] + 1; s < P.cp[j + 1]; s++) {
+                const int ps = P.pos[s];
+                if (owner[ps] != k) return -8;     // fill outside the pattern
+                P.rel.push_back((uint16_t)where[ps]);
+            }
+        }
+        P.relp[k + 1] = (int64_t)P.rel.size();
+    }
     return 0;
 }
 
@@ -328,7 +366,7 @@ int lsp_plan_set_tail(LspPlan &P, int min_T, int max_T, double min_density)
     P.tail_lcut.assign(best, 0);
     P.tail_updates = 0;
     P.fkey = P.level;
-    if (best == 0) return 0;
+    if (best == 0) return lsp_plan_build_segments(P);
     for (int c = 0; c < best; c++) {
         const int k = k0 + c;
         int ds = 0;
@@ -346,28 +384,6 @@ int lsp_plan_set_tail(LspPlan &P, int min_T, int max_T, double min_density)
         int e = P.lrp[k];
         while (e < P.lrp[k + 1] && P.lci[e] < k0) e++;
         P.tail_lcut[c] = e;
-    }
-    // the update map of a block column keeps only its dependencies outside
-    // the block (a prefix of the column's entries)
-    {
-        std::vector<int64_t> nrelp(n + 1, 0);
-        for (int k = 0; k < n; k++) {
-            int64_t cnt = P.relp[k + 1] - P.relp[k];
-            if (k >= k0) {
-                cnt = 0;
-                for (int s = 0; s < P.tail_ds[k - k0]; s++) {
-                    const int j = P.pos[P.cp[k] + s];
-                    cnt += P.cp[j + 1] - P.dp[j] - 1;
-                }
-            }
-            nrelp[k + 1] = nrelp[k] + cnt;
-        }
-        std::vector<uint16_t> nrel((size_t)nrelp[n]);
-        for (int k = 0; k < n; k++)
-            std::copy(P.rel.begin() + P.relp[k], P.rel.begin() + P.relp[k] + (nrelp[k + 1] - nrelp[k]),
-                      nrel.begin() + nrelp[k]);
-        P.rel.swap(nrel);
-        P.relp.swap(nrelp);
     }
     int nkey = 0;
     for (int k = 0; k < n; k++) nkey = std::max(nkey, P.fkey[k] + 1);
@@ -405,5 +421,6 @@ int lsp_plan_set_tail(LspPlan &P, int min_T, int max_T, double min_density)
         for (int i = 0; i < k0; i++) bw[cnt[P.lvb[i]]++] = i;
     }
     P.bwd_task.swap(bw);
-    return best;
+    const int rc = lsp_plan_build_segments(P);
+    return rc != 0 ? rc : best;
 }

@@ -134,6 +134,16 @@ template <typename T> __device__ __forceinline__ T ld_coherent_val(const T *p)
 {
     return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Element i of a wave-uniform base (SGPR base + 32-bit lane offset).
+template <typename T> __device__ __forceinline__ T ld_coherent_val(const LSP_GLOBAL T *base, uint32_t i)
+{
+    return __hip_atomic_load(base + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <> __device__ __forceinline__ __half ld_coherent_val<__half>(const LSP_GLOBAL __half *base, uint32_t i)
+{
+    return __ushort_as_half(__hip_atomic_load((const LSP_GLOBAL unsigned short *)base + i, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT));
+}
 template <> __device__ __forceinline__ __half ld_coherent_val<__half>(const __half *p)
 {
     return __ushort_as_half(

@@ -26,8 +26,9 @@ struct FactorDesc {
     const int *asrc;        // A entry -> input value index
     const int *aslot;       // A entry -> local slot (-1 skipped duplicate)
     const double *ain;      // input values (fp64, caller order)
-    const int64_t *relp;    // n+1 update-map pointers
-    const uint16_t *rel;    // update map (local slots)
+    const int64_t *relp;    // n+1 segment-map pointers
+    const uint16_t *rel;    // segment maps: local slots of each run's last L rows
+    const int *snend;       // n: one past the last column of j's supernode
     const int *dst;         // slot -> index in lrow / urow / diag
     T *val;                 // column storage of L and U
     T *lrow, *urow, *diag;  // row-wise copies for the trisolve

@@ -32,20 +32,72 @@
 namespace lsp {
 
 
-constexpr int kPre = 8;                 // dependencies in flight ahead of the update
+// Wave-uniform broadcast of lane `l`'s double (l uniform).
+__device__ __forceinline__ double bcast_f64(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+constexpr int kTri = kWave - 1;         // triangle steps held in registers
+
+// Unit-lower triangular solve of one run held one entry per lane (lanes ta ..
+// tb-1): step q broadcasts x[j_{ta+q}] and subtracts L(j_t, j_{ta+q}) x from
+// every later lane t of the run.  The NT triangle columns are all loaded
+// before the first step (columns past the run load a valid duplicate and are
+// masked), so the chain waits on registers only.
+template <int NT, typename T>
+__device__ __forceinline__ double run_triangle(const T *val, double xc, int lb, int ta, int tb)
+{
+    const int lane = lane_id();
+    const int bw = tb - ta;
+    T tv[NT];
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+        const int qq = min(q, bw - 2);
+        const int tp = ta + qq;
+        const int base = __builtin_amdgcn_readlane(lb, tp);
+        const int off = min(max(lane - tp - 1, 0), bw - qq - 2);
+        tv[q] = ld_coherent_val(gp(val) + (uint32_t)base, (uint32_t)off);
+    }
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+        const int tp = min(ta + q, kWave - 1);
+        const double ut = bcast_f64(xc, tp);
+        const double nx = fma(-to_f64(tv[q]), ut, xc);
+        xc = (lane > tp && lane < tb) ? nx : xc;
+    }
+    return xc;
+}
+constexpr int kGemvBatch = 16;          // L values in flight per GEMV row pass
 
 // One column of the left-looking refactorization.  XP is the work vector's
 // pointer type: an LSP_LDS pointer when the column fits the wave's LDS slice,
 // an LSP_GLOBAL scratch pointer otherwise.
+//
+// Dependencies are applied by supernode segment.  The U slots of column k
+// whose rows j_s .. e-1 are consecutive columns of one supernode D (e =
+// min(end of D, k)) form a run; inside the run L(j_t, j_t') is entry
+// (t - t' - 1) of column j_t', and below the run every column of D holds
+// the same rows (the run's later rows, then L(:, e-1)).  So a run is
+//   * a unit-lower triangular solve on the run's x entries, done in registers
+//     (lane t holds x[j_t]; step t' broadcasts x[j_t'] with readlane), and
+//   * a dense update of the rows below: row i gets sum_t' L(i, j_t') x[j_t'],
+//     with the L values read as coalesced runs and one x update per row.
+// Columns outside any supernode are runs of length 1.  Slots are taken 64 at
+// a time; each lane fetches the metadata and waits on the flag of one slot.
 template <typename T, typename XP>
 __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
-                              int epoch, int *abort_word, double ptol, int *dbg, long long *waited)
+                              int epoch, int *abort_word, double ptol, int *dbg, long long *tw)
 {
     const int lane = lane_id();
     const LSP_GLOBAL int64_t *cp = gp(M.cp);
     const LSP_GLOBAL int *dp = gp(M.dp);
     const LSP_GLOBAL int *pos = gp(M.pos);
-    const LSP_GLOBAL uint16_t *rel = gp(M.rel);
+    const LSP_GLOBAL int *snend = gp(M.snend);
+    const LSP_GLOBAL uint16_t *map = gp(M.rel);
     const int64_t c0 = cp[k];
     const int m = (int)(cp[k + 1] - c0);
     const int dl = dp[k] - (int)c0;            // local slot of the diagonal
@@ -69,70 +121,67 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
     // factored densely afterwards.
     const bool tail = k >= M.tail_k0;
     const int ndep = tail ? gp(M.tail_ds)[k - M.tail_k0] : dl;
-    int64_t u = gp(M.relp)[k];
+    int64_t u = gp(M.relp)[k];                 // map cursor
     for (int s0 = 0; s0 < ndep; s0 += kWave) {
         const int nd = min(kWave, ndep - s0);
-        int l0 = 0, len = 0;
+        int j = -2, e = -2, lb = 0, nl = 0;
         const int *flag = nullptr;
         if (lane < nd) {
-            const int j = pos[c0 + s0 + lane];
+            j = pos[c0 + s0 + lane];
+            e = min(snend[j], k);
+            lb = dp[j] + 1;                    // first L slot of column j
+            nl = (int)(cp[e] - dp[e - 1]) - 1; // |L(:, e-1)|
             flag = &M.done[j];
-            l0 = dp[j] + 1;
-            len = (int)(cp[j + 1] - l0);
         }
-        // offsets of each dependency's update-map entries within the chunk
-        int uoff = len;
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const int v = __shfl_up(uoff, o, kWave);
-            if (lane >= o) uoff += v;
-        }
-        const int chunk_len = __shfl(uoff, kWave - 1, kWave);
-        uoff -= len;
-        const long long tw = waited ? (long long)clock64() : 0;
+        const int jp = __shfl_up(j, 1, kWave);
+        const int ep = __shfl_up(e, 1, kWave);
+        const bool head = lane < nd && (lane == 0 || j != jp + 1 || e != ep);
+        unsigned long long heads = __ballot(head);
         if (!wave_wait(flag, epoch, abort_word, dbg, k)) return;   // aborted; kernel drains
-        if (waited) *waited += (long long)clock64() - tw;
-        // software pipeline: the first 64 L values and target slots of the
-        // next kPre dependencies are in flight while dependency d updates x
-        int ptg[kPre];
-        T plv[kPre];
-#pragma unroll
-        for (int r = 0; r < kPre; r++) {
-            ptg[r] = 0;
-            plv[r] = T(0);
-            if (r < nd) {
-                const int pl0 = __builtin_amdgcn_readlane(l0, r), plen = __builtin_amdgcn_readlane(len, r);
-                const int64_t pub = u + __builtin_amdgcn_readlane(uoff, r);
-                if (lane < plen) { ptg[r] = rel[pub + lane]; plv[r] = ld_coherent_val(M.val + pl0 + lane); }
+        if (tw) *tw = (long long)wall_clock64();
+        double xc = lane < nd ? (double)x[s0 + lane] : 0.0;
+        while (heads) {
+            const int ta = __builtin_ctzll(heads);
+            heads &= heads - 1;
+            const int tb = heads ? __builtin_ctzll(heads) : nd;
+            const int bw = tb - ta;
+            const int jl = __builtin_amdgcn_readlane(e, ta) - 1;        // last column of the run
+            const int tri = jl - __builtin_amdgcn_readlane(j, tb - 1);  // run rows in later chunks
+            const int nlast = __builtin_amdgcn_readlane(nl, ta);
+            if (bw > 1) {
+                if (bw <= 9) xc = run_triangle<8>(M.val, xc, lb, ta, tb);
+                else if (bw <= 25) xc = run_triangle<24>(M.val, xc, lb, ta, tb);
+                else xc = run_triangle<kTri>(M.val, xc, lb, ta, tb);
             }
-        }
-        for (int d0 = 0; d0 < nd; d0 += kPre) {
+            if (lane >= ta && lane < tb) x[s0 + lane] = xc;
+            // rows below the run's part in this chunk: the run's later rows,
+            // then the rows of L(:, jl) through the map
+            const int R = tri + nlast;
+            for (int i0 = 0; i0 < R; i0 += kWave) {
+                const int i = min(i0 + lane, R - 1);
+                double acc = 0.0;
+                for (int q0 = 0; q0 < bw; q0 += kGemvBatch) {
+                    T v[kGemvBatch];
 #pragma unroll
-            for (int r = 0; r < kPre; r++) {
-                const int d = d0 + r;
-                if (d < nd) {
-                    const int jl0 = __builtin_amdgcn_readlane(l0, d), jlen = __builtin_amdgcn_readlane(len, d);
-                    const int64_t ub = u + __builtin_amdgcn_readlane(uoff, d);
-                    const int tg = ptg[r];
-                    const double lv = to_f64(plv[r]);
-                    const int dn = d + kPre;                  // refill this ring slot
-                    if (dn < nd) {
-                        const int nl0 = __builtin_amdgcn_readlane(l0, dn), nlen = __builtin_amdgcn_readlane(len, dn);
-                        const int64_t nub = u + __builtin_amdgcn_readlane(uoff, dn);
-                        if (lane < nlen) { ptg[r] = rel[nub + lane]; plv[r] = ld_coherent_val(M.val + nl0 + lane); }
+                    for (int h = 0; h < kGemvBatch; h++) {
+                        const int tp = ta + min(q0 + h, bw - 1);
+                        const int base = __builtin_amdgcn_readlane(lb, tp) + (tb - tp - 1);
+                        v[h] = ld_coherent_val(gp(M.val) + (uint32_t)base, (uint32_t)i);
                     }
-                    // the multiplier and the first targets are read together:
-                    // one LDS round trip per dependency on the sequential chain
-                    const double xt = x[tg];
-                    const double xj = x[s0 + d];
-                    if (lane < jlen) x[tg] = xt - lv * xj;
-                    for (int e = lane + kWave; e < jlen; e += kWave)
-                        x[rel[ub + e]] -= to_f64(ld_coherent_val(M.val + jl0 + e)) * xj;
-                    wave_fence();
+#pragma unroll
+                    for (int h = 0; h < kGemvBatch; h++)
+                        if (q0 + h < bw) acc = fma(to_f64(v[h]), bcast_f64(xc, ta + q0 + h), acc);
+                }
+                if (i0 + lane < R) {
+                    const int slot = i < tri ? s0 + tb + i : (int)map[u + (i - tri)];
+                    x[slot] -= acc;
                 }
             }
+            if (tri == 0) u += nlast;
+            wave_fence();
+            if (lane >= tb && lane < nd) xc = x[s0 + lane];   // later runs of this chunk
         }
-        u += chunk_len;
+        wave_fence();
     }
 
     if (tail) {
@@ -214,8 +263,8 @@ __global__ void __launch_bounds__(256) refactor_kernel(FactorLaunch L)
         const FactorDesc<T> M = mats[tk.x];
         const int k = tk.y;
         const int m = (int)(M.cp[k + 1] - M.cp[k]);
-        long long waited = 0;
-        long long *wp = L.timeline ? &waited : nullptr;
+        long long tw = 0;
+        long long *wp = L.timeline ? &tw : nullptr;
         const long long t0 = L.timeline ? (long long)wall_clock64() : 0;
         if (m <= L.cap) {
             factor_column<T>(M, k, xl, epoch, L.abort_word, L.ptol, L.dbg, wp);
@@ -228,7 +277,7 @@ __global__ void __launch_bounds__(256) refactor_kernel(FactorLaunch L)
             long long *r = L.timeline + 4L * t;
             r[0] = t0;
             r[1] = (long long)wall_clock64();
-            r[2] = waited;
+            r[2] = tw;
             r[3] = k;
         }
     }

@@ -196,7 +196,7 @@ static int build_circuit(CircuitHost &C, const lsp_circuit_in &in, const lsp_opt
     {
         const int tmax = opt.tail_max == 0 ? 8192 : opt.tail_max;
         const double tden = opt.tail_density > 0 ? opt.tail_density : 0.3;
-        if (tmax > 0) lsp_plan_set_tail(C.plan, 64, tmax, tden);
+        if (tmax > 0 && lsp_plan_set_tail(C.plan, 64, tmax, tden) < 0) return LSP_E_STRUCTURE;
     }
     std::vector<int> rp, ci, rsrc;
     lsp_coo_to_csr(C.n, C.nnz, C.row.data(), C.col.data(), rp, ci, rsrc);
@@ -230,6 +230,7 @@ int Batch::ensure_factor_set(FactorSet<T> &fs)
         F.ain = d_ain_ + C.aoff;
         F.relp = d_relp_ + C.o_np1;
         F.rel = d_rel_ + C.o_rel;
+        F.snend = d_snend_ + C.o_n;
         F.dst = d_dst_ + C.o_slot;
         F.val = fs.val + C.o_slot;
         F.lrow = fs.lrow + C.o_l;
@@ -322,7 +323,7 @@ size_t Batch::estimate_device_bytes() const
     const double m = std::max(1, opt_.ir_inner);
     double b = 0;
     b += 16 * (n + nc) + 4 * n + 8 * slots + 4 * (n + nc) + 8 * nnz;     // cp relp dp pos dst aptr asrc aslot
-    b += 8 * (n + nc) + 4 * nlu + 8 * n + 2 * rel;                        // lrp urp lci uci rowat q rel
+    b += 8 * (n + nc) + 4 * nlu + 12 * n + 2 * rel;                       // lrp urp lci uci rowat q snend rel
     b += 8 * (slices + nc) + 8 * n + 8 * sell;                            // SELL structure
     b += 12 * n + 8 * nnz + 12 * sell;                                    // flags, values
     b += 8 * T2 + 16 * T;                                                  // dense block work
@@ -376,7 +377,7 @@ int Batch::upload_all()
     nt_tot_ = o_t; nt2_tot_ = o_t2; ntb_tot_ = o_tb;
     {
         std::vector<int64_t> cp, relp, soff;
-        std::vector<int> dp, pos, aptr, asrc, aslot, dst, lrp, lci, urp, uci, rowat, q;
+        std::vector<int> dp, pos, aptr, asrc, aslot, dst, lrp, lci, urp, uci, rowat, q, snend;
         std::vector<int> srlen, scol, sperm, ssrc;
         std::vector<uint16_t> rel;
         cp.reserve(o_np1); relp.reserve(o_np1); rel.reserve(o_rel);
@@ -396,6 +397,7 @@ int Batch::upload_all()
             uci.insert(uci.end(), P.uci.begin(), P.uci.end());
             rowat.insert(rowat.end(), P.rowat.begin(), P.rowat.end());
             q.insert(q.end(), P.q.begin(), P.q.end());
+            snend.insert(snend.end(), P.snend.begin(), P.snend.end());
             rel.insert(rel.end(), P.rel.begin(), P.rel.end());
             const LspSell &S = C.sell;
             soff.insert(soff.end(), S.off.begin(), S.off.end());
@@ -416,12 +418,12 @@ int Batch::upload_all()
         d_dp_ = upload(dp); d_pos_ = upload(pos); d_aptr_ = upload(aptr);
         d_asrc_ = upload(asrc); d_aslot_ = upload(aslot); d_dst_ = upload(dst);
         d_lrp_ = upload(lrp); d_lci_ = upload(lci); d_urp_ = upload(urp); d_uci_ = upload(uci);
-        d_rowat_ = upload(rowat); d_q_ = upload(q); d_rel_ = upload(rel);
+        d_rowat_ = upload(rowat); d_q_ = upload(q); d_rel_ = upload(rel); d_snend_ = upload(snend);
         d_srlen_ = upload(srlen); d_scol_ = upload(scol); d_sperm_ = upload(sperm);
         d_ssrc_ = upload(ssrc);
         if (!d_cp_ || !d_relp_ || !d_soff_ || !d_dp_ || !d_pos_ || !d_aptr_ || !d_asrc_ ||
             !d_aslot_ || !d_dst_ || !d_lrp_ || !d_lci_ || !d_urp_ || !d_uci_ || !d_rowat_ ||
-            !d_q_ || !d_rel_ || !d_srlen_ || !d_scol_ || !d_sperm_ || !d_ssrc_)
+            !d_q_ || !d_rel_ || !d_snend_ || !d_srlen_ || !d_scol_ || !d_sperm_ || !d_ssrc_)
             return LSP_E_ALLOC;
     }
     // flags start at 0; epochs start at 1
