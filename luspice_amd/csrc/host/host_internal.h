@@ -83,8 +83,18 @@ struct LspPlan {
     std::vector<int> urp, uci;            // U strictly upper, CSR by position
     std::vector<int> dst;                 // nslots: index into lrow/urow/diag
     std::vector<int> lvf, lvb;            // row levels forward / backward
-    int nlv_f = 0, nlv_b = 0;
-    std::vector<int> fwd_task, bwd_task;  // rows sorted by level
+    int nlv_f = 0, nlv_b = 0;             // dependency levels of the solve tasks
+    // Trisolve tasks over rows [0, tail_k0), in dependency order.  A row of a
+    // single-column supernode is one task (code 2i); a supernode D = [s, e)
+    // of two or more rows becomes one gather task per row (code 2i+1: the
+    // row's entries outside D, left of s forward / right of e backward, into
+    // a partial sum) and one triangle task (code -(s+1)) that solves D's
+    // dense diagonal block for all its rows.
+    std::vector<int> fwd_task, bwd_task;
+    std::vector<int> fwd_key, bwd_key;    // ordering keys (merge across circuits)
+    std::vector<int> fcut;                // n: end of row i's L entries left of its supernode
+    std::vector<int> bcut;                // n: start of row i's U entries right of its supernode
+    std::vector<int> ru;                  // n: U(j - ru[j] .. j-1, j) is column j's run in its supernode
     // permutations
     std::vector<int> q;                   // n: q[k] = original column
     std::vector<int> rowat;               // n: position -> original row

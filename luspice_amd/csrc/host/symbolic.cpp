@@ -291,6 +291,8 @@ int lsp_build_plan(int n, const int *Ap, const int *Ai, const int *src_index,
     return lsp_plan_build_segments(P);
 }
 
+static void plan_tri_supernodes(LspPlan &P);
+
 // ---------------------------------------------------------------------------
 // supernode segments
 // ---------------------------------------------------------------------------
@@ -340,7 +342,77 @@ int lsp_plan_build_segments(LspPlan &P)
         }
         P.relp[k + 1] = (int64_t)P.rel.size();
     }
+    plan_tri_supernodes(P);
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// supernodal trisolve tasks
+// ---------------------------------------------------------------------------
+static void plan_tri_supernodes(LspPlan &P)
+{
+    const int n = P.n;
+    const int k0 = P.tail_k0;
+    std::vector<int> sst(n);
+    for (int j = 0; j < n; j++) sst[j] = (j > 0 && j < k0 && P.snend[j - 1] == P.snend[j]) ? sst[j - 1] : j;
+    P.fcut.assign(n, 0);
+    P.bcut.assign(n, 0);
+    P.ru.assign(n, 0);
+    for (int i = 0; i < n; i++) {
+        int e = P.lrp[i];
+        while (e < P.lrp[i + 1] && P.lci[e] < sst[i]) e++;
+        P.fcut[i] = e;
+        const int end = i < k0 ? P.snend[i] : i + 1;
+        int f = P.urp[i];
+        while (f < P.urp[i + 1] && P.uci[f] < end) f++;
+        P.bcut[i] = f;
+        int r = 0;
+        for (int64_t t = P.dp[i] - 1; t >= P.cp[i] && P.pos[t] >= sst[i] && P.pos[t] == i - 1 - r; t--) r++;
+        P.ru[i] = r;
+    }
+    // F(D) = 1 + max F over the supernodes D's rows read outside D
+    auto build = [&](bool fwd, std::vector<int> &task, std::vector<int> &key, int &nlv) {
+        std::vector<int> F(k0, 0);
+        int top = 0;
+        auto visit = [&](int i) {
+            const int s = sst[i];
+            int l = F[s];
+            if (fwd) {
+                for (int e = P.lrp[i]; e < P.fcut[i]; e++) l = std::max(l, F[sst[P.lci[e]]] + 1);
+            } else {
+                for (int e = P.bcut[i]; e < P.urp[i + 1]; e++) {
+                    const int j = P.uci[e];
+                    if (j < k0) l = std::max(l, F[sst[j]] + 1);
+                }
+            }
+            F[s] = l;
+        };
+        // a supernode's level is final once all its rows are visited; rows
+        // of earlier supernodes (forward) / later ones (backward) come first
+        if (fwd) for (int i = 0; i < k0; i++) visit(i);
+        else for (int i = k0 - 1; i >= 0; i--) visit(i);
+        std::vector<std::pair<int, int>> items;
+        items.reserve(k0 + k0 / 4);
+        for (int i = 0; i < k0; i++) {
+            const int s = sst[i];
+            const int w = P.snend[s] - s;
+            top = std::max(top, F[s]);
+            if (w == 1) {
+                items.push_back({2 * F[s] + 1, 2 * i});
+            } else {
+                items.push_back({2 * F[s], 2 * i + 1});
+                if (i == s) items.push_back({2 * F[s] + 1, -(s + 1)});
+            }
+        }
+        std::stable_sort(items.begin(), items.end(),
+                         [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first < b.first; });
+        task.resize(items.size());
+        key.resize(items.size());
+        for (size_t t = 0; t < items.size(); t++) { key[t] = items[t].first; task[t] = items[t].second; }
+        nlv = k0 > 0 ? top + 1 : 0;
+    };
+    build(true, P.fwd_task, P.fwd_key, P.nlv_f);
+    build(false, P.bwd_task, P.bwd_key, P.nlv_b);
 }
 
 // ---------------------------------------------------------------------------

@@ -240,4 +240,75 @@ template <> __device__ __forceinline__ double to_f64<__half>(__half v) { return 
 template <typename T> __device__ __forceinline__ T from_f64(double v) { return (T)v; }
 template <> __device__ __forceinline__ __half from_f64<__half>(double v) { return __float2half((float)v); }
 
+// Wave-uniform broadcast of lane `l`'s double (l uniform).
+__device__ __forceinline__ double bcast_f64(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+constexpr int kTri = kWave - 1;         // triangle steps held in registers (64-row blocks)
+
+// Unit-lower triangular solve of one run held one entry per lane (lanes ta ..
+// tb-1): step q broadcasts x[j_{ta+q}] and subtracts L(j_t, j_{ta+q}) x from
+// every later lane t of the run.  The NT triangle columns are all loaded
+// before the first step (columns past the run load a valid duplicate and are
+// masked), so the chain waits on registers only.
+template <int NT, typename T>
+__device__ __forceinline__ double run_triangle(const T *val, double xc, int lb, int ta, int tb)
+{
+    const int lane = lane_id();
+    const int bw = tb - ta;
+    T tv[NT];
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+        const int qq = min(q, bw - 2);
+        const int tp = ta + qq;
+        const int base = __builtin_amdgcn_readlane(lb, tp);
+        const int off = min(max(lane - tp - 1, 0), bw - qq - 2);
+        tv[q] = ld_coherent_val(gp(val) + (uint32_t)base, (uint32_t)off);
+    }
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+        const int tp = min(ta + q, kWave - 1);
+        const double ut = bcast_f64(xc, tp);
+        const double nx = fma(-to_f64(tv[q]), ut, xc);
+        xc = (lane > tp && lane < tb) ? nx : xc;
+    }
+    return xc;
+}
+// Upper triangular solve of a block of bw rows held one per lane (row t of
+// the block in lane t): column q' of the block's U is U(j_t, j_q') =
+// val[dpc_q' - (q' - t)] for t in [q' - ru_q', q'), zero above that run, and
+// rd_t = 1 / U(j_t, j_t).  Steps go from the last column to the first: lane
+// q' finalises z_q' = num_q' rd_q' and every earlier lane subtracts
+// U(j_t, j_q') z_q'.  Returns z_t.
+template <int NT, typename T>
+__device__ __forceinline__ double run_triangle_upper(const T *val, double num, double rd, int dpc, int ru, int bw)
+{
+    const int lane = lane_id();
+    T tv[NT];
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+        const int qq = min(q, bw - 1);
+        const int base = __builtin_amdgcn_readlane(dpc, qq);
+        const int off = min(max(qq - lane, 0), qq);
+        tv[q] = ld_coherent_val(gp(val) + (uint32_t)(base - off), 0u);
+    }
+#pragma unroll
+    for (int q = NT - 1; q >= 0; q--) {
+        if (q < bw) {
+            const double zq = bcast_f64(num, q) * bcast_f64(rd, q);
+            const int run = __builtin_amdgcn_readlane(ru, q);
+            const bool in = lane < q && q - lane <= run;
+            const double nn = fma(-to_f64(tv[q]), zq, num);
+            num = in ? nn : num;
+        }
+    }
+    return num * rd;
+}
+
+
 }  // namespace lsp

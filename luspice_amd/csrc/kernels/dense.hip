@@ -12,9 +12,9 @@
 //   the persistent refactor kernel applies every column outside the block and
 //   leaves S = A22 - L21 U12 (fp64, column-major T x T).  Then, for each
 //   64-column panel p:
-//     dense_diag_kernel   S[p:p+64, p:p+64] = L11 \ U11       (one wave)
+//     dense_diag_kernel   S[p:p+64, p:p+64] = L11 \ U11, and L11^-1, U11^-1
 //     dense_trsm_kernel   L21 = A21 U11^-1 (64-row tiles) and
-//                         U12 = L11^-1 A12 (64-column tiles)   (one wave each)
+//                         U12 = L11^-1 A12 (64-column tiles), as tile products
 //     dense_gemm_kernel   S22 -= L21 U12 (64x64 tiles, fp64 FMA, 4x4 per thread)
 //   dense_finish_kernel  applies the pivot test (|pivot| >= ptol max_i |x_i|,
 //   i.e. ptol * max |L(:,c)| <= 1), scatters the block into the sparse factor
@@ -54,56 +54,206 @@ __device__ __forceinline__ double bcast(double v, int src)
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal block LU, one wave: lane r keeps row r of the 64x64 block in
-// registers; row c reaches the other lanes by readlane.  A partial last block
-// is padded with identity rows/columns, so every loop is fully unrolled.
-__global__ void __launch_bounds__(64) dense_diag_kernel(const DenseJob *jobs, const int *active, int p)
+// Diagonal block: LU of the 64x64 block and the inverses of both factors.
+// 256 threads; thread (r, s) = (tid % 16, tid / 16) keeps elements (r + 16a,
+// s + 16b), a, b < 4, in registers.  A partial last block is padded with
+// identity rows/columns.
+//   * elimination, step c: the owners of row c and column c publish them in
+//     LDS (double-buffered by step parity, one barrier per step); every thread
+//     forms l_i = A(i,c) / A(c,c) and updates its trailing elements.  The
+//     same row operations applied to I build L^-1 (Gauss-Jordan): row c of
+//     L^-1 has no entry right of c and the trailing part of A none left of
+//     c, so one published row carries both.
+//   * U^-1 by rows from the bottom: row k is finalised from its accumulated
+//     sums, then every row above adds U(i,k) times it.
+// Outputs: L\U over the block in S, L^-1 (unit lower) and U^-1 (upper) into
+// the job's workspace W (column-major, leading dimension 64), and the column
+// maxima of |L| for the pivot test.
+constexpr int kWs = 2 * kNb * kNb;
+
+__device__ __forceinline__ double max16(double v)
+{
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+__global__ void __launch_bounds__(256) dense_diag_kernel(const DenseJob *jobs, const int *active, int p)
 {
     const DenseJob J = jobs[blockIdx.x];
     if (p >= J.T || (active && !active[J.mat])) return;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
+    const int r = tid & 15, s = tid >> 4;
     const int b = min(kNb, J.T - p);
     const size_t ld = (size_t)J.T;
     double *Sp = J.S + (size_t)p * ld + p;
-    const int rr = min(lane, b - 1);
-    double a[kNb];
+    __shared__ double bc[2][3][kNb];                   // [parity][row c of A | row c of L^-1 | column c]
+    __shared__ double rpiv[2];
+    __shared__ double dg[kNb];
+    double A[4][4], X[4][4];
 #pragma unroll
-    for (int j = 0; j < kNb; j++) {
-        const double v = Sp[(size_t)min(j, b - 1) * ld + rr];
-        a[j] = (j < b && lane < b) ? v : (j == lane ? 1.0 : 0.0);
-    }
+    for (int a = 0; a < 4; a++) {
+        const int i = r + 16 * a;
 #pragma unroll
-    for (int c = 0; c < kNb; c++) {
-        const double l = a[c] / bcast(a[c], c);
-        if (lane > c) a[c] = l;
-#pragma unroll
-        for (int j = c + 1; j < kNb; j++) {
-            const double u = bcast(a[j], c);
-            if (lane > c) a[j] -= l * u;
+        for (int bb = 0; bb < 4; bb++) {
+            const int j = s + 16 * bb;
+            const double v = Sp[(size_t)min(j, b - 1) * ld + min(i, b - 1)];
+            A[a][bb] = (i < b && j < b) ? v : (i == j ? 1.0 : 0.0);
+            X[a][bb] = 0.0;
         }
     }
+    // The 16-step blocks are unrolled (register indices c/16 are compile-time
+    // constants); the steps inside a block stay a loop.  Masks are applied to
+    // the broadcast operands (row multipliers zero for rows <= c, row values
+    // zero outside their columns), so each element costs one fma; column c is
+    // then overwritten with l and -l.
 #pragma unroll
-    for (int c = 0; c < kNb; c++) {
-        const double am = wave_max(lane > c && lane < b ? fabs(a[c]) : 0.0);
-        if (lane == 0 && c < b) atomicMax(&J.colmax[p + c], dbits(am));
+    for (int ca = 0; ca < 4; ca++)
+#pragma unroll 1
+    for (int cl = 0; cl < 16; cl++) {
+        const int c = 16 * ca + cl;
+        const int par = c & 1;
+        if (r == (c & 15)) {
+#pragma unroll
+            for (int bb = 0; bb < 4; bb++) {
+                bc[par][0][s + 16 * bb] = A[ca][bb];
+                bc[par][1][s + 16 * bb] = X[ca][bb];
+            }
+            if (s == (c & 15)) rpiv[par] = 1.0 / A[ca][ca];
+        }
+        if (s == (c & 15)) {
+#pragma unroll
+            for (int a = 0; a < 4; a++) bc[par][2][r + 16 * a] = A[a][ca];
+        }
+        __syncthreads();
+        const double rp = rpiv[par];
+        double ur[4], xr[4], l[4];
+#pragma unroll
+        for (int bb = 0; bb < 4; bb++) {
+            const int j = s + 16 * bb;
+            ur[bb] = j > c ? bc[par][0][j] : 0.0;
+            xr[bb] = j < c ? bc[par][1][j] : 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const int i = r + 16 * a;
+            l[a] = i > c ? bc[par][2][i] * rp : 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int bb = 0; bb < 4; bb++) {
+                A[a][bb] = fma(-l[a], ur[bb], A[a][bb]);
+                X[a][bb] = fma(-l[a], xr[bb], X[a][bb]);
+            }
+        if (s == (c & 15)) {
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                if (r + 16 * a > c) {
+                    A[a][ca] = l[a];
+                    X[a][ca] = -l[a];
+                }
+            }
+        }
     }
-    if (lane < b) {
+    // U^-1: Y holds the running sums of a row until the row is final
+    if (r == s) {
 #pragma unroll
-        for (int j = 0; j < kNb; j++)
-            if (j < b) Sp[(size_t)j * ld + lane] = a[j];
+        for (int a = 0; a < 4; a++) dg[r + 16 * a] = 1.0 / A[a][a];
+    }
+    double Y[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int bb = 0; bb < 4; bb++) Y[a][bb] = 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int ka = 3; ka >= 0; ka--)
+#pragma unroll 1
+    for (int kl = 15; kl >= 0; kl--) {
+        const int k = 16 * ka + kl;
+        const int par = k & 1;
+        if (r == (k & 15)) {
+            const double rd = dg[k];
+#pragma unroll
+            for (int bb = 0; bb < 4; bb++) {
+                const int j = s + 16 * bb;
+                const double y = j >= k ? ((j == k ? 1.0 : 0.0) - Y[ka][bb]) * rd : 0.0;
+                Y[ka][bb] = y;
+                bc[par][0][j] = y;
+            }
+        }
+        if (s == (k & 15)) {
+#pragma unroll
+            for (int a = 0; a < 4; a++) bc[par][2][r + 16 * a] = A[a][ka];
+        }
+        __syncthreads();
+        double yr[4], u[4];
+#pragma unroll
+        for (int bb = 0; bb < 4; bb++) yr[bb] = bc[par][0][s + 16 * bb];
+#pragma unroll
+        for (int a = 0; a < 4; a++) u[a] = r + 16 * a < k ? bc[par][2][r + 16 * a] : 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int bb = 0; bb < 4; bb++) Y[a][bb] = fma(u[a], yr[bb], Y[a][bb]);
+    }
+    double *W = J.W;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        const int i = r + 16 * a;
+#pragma unroll
+        for (int bb = 0; bb < 4; bb++) {
+            const int j = s + 16 * bb;
+            if (i < b && j < b) Sp[(size_t)j * ld + i] = A[a][bb];
+            W[j * kNb + i] = i > j ? X[a][bb] : (i == j ? 1.0 : 0.0);
+            W[kNb * kNb + j * kNb + i] = i <= j ? Y[a][bb] : 0.0;
+        }
+    }
+    // column maxima of |L| inside the block: the 16 threads of one s are
+    // consecutive lanes of one wave
+#pragma unroll
+    for (int bb = 0; bb < 4; bb++) {
+        const int j = s + 16 * bb;
+        double am = 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const int i = r + 16 * a;
+            if (i > j && i < b) am = fmax(am, fabs(A[a][bb]));
+        }
+        am = max16(am);
+        if (r == 0 && j < b) atomicMax(&J.colmax[p + j], dbits(am));
     }
 }
 
 // ---------------------------------------------------------------------------
-// L21 = A21 U11^-1 (64-row tiles) and U12 = L11^-1 A12 (64-column tiles), one
-// wave per tile.  The factored diagonal block sits in LDS (read as
-// broadcasts); each lane's row (L21) or column (U12) is a padded LDS column,
-// so every LDS access is conflict-free, and both tiles are moved between HBM
-// and LDS with coalesced accesses (lanes along the rows of a column).
-constexpr int kWLd = kNb + 1;                          // padded LDS stride of a tile
-constexpr size_t kTrsmLds = sizeof(double) * (kNb * kNb + kNb * kWLd);
+// 64x64x64 tile product on 256 threads: thread (ty, tx) = (tid % 16, tid / 16)
+// accumulates rows ty + 16 i and columns tx + 16 j (i, j < 4) of As * Bs,
+// As[k*64 + i] = left(i, k), Bs[k*65 + j] = right(k, j).  Consecutive lanes
+// read consecutive LDS words.
+constexpr size_t kGemmLds = sizeof(double) * (kNb * 64 + kNb * kBsLd);
 
-__global__ void __launch_bounds__(64) dense_trsm_kernel(const DenseJob *jobs, const int *active, int p)
+__device__ __forceinline__ void tile_mm(const double *As, const double *Bs, int kmax, double (&acc)[4][4])
+{
+    const int tid = threadIdx.x;
+    const int ty = tid & 15, tx = tid >> 4;
+#pragma unroll 4
+    for (int k = 0; k < kmax; k++) {
+        double a[4], bb[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) a[i] = As[k * 64 + ty + 16 * i];
+#pragma unroll
+        for (int j = 0; j < 4; j++) bb[j] = Bs[k * kBsLd + tx + 16 * j];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = fma(a[i], bb[j], acc[i][j]);
+    }
+}
+
+// L21 = A21 U11^-1 (64-row tiles) and U12 = L11^-1 A12 (64-column tiles) as
+// tile products with the inverses from the diagonal kernel.
+__global__ void __launch_bounds__(256) dense_trsm_kernel(const DenseJob *jobs, const int *active, int p)
 {
     const DenseJob J = jobs[blockIdx.y];
     if (p >= J.T || (active && !active[J.mat])) return;
@@ -114,73 +264,66 @@ __global__ void __launch_bounds__(64) dense_trsm_kernel(const DenseJob *jobs, co
     const int t = blockIdx.x;
     if (t >= 2 * ntile) return;
     extern __shared__ double lds[];
-    double *__restrict__ d = lds;                      // d[j*64 + i] = D(i, j), identity-padded
-    double *__restrict__ w = lds + kNb * kNb;          // tile, element (row i, col c) at w[c*65 + i]
-    const int lane = threadIdx.x;
+    double *As = lds;
+    double *Bs = lds + kNb * 64;
+    const int tid = threadIdx.x;
     const size_t ld = (size_t)J.T;
-    const double *Dp = J.S + (size_t)p * ld + p;
-    // every load loop is unrolled so its 64 loads are in flight together
-    {
-        const int rr = min(lane, b - 1);
-#pragma unroll
-        for (int j = 0; j < kNb; j++) {
-            const double v = Dp[(size_t)min(j, b - 1) * ld + rr];
-            d[j * kNb + lane] = (j < b && lane < b) ? v : (j == lane ? 1.0 : 0.0);
+    const double *W = J.W;
+    const bool lower = t < ntile;
+    const int r0 = p + b + t * kNb;                    // L21 tile rows
+    const int c0 = p + b + (t - ntile) * kNb;          // U12 tile columns
+    for (int e = tid; e < kNb * 64; e += 256) {
+        const int k = e >> 6, i = e & 63;
+        if (lower) {
+            const int rr = r0 + i;
+            As[k * 64 + i] = (rr < J.T && k < b) ? J.S[(size_t)(p + k) * ld + rr] : 0.0;
+            Bs[k * kBsLd + i] = W[kNb * kNb + i * kNb + k];             // U^-1(k, i)
+        } else {
+            As[k * 64 + i] = W[k * kNb + i];                             // L^-1(i, k)
+            const int cc = c0 + k;                                        // column k of the tile, row i
+            Bs[i * kBsLd + k] = (cc < J.T && i < b) ? J.S[(size_t)cc * ld + p + i] : 0.0;
         }
     }
-    if (t < ntile) {
-        // rows r0 + lane of L21: x U11 = a; w[c*65 + lane] = x_c of this lane's row
-        const int r = p + b + t * kNb + lane;
-        const bool ok = r < J.T;
-        const int rc = ok ? r : J.T - 1;
+    __syncthreads();
+    double acc[4][4];
 #pragma unroll
-        for (int c = 0; c < kNb; c++) {
-            const double v = J.S[(size_t)(p + min(c, b - 1)) * ld + rc];
-            w[c * kWLd + lane] = (ok && c < b) ? v : 0.0;
-        }
-        wave_fence();
-        for (int c = 0; c < b; c++) {
-            const double xc = w[c * kWLd + lane] / d[c * kNb + c];
-            w[c * kWLd + lane] = xc;
-#pragma unroll 8
-            for (int j = c + 1; j < b; j++) w[j * kWLd + lane] -= xc * d[j * kNb + c];
-        }
-        for (int c = 0; c < b; c++) {
-            const double v = w[c * kWLd + lane];
-            if (ok) J.S[(size_t)(p + c) * ld + r] = v;
-            const double am = wave_max(ok ? fabs(v) : 0.0);
-            if (lane == 0) atomicMax(&J.colmax[p + c], dbits(am));
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+    tile_mm(As, Bs, kNb, acc);
+    const int ty = tid & 15, tx = tid >> 4;
+    if (lower) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int c = tx + 16 * j;
+            double am = 0.0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int rr = r0 + ty + 16 * i;
+                if (rr < J.T && c < b) {
+                    J.S[(size_t)(p + c) * ld + rr] = acc[i][j];
+                    am = fmax(am, fabs(acc[i][j]));
+                }
+            }
+            am = max16(am);
+            if (ty == 0 && c < b) atomicMax(&J.colmax[p + c], dbits(am));
         }
     } else {
-        // columns c0 + lane of U12: L11 y = a; tile element (row i, column lane)
-        // lives at w[i*65 + lane] (transposed on the way in and out)
-        const int c0 = p + b + (t - ntile) * kNb;
-        const int ncol = min(kNb, J.T - c0);
-        const int rl = min(lane, b - 1);
 #pragma unroll
-        for (int cc = 0; cc < kNb; cc++) {
-            const double v = J.S[(size_t)(c0 + min(cc, ncol - 1)) * ld + p + rl];
-            w[lane * kWLd + cc] = lane < b ? v : 0.0;
+        for (int j = 0; j < 4; j++) {
+            const int cc = c0 + tx + 16 * j;
+            if (cc >= J.T) continue;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int rr = ty + 16 * i;
+                if (rr < b) J.S[(size_t)cc * ld + p + rr] = acc[i][j];
+            }
         }
-        wave_fence();
-        for (int r = 0; r < b; r++) {
-            const double yr = w[r * kWLd + lane];
-#pragma unroll 8
-            for (int i = r + 1; i < b; i++) w[i * kWLd + lane] -= d[r * kNb + i] * yr;
-        }
-        wave_fence();
-        if (lane < b)
-            for (int cc = 0; cc < ncol; cc++) J.S[(size_t)(c0 + cc) * ld + p + lane] = w[lane * kWLd + cc];
     }
 }
 
 // ---------------------------------------------------------------------------
-// S22 -= L21 U12 on 64x64 tiles.  Thread (ty, tx) = (tid % 16, tid / 16) owns
-// rows ty + 16 i and columns tx + 16 j (i, j < 4): consecutive lanes touch
-// consecutive rows, so the tile loads, the LDS reads and the stores are
-// contiguous.
-constexpr size_t kGemmLds = sizeof(double) * (kNb * 64 + kNb * kBsLd);
-
+// S22 -= L21 U12 on 64x64 tiles (same thread layout as tile_mm).
 __global__ void __launch_bounds__(256) dense_gemm_kernel(const DenseJob *jobs, const int *active, int p)
 {
     const DenseJob J = jobs[blockIdx.y];
@@ -201,8 +344,8 @@ __global__ void __launch_bounds__(256) dense_gemm_kernel(const DenseJob *jobs, c
     const size_t ld = (size_t)J.T;
     for (int e = tid; e < b * 64; e += 256) {
         const int k = e >> 6, i = e & 63;
-        const int r = r0 + i;
-        As[k * 64 + i] = r < J.T ? J.S[(size_t)(p + k) * ld + r] : 0.0;
+        const int rr = r0 + i;
+        As[k * 64 + i] = rr < J.T ? J.S[(size_t)(p + k) * ld + rr] : 0.0;
     }
     for (int e = tid; e < b * 64; e += 256) {
         const int j = e / b, k = e - j * b;
@@ -216,25 +359,15 @@ __global__ void __launch_bounds__(256) dense_gemm_kernel(const DenseJob *jobs, c
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
-    for (int k = 0; k < b; k++) {
-        double a[4], bb[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) a[i] = As[k * 64 + ty + 16 * i];
-#pragma unroll
-        for (int j = 0; j < 4; j++) bb[j] = Bs[k * kBsLd + tx + 16 * j];
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) acc[i][j] = fma(a[i], bb[j], acc[i][j]);
-    }
+    tile_mm(As, Bs, b, acc);
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const int c = c0 + tx + 16 * j;
         if (c >= J.T) continue;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int r = r0 + ty + 16 * i;
-            if (r < J.T) J.S[(size_t)c * ld + r] -= acc[i][j];
+            const int rr = r0 + ty + 16 * i;
+            if (rr < J.T) J.S[(size_t)c * ld + rr] -= acc[i][j];
         }
     }
 }
@@ -414,17 +547,17 @@ hipError_t dense_set_lds()
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGemmLds);
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void *)dense_trsm_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)kTrsmLds);
+                               (int)kGemmLds);
 }
 
 hipError_t launch_dense_lu(const DenseJob *jobs, int njobs, int maxT, const int *active, hipStream_t st)
 {
     for (int p = 0; p < maxT; p += kNb) {
         const int rem = maxT - p - min(kNb, maxT - p);
-        hipLaunchKernelGGL(dense_diag_kernel, dim3(njobs), dim3(64), 0, st, jobs, active, p);
+        hipLaunchKernelGGL(dense_diag_kernel, dim3(njobs), dim3(256), 0, st, jobs, active, p);
         if (rem > 0) {
             const int nt = (rem + kNb - 1) / kNb;
-            hipLaunchKernelGGL(dense_trsm_kernel, dim3(2 * nt, njobs), dim3(64), kTrsmLds, st, jobs, active, p);
+            hipLaunchKernelGGL(dense_trsm_kernel, dim3(2 * nt, njobs), dim3(256), kGemmLds, st, jobs, active, p);
             hipLaunchKernelGGL(dense_gemm_kernel, dim3(nt * nt, njobs), dim3(256), kGemmLds, st, jobs,
                                active, p);
         }

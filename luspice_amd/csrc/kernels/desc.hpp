@@ -57,6 +57,13 @@ struct TriDesc {
     const int *rowat;       // position -> original row (P)
     const int *q;           // position -> original column (Q)
     int *fdone, *bdone;     // per-row epoch flags
+    // supernodes (rows below tail_k0)
+    const T *val;           // column storage of L and U (FactorDesc::val)
+    const int *dp;          // n diagonal slots (absolute)
+    const int *snend;       // n: one past the last row of i's supernode
+    const int *ru;          // n: U(j - ru[j] .. j-1, j) is column j's run in its supernode
+    const int *fcut;        // n: end of row i's L entries left of its supernode
+    const int *bcut;        // n: start of row i's U entries right of its supernode
     // dense trailing block: rows [tail_k0, n) are solved with tail_F
     const T *tail_F;        // tail_T^2, column-major, unit-L below / U on and above
     const int *tail_lcut;   // tail_T: end of the sparse part of L row tail_k0 + r
@@ -71,6 +78,7 @@ struct TriDesc {
 // One dense trailing block to factor (fp64 work, fixed pivot order).
 struct DenseJob {
     double *S;              // T x T, column-major, leading dimension T
+    double *W;              // 2 x 64 x 64 workspace: inverses of the current diagonal block
     unsigned long long *colmax;
     int T;
     int mat;                // matrix index (descriptor, mstat, active mask)

@@ -32,45 +32,6 @@
 namespace lsp {
 
 
-// Wave-uniform broadcast of lane `l`'s double (l uniform).
-__device__ __forceinline__ double bcast_f64(double v, int l)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, l);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
-constexpr int kTri = kWave - 1;         // triangle steps held in registers
-
-// Unit-lower triangular solve of one run held one entry per lane (lanes ta ..
-// tb-1): step q broadcasts x[j_{ta+q}] and subtracts L(j_t, j_{ta+q}) x from
-// every later lane t of the run.  The NT triangle columns are all loaded
-// before the first step (columns past the run load a valid duplicate and are
-// masked), so the chain waits on registers only.
-template <int NT, typename T>
-__device__ __forceinline__ double run_triangle(const T *val, double xc, int lb, int ta, int tb)
-{
-    const int lane = lane_id();
-    const int bw = tb - ta;
-    T tv[NT];
-#pragma unroll
-    for (int q = 0; q < NT; q++) {
-        const int qq = min(q, bw - 2);
-        const int tp = ta + qq;
-        const int base = __builtin_amdgcn_readlane(lb, tp);
-        const int off = min(max(lane - tp - 1, 0), bw - qq - 2);
-        tv[q] = ld_coherent_val(gp(val) + (uint32_t)base, (uint32_t)off);
-    }
-#pragma unroll
-    for (int q = 0; q < NT; q++) {
-        const int tp = min(ta + q, kWave - 1);
-        const double ut = bcast_f64(xc, tp);
-        const double nx = fma(-to_f64(tv[q]), ut, xc);
-        xc = (lane > tp && lane < tb) ? nx : xc;
-    }
-    return xc;
-}
 constexpr int kGemvBatch = 16;          // L values in flight per GEMV row pass
 
 // One column of the left-looking refactorization.  XP is the work vector's

@@ -15,7 +15,11 @@
 //
 // One wave per row: lanes stride over the row's entries (coalesced index and
 // value loads), wait on their own dependency flags in parallel, and the wave
-// sums in fp64 with a DPP reduction.  Solved values are stored agent-coherent
+// sums in fp64 with a DPP reduction.  Rows of a supernode (consecutive
+// columns whose L patterns nest, so their diagonal block is dense) first
+// gather only their entries outside it, in parallel; one task per supernode
+// then solves the dense block in registers, so the solve has one dependency
+// level per supernode instead of one per row.  Solved values are stored agent-coherent
 // and flagged once acknowledged; readers load them agent-coherent, so no L2
 // write-back or invalidate is needed (see common.hpp).
 #include "common.hpp"
@@ -24,6 +28,109 @@
 namespace lsp {
 
 
+constexpr int kSolveBatch = 16;         // dense-part values in flight per pass
+
+// Forward triangle task of supernode D = [s, e): its rows hold partial sums
+// p_i = b_i - sum over L entries left of s (flag -epoch); y_D = L_DD^-1 p in
+// 64-row blocks, each block first subtracting the earlier blocks' rows
+// (L(i, j) = val[dp[j] + i - j]) and then solving its unit-lower triangle in
+// registers.  Returns false when a wait was abandoned.
+template <typename T>
+__device__ bool tri_forward(const TriDesc<T> &M, int s, int e, double *y, int *flags, int epoch, int *abort_word)
+{
+    const int lane = lane_id();
+    const LSP_GLOBAL int *dp = gp(M.dp);
+    const int w = e - s;
+    for (int b0 = 0; b0 < w; b0 += kWave) {
+        const int bw = min(kWave, w - b0);
+        const int row = s + b0 + lane;
+        if (!wave_wait(lane < bw ? &flags[row] : nullptr, -epoch, abort_word)) return false;
+        const int rr = s + b0 + min(lane, bw - 1);
+        double p = lane < bw ? ld_coherent_val(&y[row]) : 0.0;
+        double acc = 0.0;
+        for (int q0 = 0; q0 < b0; q0 += kSolveBatch) {
+            T v[kSolveBatch];
+            double yv[kSolveBatch];
+#pragma unroll
+            for (int h = 0; h < kSolveBatch; h++) {
+                const int jq = s + min(q0 + h, b0 - 1);
+                v[h] = ld_coherent_val(gp(M.val) + (uint32_t)(dp[jq] + (rr - jq)), 0u);
+                yv[h] = ld_coherent_val(&y[jq]);
+            }
+#pragma unroll
+            for (int h = 0; h < kSolveBatch; h++)
+                if (q0 + h < b0) acc = fma(to_f64(v[h]), yv[h], acc);
+        }
+        p -= acc;
+        const int lb = dp[rr] + 1;
+        if (bw == 1) {
+        } else if (bw <= 9) p = run_triangle<8>(M.val, p, lb, 0, bw);
+        else if (bw <= 25) p = run_triangle<24>(M.val, p, lb, 0, bw);
+        else p = run_triangle<kTri>(M.val, p, lb, 0, bw);
+        if (lane < bw) st_coherent(&y[row], p);
+        store_complete();
+        if (lane < bw) st_relaxed(&flags[row], epoch);
+    }
+    return true;
+}
+
+// Backward triangle task of D = [s, e): partial sums q_i = y_i - sum over U
+// entries right of e (in z, flag -epoch); z_D = U_DD^-1 q in 64-row blocks
+// from the bottom, each block first subtracting the later blocks' rows
+// (U(i, j) = val[dp[j] - (j - i)] within column j's run ru[j], else 0).
+template <typename T>
+__device__ bool tri_backward(const TriDesc<T> &M, int s, int e, double *z, double *x, int *flags, int epoch,
+                             int *abort_word)
+{
+    const int lane = lane_id();
+    const LSP_GLOBAL int *dp = gp(M.dp);
+    const LSP_GLOBAL int *ru = gp(M.ru);
+    const int w = e - s;
+    for (int b1 = w; b1 > 0; b1 -= kWave) {
+        const int bw = min(kWave, b1);
+        const int base = s + b1 - bw;
+        const int row = base + lane;
+        if (!wave_wait(lane < bw ? &flags[row] : nullptr, -epoch, abort_word)) return false;
+        const int rr = base + min(lane, bw - 1);
+        double num = lane < bw ? ld_coherent_val(&z[row]) : 0.0;
+        const int cnt = e - (base + bw);                 // rows of later blocks
+        double acc = 0.0;
+        for (int q0 = 0; q0 < cnt; q0 += kSolveBatch) {
+            T v[kSolveBatch];
+            double zv[kSolveBatch];
+            int in[kSolveBatch];
+#pragma unroll
+            for (int h = 0; h < kSolveBatch; h++) {
+                const int jq = base + bw + min(q0 + h, cnt - 1);
+                const int d = jq - rr;                   // >= 1
+                in[h] = d <= ru[jq];
+                v[h] = ld_coherent_val(gp(M.val) + (uint32_t)(dp[jq] - min(d, ru[jq])), 0u);
+                zv[h] = ld_coherent_val(&z[jq]);
+            }
+#pragma unroll
+            for (int h = 0; h < kSolveBatch; h++)
+                if (q0 + h < cnt && in[h]) acc = fma(to_f64(v[h]), zv[h], acc);
+        }
+        num -= acc;
+        const double rd = 1.0 / to_f64(M.diag[rr]);
+        const int dpc = dp[rr];
+        const int run = ru[rr];
+        double zt;
+        if (bw <= 8) zt = run_triangle_upper<8>(M.val, num, rd, dpc, run, bw);
+        else if (bw <= 24) zt = run_triangle_upper<24>(M.val, num, rd, dpc, run, bw);
+        else zt = run_triangle_upper<kWave>(M.val, num, rd, dpc, run, bw);
+        if (lane < bw) {
+            st_coherent(&z[row], zt);
+            x[M.q[row]] = zt;
+        }
+        store_complete();
+        if (lane < bw) st_relaxed(&flags[row], epoch);
+    }
+    return true;
+}
+
+// Task codes (see the plan): 2i = row i solved whole, 2i+1 = row i's entries
+// outside its supernode into a partial sum, -(s+1) = supernode triangle.
 template <typename T, bool kForward>
 __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
 {
@@ -37,18 +144,33 @@ __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
         const int2 tk = L.task[t];
         if (L.active && !L.active[tk.x]) continue;
         const TriDesc<T> M = mats[tk.x];
-        const int i = tk.y;
         const long vo = L.voff[tk.x];
-        const int *rp = kForward ? M.lrp : M.urp;
-        const int *ci = kForward ? M.lci : M.uci;
-        const T *rv = kForward ? M.lrow : M.urow;
         double *y = L.y + vo;
         double *z = L.z + vo;
-        const double *src = kForward ? y : z;
         int *flags = kForward ? M.fdone : M.bdone;
+        const int code = tk.y;
+        if (code < 0) {
+            const int s0 = -code - 1;
+            const int e0 = M.snend[s0];
+            if (kForward) tri_forward<T>(M, s0, e0, y, flags, epoch, L.abort_word);
+            else tri_backward<T>(M, s0, e0, z, L.x + vo, flags, epoch, L.abort_word);
+            continue;
+        }
+        const int i = code >> 1;
+        const bool part = code & 1;
+        const int *ci = kForward ? M.lci : M.uci;
+        const T *rv = kForward ? M.lrow : M.urow;
+        const double *src = kForward ? y : z;
+        int e0, e1;
+        if (kForward) {
+            e0 = M.lrp[i];
+            e1 = part ? M.fcut[i] : M.lrp[i + 1];
+        } else {
+            e0 = part ? M.bcut[i] : M.urp[i];
+            e1 = M.urp[i + 1];
+        }
         double acc = 0.0;
         bool ok = true;
-        const int e0 = rp[i], e1 = rp[i + 1];
         for (int b0 = e0; b0 < e1 && ok; b0 += kWave) {
             const int e = b0 + lane;
             const int j = e < e1 ? ci[e] : 0;
@@ -61,13 +183,15 @@ __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
         if (lane == 0) {
             if (kForward) {
                 st_coherent(&y[i], L.b[vo + M.rowat[i]] - acc);
+            } else if (part) {
+                st_coherent(&z[i], y[i] - acc);
             } else {
                 const double zi = (y[i] - acc) / to_f64(M.diag[i]);
                 st_coherent(&z[i], zi);
                 L.x[vo + M.q[i]] = zi;
             }
             store_complete();
-            st_relaxed(&flags[i], epoch);
+            st_relaxed(&flags[i], part ? -epoch : epoch);
         }
     }
     if (lane == 0) {

@@ -259,6 +259,12 @@ int Batch::ensure_factor_set(FactorSet<T> &fs)
         D.bdone = d_bdone_ + C.o_n;
         D.n = C.n;
         D.tail_F = fs.tailF + C.o_t2;
+        D.val = F.val;
+        D.dp = F.dp;
+        D.snend = d_snend_ + C.o_n;
+        D.ru = d_ru_ + C.o_n;
+        D.fcut = d_fcut_ + C.o_n;
+        D.bcut = d_bcut_ + C.o_n;
         D.tail_lcut = d_tail_lcut_ + C.o_t;
         D.tail_flag = d_tail_flag_ + C.o_tb;
         D.tail_ctr = d_tail_ctr_ + 4 * m;
@@ -323,10 +329,10 @@ size_t Batch::estimate_device_bytes() const
     const double m = std::max(1, opt_.ir_inner);
     double b = 0;
     b += 16 * (n + nc) + 4 * n + 8 * slots + 4 * (n + nc) + 8 * nnz;     // cp relp dp pos dst aptr asrc aslot
-    b += 8 * (n + nc) + 4 * nlu + 12 * n + 2 * rel;                       // lrp urp lci uci rowat q snend rel
+    b += 8 * (n + nc) + 4 * nlu + 24 * n + 2 * rel;                       // lrp urp lci uci rowat q snend ru fcut bcut rel
     b += 8 * (slices + nc) + 8 * n + 8 * sell;                            // SELL structure
     b += 12 * n + 8 * nnz + 12 * sell;                                    // flags, values
-    b += 8 * T2 + 16 * T;                                                  // dense block work
+    b += 8 * T2 + 16 * T + 65536.0 * nc;                                   // dense block work
     b += 24 * n + 8 * slices;                                             // task lists
     b += 64 * n + 8 * (m + 1) * n;                                        // vectors, Krylov basis
     b += (4 + 8) * (slots + nlu + n + T2);                                 // fp32 + fp64 factor sets
@@ -377,7 +383,7 @@ int Batch::upload_all()
     nt_tot_ = o_t; nt2_tot_ = o_t2; ntb_tot_ = o_tb;
     {
         std::vector<int64_t> cp, relp, soff;
-        std::vector<int> dp, pos, aptr, asrc, aslot, dst, lrp, lci, urp, uci, rowat, q, snend;
+        std::vector<int> dp, pos, aptr, asrc, aslot, dst, lrp, lci, urp, uci, rowat, q, snend, ru, fcut, bcut;
         std::vector<int> srlen, scol, sperm, ssrc;
         std::vector<uint16_t> rel;
         cp.reserve(o_np1); relp.reserve(o_np1); rel.reserve(o_rel);
@@ -398,6 +404,9 @@ int Batch::upload_all()
             rowat.insert(rowat.end(), P.rowat.begin(), P.rowat.end());
             q.insert(q.end(), P.q.begin(), P.q.end());
             snend.insert(snend.end(), P.snend.begin(), P.snend.end());
+            ru.insert(ru.end(), P.ru.begin(), P.ru.end());
+            fcut.insert(fcut.end(), P.fcut.begin(), P.fcut.end());
+            bcut.insert(bcut.end(), P.bcut.begin(), P.bcut.end());
             rel.insert(rel.end(), P.rel.begin(), P.rel.end());
             const LspSell &S = C.sell;
             soff.insert(soff.end(), S.off.begin(), S.off.end());
@@ -419,11 +428,12 @@ int Batch::upload_all()
         d_asrc_ = upload(asrc); d_aslot_ = upload(aslot); d_dst_ = upload(dst);
         d_lrp_ = upload(lrp); d_lci_ = upload(lci); d_urp_ = upload(urp); d_uci_ = upload(uci);
         d_rowat_ = upload(rowat); d_q_ = upload(q); d_rel_ = upload(rel); d_snend_ = upload(snend);
+        d_ru_ = upload(ru); d_fcut_ = upload(fcut); d_bcut_ = upload(bcut);
         d_srlen_ = upload(srlen); d_scol_ = upload(scol); d_sperm_ = upload(sperm);
         d_ssrc_ = upload(ssrc);
         if (!d_cp_ || !d_relp_ || !d_soff_ || !d_dp_ || !d_pos_ || !d_aptr_ || !d_asrc_ ||
             !d_aslot_ || !d_dst_ || !d_lrp_ || !d_lci_ || !d_urp_ || !d_uci_ || !d_rowat_ ||
-            !d_q_ || !d_rel_ || !d_snend_ || !d_srlen_ || !d_scol_ || !d_sperm_ || !d_ssrc_)
+            !d_q_ || !d_rel_ || !d_snend_ || !d_ru_ || !d_fcut_ || !d_bcut_ || !d_srlen_ || !d_scol_ || !d_sperm_ || !d_ssrc_)
             return LSP_E_ALLOC;
     }
     // flags start at 0; epochs start at 1
@@ -446,7 +456,10 @@ int Batch::upload_all()
         std::vector<DenseJob> jobs;
         d_tailS_ = (double *)dalloc(nt2_tot_ * sizeof(double));
         d_colmax_ = (unsigned long long *)dalloc(nt_tot_ * sizeof(unsigned long long));
-        if (!d_tailS_ || !d_colmax_) return LSP_E_ALLOC;
+        int ntail = 0;
+        for (const auto &C : circ_) ntail += C.plan.tail_T > 0;
+        d_tailW_ = (double *)dalloc((size_t)std::max(ntail, 1) * 2 * 64 * 64 * sizeof(double));
+        if (!d_tailS_ || !d_colmax_ || !d_tailW_) return LSP_E_ALLOC;
         for (int m = 0; m < nc; m++) {
             const CircuitHost &C = circ_[m];
             const LspPlan &P = C.plan;
@@ -455,6 +468,7 @@ int Batch::upload_all()
             if (P.tail_T > 0) {
                 DenseJob J{};
                 J.S = d_tailS_ + C.o_t2;
+                J.W = d_tailW_ + jobs.size() * 2 * 64 * 64;
                 J.colmax = d_colmax_ + C.o_t;
                 J.T = P.tail_T;
                 J.mat = m;
@@ -517,13 +531,13 @@ int Batch::upload_all()
     }
     // merged task lists: (level, circuit, index)
     {
-        auto merge = [&](auto level_of, auto order_of, long &width) {
+        auto merge = [&](auto key_of, auto order_of, long &width) {
             std::vector<std::pair<long, int2>> all;
             all.reserve(ntot_);
             for (int m = 0; m < nc; m++) {
                 const LspPlan &P = circ_[m].plan;
                 const std::vector<int> &ord = order_of(P);
-                for (int k : ord) all.push_back({(long)level_of(P, k), make_int2(m, k)});
+                for (size_t t = 0; t < ord.size(); t++) all.push_back({(long)key_of(P, t), make_int2(m, ord[t])});
             }
             std::stable_sort(all.begin(), all.end(),
                              [](const auto &a, const auto &b) { return a.first < b.first; });
@@ -536,13 +550,13 @@ int Batch::upload_all()
             }
             return out;
         };
-        auto tf = merge([](const LspPlan &P, int k) { return P.fkey[k]; },
+        auto tf = merge([](const LspPlan &P, size_t t) { return P.fkey[P.ftask[t]]; },
                         [](const LspPlan &P) -> const std::vector<int> & { return P.ftask; },
                         width_f_);
-        auto tw = merge([](const LspPlan &P, int k) { return P.lvf[k]; },
+        auto tw = merge([](const LspPlan &P, size_t t) { return P.fwd_key[t]; },
                         [](const LspPlan &P) -> const std::vector<int> & { return P.fwd_task; },
                         width_tf_);
-        auto tb = merge([](const LspPlan &P, int k) { return P.lvb[k]; },
+        auto tb = merge([](const LspPlan &P, size_t t) { return P.bwd_key[t]; },
                         [](const LspPlan &P) -> const std::vector<int> & { return P.bwd_task; },
                         width_tb_);
         nfw_tot_ = (long)tw.size();

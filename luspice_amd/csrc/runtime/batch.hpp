@@ -118,7 +118,8 @@ private:
         *d_done_ = nullptr, *d_fdone_ = nullptr, *d_bdone_ = nullptr, *d_srlen_ = nullptr,
         *d_scol_ = nullptr, *d_sperm_ = nullptr, *d_ssrc_ = nullptr;
     uint16_t *d_rel_ = nullptr;
-    int *d_snend_ = nullptr;
+    int *d_snend_ = nullptr, *d_ru_ = nullptr, *d_fcut_ = nullptr, *d_bcut_ = nullptr;
+    double *d_tailW_ = nullptr;
     double *d_ain_ = nullptr, *d_sval64_ = nullptr;
     float *d_sval32_ = nullptr;
     SellDesc<double> *d_sdesc64_ = nullptr;
