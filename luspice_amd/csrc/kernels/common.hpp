@@ -209,6 +209,36 @@ __device__ __forceinline__ bool wave_wait(const int *flag, int epoch, int *abort
     }
 }
 
+// wave_wait over up to N flags per lane (null = nothing to wait for): each
+// round issues the outstanding polls together, so N chunks of a long
+// dependency list cost one round trip instead of N.
+template <int N>
+__device__ __forceinline__ bool wave_wait_n(const int *const (&flag)[N], int epoch, int *abort_word)
+{
+    unsigned ok = 0;
+#pragma unroll
+    for (int g = 0; g < N; g++)
+        if (flag[g] == nullptr) ok |= 1u << g;
+    const unsigned full = (1u << N) - 1;
+    int backoff = 0;
+    for (int round = 0;; round++) {
+        int seen[N];
+#pragma unroll
+        for (int g = 0; g < N; g++) seen[g] = (ok >> g) & 1 ? epoch : ld_relaxed(flag[g]);
+#pragma unroll
+        for (int g = 0; g < N; g++)
+            if (seen[g] == epoch) ok |= 1u << g;
+        if (__all(ok == full)) return true;
+        if ((round & 15) == 15 && aborted(abort_word)) return false;
+        if (round > kRoundLimit) {
+            if (ok != full) atomicCAS(abort_word, 0, kErrSpin);
+            return false;
+        }
+        for (int s = 0; s <= backoff; s++) __builtin_amdgcn_s_sleep(4);
+        if (backoff < 7) backoff = 2 * backoff + 1;
+    }
+}
+
 // Makes data published before the observed flags visible to this wave.
 __device__ __forceinline__ void acquire_fence()
 {

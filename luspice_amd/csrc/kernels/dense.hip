@@ -272,16 +272,29 @@ __global__ void __launch_bounds__(256) dense_trsm_kernel(const DenseJob *jobs, c
     const bool lower = t < ntile;
     const int r0 = p + b + t * kNb;                    // L21 tile rows
     const int c0 = p + b + (t - ntile) * kNb;          // U12 tile columns
-    for (int e = tid; e < kNb * 64; e += 256) {
-        const int k = e >> 6, i = e & 63;
-        if (lower) {
-            const int rr = r0 + i;
-            As[k * 64 + i] = (rr < J.T && k < b) ? J.S[(size_t)(p + k) * ld + rr] : 0.0;
-            Bs[k * kBsLd + i] = W[kNb * kNb + i * kNb + k];             // U^-1(k, i)
-        } else {
-            As[k * 64 + i] = W[k * kNb + i];                             // L^-1(i, k)
-            const int cc = c0 + k;                                        // column k of the tile, row i
-            Bs[i * kBsLd + k] = (cc < J.T && i < b) ? J.S[(size_t)cc * ld + p + i] : 0.0;
+    {
+        double va[16], vb[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int e = tid + 256 * u;
+            const int k = e >> 6, i = e & 63;
+            if (lower) {
+                const int rr = r0 + i;
+                va[u] = (rr < J.T && k < b) ? J.S[(size_t)(p + k) * ld + rr] : 0.0;
+                vb[u] = W[kNb * kNb + i * kNb + k];                     // U^-1(k, i)
+            } else {
+                va[u] = W[k * kNb + i];                                 // L^-1(i, k)
+                const int cc = c0 + k;                                  // column k of the tile, row i
+                vb[u] = (cc < J.T && i < b) ? J.S[(size_t)cc * ld + p + i] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int e = tid + 256 * u;
+            const int k = e >> 6, i = e & 63;
+            As[k * 64 + i] = va[u];
+            if (lower) Bs[k * kBsLd + i] = vb[u];
+            else Bs[i * kBsLd + k] = vb[u];
         }
     }
     __syncthreads();
@@ -323,7 +336,15 @@ __global__ void __launch_bounds__(256) dense_trsm_kernel(const DenseJob *jobs, c
 }
 
 // ---------------------------------------------------------------------------
-// S22 -= L21 U12 on 64x64 tiles (same thread layout as tile_mm).
+// S22 -= L21 U12 on 64x64 tiles with fp64 matrix cores.  Wave w computes the
+// 32x32 quadrant (rows 32 (w & 1), columns 32 (w >> 1)) as 2x2 products of
+// v_mfma_f64_16x16x4f64, whose operands and result sit in the lanes as
+//   A(i, k): lane i + 16k      B(k, j): lane j + 16k
+//   D(i, j): lane j + 16 (i % 4), register i / 4.
+// K (the panel width) is zero-padded to a multiple of 4 in LDS.  The result
+// goes through LDS so that the read-modify-write of S runs down columns.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 __global__ void __launch_bounds__(256) dense_gemm_kernel(const DenseJob *jobs, const int *active, int p)
 {
     const DenseJob J = jobs[blockIdx.y];
@@ -342,33 +363,76 @@ __global__ void __launch_bounds__(256) dense_gemm_kernel(const DenseJob *jobs, c
     double *Bs = lds + kNb * 64;                       // Bs[k*65 + j] = U12(p+k, c0+j)
     const int tid = threadIdx.x;
     const size_t ld = (size_t)J.T;
-    for (int e = tid; e < b * 64; e += 256) {
-        const int k = e >> 6, i = e & 63;
-        const int rr = r0 + i;
-        As[k * 64 + i] = rr < J.T ? J.S[(size_t)(p + k) * ld + rr] : 0.0;
-    }
-    for (int e = tid; e < b * 64; e += 256) {
-        const int j = e / b, k = e - j * b;
-        const int c = c0 + j;
-        Bs[k * kBsLd + j] = c < J.T ? J.S[(size_t)c * ld + p + k] : 0.0;
+    const int b4 = (b + 3) & ~3;
+    // 16 elements per thread of each operand; all loads are issued before
+    // the first LDS store so they are in flight together
+    {
+        double va[16], vb[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int e = tid + 256 * u;
+            const int k = e >> 6, i = e & 63;            // A: column k, row i
+            const int rr = r0 + i;
+            va[u] = (rr < J.T && k < b) ? J.S[(size_t)(p + k) * ld + rr] : 0.0;
+            const int c = c0 + k;                         // B: column k of the tile, row i of the panel
+            vb[u] = (c < J.T && i < b) ? J.S[(size_t)c * ld + p + i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int e = tid + 256 * u;
+            const int k = e >> 6, i = e & 63;
+            As[k * 64 + i] = va[u];
+            Bs[i * kBsLd + k] = vb[u];
+        }
     }
     __syncthreads();
-    const int ty = tid & 15, tx = tid >> 4;
-    double acc[4][4];
+    const int lane = tid & 63, w = tid >> 6;
+    const int rb = (w & 1) * 32, cb = (w >> 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    v4d acc[2][2];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int x = 0; x < 2; x++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
-    tile_mm(As, Bs, b, acc);
+        for (int y = 0; y < 2; y++) acc[x][y] = (v4d){0.0, 0.0, 0.0, 0.0};
+    for (int k4 = 0; k4 < b4; k4 += 4) {
+        const int k = k4 + lk;
+        double av[2], bv[2];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int c = c0 + tx + 16 * j;
-        if (c >= J.T) continue;
+        for (int x = 0; x < 2; x++) av[x] = As[k * 64 + rb + 16 * x + li];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int rr = r0 + ty + 16 * i;
-            if (rr < J.T) J.S[(size_t)c * ld + rr] -= acc[i][j];
-        }
+        for (int y = 0; y < 2; y++) bv[y] = Bs[k * kBsLd + cb + 16 * y + li];
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 2; y++) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], acc[x][y], 0, 0, 0);
+    }
+    __syncthreads();                                   // As/Bs reads done: reuse LDS for the product
+    double *Cs = lds;                                  // Cs[j*65 + i]
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = rb + 16 * x + 4 * r + lk;
+                const int j = cb + 16 * y + li;
+                Cs[j * kBsLd + i] = acc[x][y][r];
+            }
+    __syncthreads();
+    double cv[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int e = tid + 256 * u;
+        const int j = e >> 6, i = e & 63;
+        const int rr = r0 + i, c = c0 + j;
+        cv[u] = (rr < J.T && c < J.T) ? J.S[(size_t)c * ld + rr] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int e = tid + 256 * u;
+        const int j = e >> 6, i = e & 63;
+        const int rr = r0 + i, c = c0 + j;
+        if (rr < J.T && c < J.T) J.S[(size_t)c * ld + rr] = cv[u] - Cs[j * kBsLd + i];
     }
 }
 

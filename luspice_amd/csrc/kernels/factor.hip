@@ -34,6 +34,54 @@ namespace lsp {
 
 constexpr int kGemvBatch = 16;          // L values in flight per GEMV row pass
 
+// Rows below a run's part in a chunk: row i gets sum_t' L(i, j_t') x[j_t']
+// over the run's columns t' in [ta, tb), with L(i, j_t') at column offset
+// (tb - t' - 1) + i.  Rows i < tri are the run's own later rows (slots
+// s0 + tb + i); the rest are L(:, jl) through the map.  G row chunks of 64
+// are processed per pass with B columns' values in flight, so a long run
+// keeps G * B loads outstanding; every row is updated once.
+template <int G, int B, typename T, typename XP>
+__device__ __forceinline__ void run_gemv(const T *val, XP x, const LSP_GLOBAL uint16_t *map, int64_t u, double xc,
+                                         int lb, int ta, int tb, int tri, int R, int s0)
+{
+    const int lane = lane_id();
+    const int bw = tb - ta;
+    for (int i0 = 0; i0 < R; i0 += G * kWave) {
+        int i[G];
+        double acc[G];
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            i[g] = min(i0 + g * kWave + lane, R - 1);
+            acc[g] = 0.0;
+        }
+        for (int q0 = 0; q0 < bw; q0 += B) {
+            T v[B][G];
+#pragma unroll
+            for (int h = 0; h < B; h++) {
+                const int tp = ta + min(q0 + h, bw - 1);
+                const int base = __builtin_amdgcn_readlane(lb, tp) + (tb - tp - 1);
+#pragma unroll
+                for (int g = 0; g < G; g++) v[h][g] = ld_coherent_val(gp(val) + (uint32_t)base, (uint32_t)i[g]);
+            }
+#pragma unroll
+            for (int h = 0; h < B; h++) {
+                if (q0 + h < bw) {
+                    const double w = bcast_f64(xc, ta + q0 + h);
+#pragma unroll
+                    for (int g = 0; g < G; g++) acc[g] = fma(to_f64(v[h][g]), w, acc[g]);
+                }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            if (i0 + g * kWave + lane < R) {
+                const int slot = i[g] < tri ? s0 + tb + i[g] : (int)map[u + (i[g] - tri)];
+                x[slot] -= acc[g];
+            }
+        }
+    }
+}
+
 // One column of the left-looking refactorization.  XP is the work vector's
 // pointer type: an LSP_LDS pointer when the column fits the wave's LDS slice,
 // an LSP_GLOBAL scratch pointer otherwise.
@@ -118,26 +166,10 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
             // rows below the run's part in this chunk: the run's later rows,
             // then the rows of L(:, jl) through the map
             const int R = tri + nlast;
-            for (int i0 = 0; i0 < R; i0 += kWave) {
-                const int i = min(i0 + lane, R - 1);
-                double acc = 0.0;
-                for (int q0 = 0; q0 < bw; q0 += kGemvBatch) {
-                    T v[kGemvBatch];
-#pragma unroll
-                    for (int h = 0; h < kGemvBatch; h++) {
-                        const int tp = ta + min(q0 + h, bw - 1);
-                        const int base = __builtin_amdgcn_readlane(lb, tp) + (tb - tp - 1);
-                        v[h] = ld_coherent_val(gp(M.val) + (uint32_t)base, (uint32_t)i);
-                    }
-#pragma unroll
-                    for (int h = 0; h < kGemvBatch; h++)
-                        if (q0 + h < bw) acc = fma(to_f64(v[h]), bcast_f64(xc, ta + q0 + h), acc);
-                }
-                if (i0 + lane < R) {
-                    const int slot = i < tri ? s0 + tb + i : (int)map[u + (i - tri)];
-                    x[slot] -= acc;
-                }
-            }
+            if (R > 2 * kWave && bw >= 8)
+                run_gemv<4, 8>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
+            else
+                run_gemv<1, kGemvBatch>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
             if (tri == 0) u += nlast;
             wave_fence();
             if (lane >= tb && lane < nd) xc = x[s0 + lane];   // later runs of this chunk

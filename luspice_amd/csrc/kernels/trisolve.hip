@@ -29,6 +29,7 @@ namespace lsp {
 
 
 constexpr int kSolveBatch = 16;         // dense-part values in flight per pass
+constexpr int kGather = 4;              // 64-entry chunks of a row per wait round
 
 // Forward triangle task of supernode D = [s, e): its rows hold partial sums
 // p_i = b_i - sum over L entries left of s (flag -epoch); y_D = L_DD^-1 p in
@@ -171,12 +172,24 @@ __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
         }
         double acc = 0.0;
         bool ok = true;
-        for (int b0 = e0; b0 < e1 && ok; b0 += kWave) {
-            const int e = b0 + lane;
-            const int j = e < e1 ? ci[e] : 0;
-            ok = wave_wait(e < e1 ? &flags[j] : nullptr, epoch, L.abort_word);
+        for (int b0 = e0; b0 < e1 && ok; b0 += kGather * kWave) {
+            int jj[kGather];
+            const int *fl[kGather];
+            T vv[kGather];
+#pragma unroll
+            for (int g = 0; g < kGather; g++) {
+                const int e = b0 + g * kWave + lane;
+                jj[g] = e < e1 ? ci[e] : 0;
+                vv[g] = e < e1 ? rv[e] : T(0);
+                fl[g] = e < e1 ? &flags[jj[g]] : nullptr;
+            }
+            ok = wave_wait_n<kGather>(fl, epoch, L.abort_word);
             if (!ok) break;
-            if (e < e1) acc += to_f64(rv[e]) * ld_coherent_val(&src[j]);
+            double sv[kGather];
+#pragma unroll
+            for (int g = 0; g < kGather; g++) sv[g] = fl[g] ? ld_coherent_val(&src[jj[g]]) : 0.0;
+#pragma unroll
+            for (int g = 0; g < kGather; g++) acc += to_f64(vv[g]) * sv[g];
         }
         if (!ok) continue;
         acc = wave_sum(acc);
