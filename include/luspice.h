@@ -62,6 +62,8 @@ typedef struct {
     int trisolve_blocks;   /* persistent trisolve grid (0 = auto) */
     int tail_max;          /* largest dense trailing block (0 = 8192, <0 = none) */
     double tail_density;   /* min pattern density of that block (0 = 0.3) */
+    int ruiz_sweeps;       /* Ruiz sweeps ahead of FP16 factor storage (0 = 30) */
+    int pivot_scaled;      /* 1: choose the pivot sequence on the Ruiz-scaled matrix */
 } lsp_options;
 
 typedef struct {
@@ -98,6 +100,7 @@ typedef struct {
     long refactor_count;
     double fwd_ms, bwd_ms; /* summed device time of forward / backward solves */
     long tri_count;        /* trisolve pairs timed */
+    long graph_steps;      /* Newton steps replayed as a HIP graph */
 } lsp_kernel_stats;
 
 typedef struct lsp_batch lsp_batch;
@@ -152,6 +155,14 @@ int  lsp_batch_export_factors(lsp_batch *b, int circuit, int *pinv,
                               int *Up, int *Ui, double *Ux);
 /* Device pointers of the vector arenas (length total_n) for zero-copy use. */
 int  lsp_batch_device_vectors(lsp_batch *b, double **rhs, double **x);
+/* One Newton step on the resident arenas as a replayed HIP graph: refactor
+ * in `prec`, one GMRES cycle of up to ir_inner iterations, true residual.
+ * Recorded on first use (and when prec changes); no host synchronisation
+ * inside.  Circuits not converged after the cycle are finished by the eager
+ * solve.  No counterpart in the reference (its graph-free design:
+ * spmv.h:5-7); this is the capturable refactor -> trisolve -> IR sequence. */
+int  lsp_batch_graph_step(lsp_batch *b, int prec, lsp_refactor_report *rrep,
+                          lsp_solve_report *srep);
 /* Copies total_n doubles into the resident right-hand-side arena / out of
  * the resident solution arena, ordered on the batch's stream (src / dst may
  * be host or device memory; the call returns when the copy is complete). */
@@ -178,7 +189,9 @@ int  lsp_symbolic_factor(int n, long nnz, const int *row, const int *col,
                          long *lnnz, long *unnz);
 /* Synthetic MNA circuits.  kind 0: 25x40 grid + 20 sources (n = 1020);
  * kind 1: 316x316 power grid + 50 pads (n = 99906); kind 2: circuit "like
- * kind 0" with n unknowns and long-range fraction lr.  Two calls: first
+ * kind 0" with n unknowns and long-range fraction lr; kind 3: precision-sweep
+ * circuit with n unknowns, ~6 nonzeros per row and conductances spread over
+ * up to 15 decades (lr ignored).  Two calls: first
  * with row == NULL to get n and nnz, then with arrays. */
 int  lsp_mna_generate(int kind, int n_req, double lr, uint64_t seed, int *n,
                       long *nnz, int *row, int *col, double *val, double *rhs);

@@ -57,8 +57,10 @@ class Circuit:
 
 def mna(kind: int | str, n: int = 0, lr: float = 0.02, seed: int = 0) -> Circuit:
     """Synthetic MNA circuit.  kind 0 / 'c0': 25x40 grid + 20 sources (n=1020);
-    1 / 'c1': 316x316 power grid + 50 pads (n=99906); 2 / 'sized': n unknowns."""
-    k = {"c0": 0, "c1": 1, "sized": 2}.get(kind, kind)
+    1 / 'c1': 316x316 power grid + 50 pads (n=99906); 2 / 'sized': n unknowns;
+    3 / 'c2': n unknowns, ~6 nonzeros per row, conductances spread over up to
+    15 decades (precision sweep)."""
+    k = {"c0": 0, "c1": 1, "sized": 2, "c2": 3}.get(kind, kind)
     nn = N.C.c_int()
     nz = N.C.c_long()
     check(N.lib.lsp_mna_generate(k, int(n), float(lr), int(seed), N.C.byref(nn), N.C.byref(nz),
@@ -274,6 +276,20 @@ class Batch:
         out = {k: getattr(rep, k) for k, _ in N.SolveReport._fields_}
         out["rc"] = rc
         return x, out
+
+    def graph_step(self, prec="fp32"):
+        """One Newton step on the resident b / x as a replayed HIP graph
+        (refactor + GMRES cycle + residual); returns (refactor, solve) reports."""
+        rr = N.RefactorReport()
+        sr = N.SolveReport()
+        rc = N.lib.lsp_batch_graph_step(self._h, prec_code(prec), N.C.byref(rr), N.C.byref(sr))
+        ro = {k: getattr(rr, k) for k, _ in N.RefactorReport._fields_}
+        so = {k: getattr(sr, k) for k, _ in N.SolveReport._fields_}
+        ro["rc"] = rc
+        so["rc"] = rc
+        if rc != 0:
+            so["error"] = N.lib.lsp_batch_last_error(self._h).decode()
+        return ro, so
 
     def solve_device(self):
         """GMRES-IR on the resident RHS arena (see device_vectors)."""

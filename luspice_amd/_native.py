@@ -39,7 +39,8 @@ class Options(C.Structure):
                 ("lds_cap", C.c_int), ("blocks_per_cu", C.c_int), ("chunk_rows", C.c_int),
                 ("sell_sigma", C.c_int), ("fallback_fp64", C.c_int),
                 ("refactor_blocks", C.c_int), ("trisolve_blocks", C.c_int),
-                ("tail_max", C.c_int), ("tail_density", C.c_double)]
+                ("tail_max", C.c_int), ("tail_density", C.c_double), ("ruiz_sweeps", C.c_int),
+                ("pivot_scaled", C.c_int)]
 
 
 class RefactorReport(C.Structure):
@@ -62,7 +63,7 @@ class CircuitInfo(C.Structure):
 
 class KernelStats(C.Structure):
     _fields_ = [("refactor_ms", C.c_double), ("refactor_count", C.c_long),
-                ("fwd_ms", C.c_double), ("bwd_ms", C.c_double), ("tri_count", C.c_long)]
+                ("fwd_ms", C.c_double), ("bwd_ms", C.c_double), ("tri_count", C.c_long), ("graph_steps", C.c_long)]
 
 
 def _sig(name, res, *args):
@@ -93,6 +94,7 @@ _sig("lsp_batch_solve", C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
 _sig("lsp_batch_spmv", C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int)
 _sig("lsp_batch_export_factors", C.c_int, C.c_void_p, C.c_int, c_int_p, c_int_p, c_int_p,
      c_double_p, c_int_p, c_int_p, c_double_p)
+_sig("lsp_batch_graph_step", C.c_int, C.c_void_p, C.c_int, C.POINTER(RefactorReport), C.POINTER(SolveReport))
 _sig("lsp_batch_device_vectors", C.c_int, C.c_void_p, C.POINTER(C.c_void_p),
      C.POINTER(C.c_void_p))
 _sig("lsp_batch_set_rhs", C.c_int, C.c_void_p, C.c_void_p)

@@ -8,7 +8,8 @@ and draws b ~ U[-1, 1].  One unit = one circuit refactored and solved to
 rel err <= 1e-6.
 
   batched: all circuits in one resident batch, fp32 factors, one persistent
-           refactor launch + one batched fp64 GMRES-IR per step, inputs in HBM
+           refactor launch + one batched fp64 GMRES-IR per step, inputs in HBM,
+           the whole step replayed as one HIP graph (--no-graph: eager)
   plain:   one batch per circuit, fp64 factors, host inputs each step (timed
            over --plain-steps steps)
 
@@ -36,6 +37,7 @@ def main(argv=None):
     ap.add_argument("--nmax", type=int, default=5000)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--check", type=int, default=-1, help="circuits checked vs the CPU LU (-1 = all)")
+    ap.add_argument("--no-graph", action="store_true", help="eager launches instead of the HIP graph")
     args = ap.parse_args(argv)
 
     from . import device_sync
@@ -54,14 +56,14 @@ def main(argv=None):
     tails = [NB.B.info(i)["tail_T"] for i in range(len(circ))]
     log(f"[newton] batch built {time.time() - t0:.1f}s, dense tails {sum(t > 0 for t in tails)}, max T {max(tails)}")
     for s in range(args.warmup):
-        NB.run_step(s)
+        NB.run_step(s, graph=not args.no_graph)
     device_sync()
     lat = []
     reps = []
     t1 = time.perf_counter()
     for s in range(args.warmup, args.warmup + args.steps):
         ts = time.perf_counter()
-        rr, sr = NB.run_step(s)
+        rr, sr = NB.run_step(s, graph=not args.no_graph)
         lat.append((time.perf_counter() - ts) * 1e3)
         reps.append((rr, sr))
     device_sync()
@@ -117,6 +119,7 @@ def main(argv=None):
         "pass_1e-6": bool(errs.max() <= 1e-6 and all(r[1]["rc"] == 0 for r in reps)),
         "ir_inner_median": int(np.median(inner)), "fallback_steps": int(fallback),
         "dense_tails": int(sum(t > 0 for t in tails)), "max_tail_T": int(max(tails)),
+        "hip_graph": not args.no_graph,
     }
     print(json.dumps(line), flush=True)
 

@@ -127,6 +127,11 @@ int lsp_plan_set_tail(LspPlan &P, int min_T, int max_T, double min_density);
 // target pattern (both mean the symbolic pattern is not closed under fill).
 int lsp_plan_build_segments(LspPlan &P);
 
+// Ruiz equilibration scales: after `sweeps` sweeps, R A C has every row and
+// column maximum close to 1 (used to pivot on the scaled matrix).
+void lsp_ruiz_scales(int n, long nnz, const int *row, const int *col, const double *val, int sweeps,
+                     double *R, double *C);
+
 // Build the plan from A (q-order CSC, values only used for nothing here) and
 // the symbolic factors.  src_index maps each q-CSC entry to the caller's value
 // order (nullptr = identity).  Returns 0 ok, <0 on an inconsistent structure.

@@ -7,6 +7,8 @@
 //   grid of rows x cols nodes (the last row may be partial), node v = r*cols+c
 //   each grid edge      conductance g ~ logU[g_lo, g_hi], stamped +g on both
 //                       diagonals and -g on both off-diagonals
+//   diagonal edges      (r,c)-(r+1,c+1) with probability diag_frac, drawn
+//                       right after the grid edges of node v (none when 0)
 //   long-range branches round(lr_frac * n_grid_edges) distinct non-adjacent
 //                       node pairs, same conductance law
 //   node capacitors     C ~ logU[c_lo, c_hi] stamped as C/h on the diagonal
@@ -101,6 +103,11 @@ int lsp_mna_build(const LspMnaSpec *sp, LspCoo &out)
             adj.insert(key(v, v + sp->cols));
             n_edges++;
         }
+        if (sp->diag_frac > 0 && c + 1 < sp->cols && v + sp->cols + 1 < N && rng.u01() < sp->diag_frac) {
+            stamp_g(v, v + sp->cols + 1, rng.loguniform(sp->g_lo, sp->g_hi));
+            adj.insert(key(v, v + sp->cols + 1));
+            n_edges++;
+        }
         (void)r;
     }
     int n_lr = (int)std::lround(sp->lr_frac * n_edges);
@@ -189,3 +196,22 @@ void lsp_spec_sized(LspMnaSpec *sp, int n, double lr_frac, uint64_t seed)
     sp->c_lo = 1e-12; sp->c_hi = 1e-9; sp->h = 1e-9;
     sp->n_src = n - N; sp->seed = seed;
 }
+
+// Config 2 (precision sweep): like config 0 with n unknowns and ~6 nonzeros
+// per row (2% long-range branches as in config 0, plus diagonal grid edges
+// with probability 0.55, which keeps the fill of a 2-D mesh); conductances
+// logU[g_lo, 1e3] with
+// g_lo = 10^-(6 + 4u), u ~ U[0,1] drawn from the seed (about 10-35% of the entries
+// below 6e-5), and node capacitors logU[1e-18, 1e-15] F at h = 1e-9, small
+// enough not to make the diagonal dominant.
+void lsp_spec_c2(LspMnaSpec *sp, int n, uint64_t seed)
+{
+    lsp_spec_sized(sp, n, 0.02, seed);
+    sp->diag_frac = 0.55;
+    Rng rng(seed ^ 0xC2C2C2C2ULL);
+    sp->g_lo = std::pow(10.0, -(6.0 + 4.0 * rng.u01()));
+    sp->g_hi = 1e3;
+    sp->c_lo = 1e-18;
+    sp->c_hi = 1e-15;
+}
+

@@ -496,3 +496,30 @@ int lsp_plan_set_tail(LspPlan &P, int min_T, int max_T, double min_density)
     const int rc = lsp_plan_build_segments(P);
     return rc != 0 ? rc : best;
 }
+
+// ---------------------------------------------------------------------------
+// Ruiz equilibration scales (D. Ruiz, "A scaling algorithm to equilibrate both
+// rows and columns norms in matrices", RAL-TR-2001-034): each sweep divides
+// every row by the square root of its largest |entry| of R A C, then every
+// column likewise.  Reference behaviour: test_lu_gpu_f32.cu:247-269, with the
+// per-sweep factor applied (so that the scaled matrix is exactly R A C).
+void lsp_ruiz_scales(int n, long nnz, const int *row, const int *col, const double *val, int sweeps,
+                     double *R, double *C)
+{
+    std::vector<double> mx(n);
+    for (int i = 0; i < n; i++) { R[i] = 1.0; C[i] = 1.0; }
+    for (int it = 0; it < sweeps; it++) {
+        for (int side = 0; side < 2; side++) {
+            const int *idx = side == 0 ? row : col;
+            double *sc = side == 0 ? R : C;
+            std::fill(mx.begin(), mx.end(), 0.0);
+            for (long e = 0; e < nnz; e++) {
+                const double v = std::fabs(val[e] * R[row[e]] * C[col[e]]);
+                if (v > mx[idx[e]]) mx[idx[e]] = v;
+            }
+            for (int i = 0; i < n; i++)
+                if (mx[i] > 0) sc[i] *= 1.0 / std::sqrt(mx[i]);
+        }
+    }
+}
+

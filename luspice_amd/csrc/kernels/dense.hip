@@ -472,7 +472,8 @@ __global__ void __launch_bounds__(64) dense_finish_kernel(const DenseJob *jobs, 
             atomicAdd(&M.mstat[0], 1);
             atomicMin(&M.mstat[1], k);
         }
-        if (!(fabs(piv) <= 1.0e308) || !(lmax <= 1.0e308)) atomicAdd(&M.mstat[3], 1);
+        const double vmax = sizeof(T) == 2 ? 65504.0 : sizeof(T) == 4 ? 3.4028234663852886e38 : 1.0e308;
+        if (!(fabs(piv) <= vmax) || !(lmax <= vmax)) atomicAdd(&M.mstat[3], 1);
     }
 }
 
@@ -481,7 +482,7 @@ __global__ void __launch_bounds__(64) dense_finish_kernel(const DenseJob *jobs, 
 template <typename T>
 __global__ void __launch_bounds__(256) tail_fwd_gather_kernel(const void *mats, const int *tails,
                                                               const long *voff, const int *active,
-                                                              const double *b, double *y)
+                                                              const double *b, double *y, const double *rs)
 {
     const int mi = tails[blockIdx.y];
     if (active && !active[mi]) return;
@@ -495,7 +496,10 @@ __global__ void __launch_bounds__(256) tail_fwd_gather_kernel(const void *mats, 
     double acc = 0.0;
     for (int e = M.lrp[i] + lane; e < e1; e += kWave) acc += to_f64(M.lrow[e]) * y[vo + M.lci[e]];
     acc = wave_sum(acc);
-    if (lane == 0) y[vo + i] = b[vo + M.rowat[i]] - acc;
+    if (lane == 0) {
+        const int ro = M.rowat[i];
+        y[vo + i] = (rs ? b[vo + ro] * rs[vo + ro] : b[vo + ro]) - acc;
+    }
 }
 
 // Dense triangular solve of the block rows (see the file comment).
@@ -509,7 +513,7 @@ template <typename T, bool kLower>
 __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const int *tails,
                                                         const long *voff, const int *active,
                                                         double *y, double *z, double *x,
-                                                        const int *bwd_epoch, int *abort_word)
+                                                        const int *bwd_epoch, int *abort_word, const double *cs)
 {
     const int mi = tails[blockIdx.y];
     const TriDesc<T> M = ((const TriDesc<T> *)mats)[mi];
@@ -582,7 +586,8 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
                     if (lane < bi) {
                         const int row = k0 + r0 + lane;
                         st_coherent(&z[vo + row], tr);
-                        x[vo + M.q[row]] = tr;
+                        const int qo = M.q[row];
+                        x[vo + qo] = cs ? tr * cs[vo + qo] : tr;
                     }
                 }
                 store_complete();
@@ -643,28 +648,29 @@ template hipError_t launch_dense_finish<__half>(const DenseJob *, int, int, cons
 template <typename T>
 hipError_t launch_tail_solve(const void *mats, const int *tails, int ntails, int maxT, const long *voff,
                              const int *active, const double *b, double *y, double *z, double *x,
-                             bool forward, const int *bwd_epoch, int *abort_word, hipStream_t st)
+                             bool forward, const int *bwd_epoch, int *abort_word, const double *rs,
+                             const double *cs, hipStream_t st)
 {
     const int nblk = (maxT + kNb - 1) / kNb;
     if (forward) {
         hipLaunchKernelGGL(tail_fwd_gather_kernel<T>, dim3((maxT + 3) / 4, ntails), dim3(256), 0, st, mats,
-                           tails, voff, active, b, y);
+                           tails, voff, active, b, y, rs);
         hipLaunchKernelGGL((tail_trsv_kernel<T, true>), dim3(nblk, ntails), dim3(256), 0, st, mats, tails,
-                           voff, active, y, z, x, bwd_epoch, abort_word);
+                           voff, active, y, z, x, bwd_epoch, abort_word, cs);
     } else {
         hipLaunchKernelGGL((tail_trsv_kernel<T, false>), dim3(nblk, ntails), dim3(256), 0, st, mats, tails,
-                           voff, active, y, z, x, bwd_epoch, abort_word);
+                           voff, active, y, z, x, bwd_epoch, abort_word, cs);
     }
     return hipGetLastError();
 }
 template hipError_t launch_tail_solve<float>(const void *, const int *, int, int, const long *, const int *,
                                              const double *, double *, double *, double *, bool, const int *, int *,
-                                             hipStream_t);
+                                             const double *, const double *, hipStream_t);
 template hipError_t launch_tail_solve<double>(const void *, const int *, int, int, const long *, const int *,
                                               const double *, double *, double *, double *, bool, const int *, int *,
-                                              hipStream_t);
+                                              const double *, const double *, hipStream_t);
 template hipError_t launch_tail_solve<__half>(const void *, const int *, int, int, const long *, const int *,
                                               const double *, double *, double *, double *, bool, const int *, int *,
-                                              hipStream_t);
+                                              const double *, const double *, hipStream_t);
 
 }  // namespace lsp

@@ -195,15 +195,18 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
     }
 
     const double piv = x[dl];
+    // a stored value must be finite in the storage type (65504 for FP16)
+    const double vmax = sizeof(T) == 2 ? 65504.0 : sizeof(T) == 4 ? 3.4028234663852886e38 : 1.0e308;
     double amax = 0.0;
     int nonfinite = 0;
     for (int i = dl + 1 + lane; i < m; i += kWave) {
         const double a = fabs(x[i]);
         amax = fmax(amax, a);
-        nonfinite |= !(a <= 1.0e308);
+        nonfinite |= !(a <= 1.0e308) || !(a <= vmax * fabs(piv));
     }
+    for (int i = lane; i < dl; i += kWave) nonfinite |= !(fabs(x[i]) <= vmax);
     amax = wave_max(amax);
-    nonfinite = __any(nonfinite) || !(fabs(piv) <= 1.0e308);
+    nonfinite = __any(nonfinite) || !(fabs(piv) <= vmax);
     if (lane == 0) {
         if (piv == 0.0) {
             atomicMin(&M.mstat[2], k);

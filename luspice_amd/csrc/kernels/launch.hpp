@@ -35,6 +35,8 @@ struct TriLaunch {
     const double *b;        // right-hand sides (original row order)
     double *x;              // solutions (original column order)
     double *y, *z;          // work vectors (position order)
+    const double *rs, *cs;  // optional row / column scales (Ruiz): b_s = R b, x = C z
+    const int *n_active;    // optional: when *n_active == 0 no circuit is active, tasks are skipped
     int ntask;
     int *epoch;             // device epoch counter (see FactorLaunch)
     int *ctr;
@@ -81,7 +83,8 @@ hipError_t launch_dense_finish(const DenseJob *jobs, int njobs, int maxT, const 
 template <typename T>
 hipError_t launch_tail_solve(const void *mats, const int *tails, int ntails, int maxT, const long *voff,
                              const int *active, const double *b, double *y, double *z, double *x,
-                             bool forward, const int *bwd_epoch, int *abort_word, hipStream_t st);
+                             bool forward, const int *bwd_epoch, int *abort_word, const double *rs,
+                             const double *cs, hipStream_t st);
 
 // trisolve.hip
 template <typename T>
@@ -115,5 +118,20 @@ hipError_t k_set_all_active(const KrylovArgs &K, int v, hipStream_t s);
 hipError_t k_zero(double *p, long n, hipStream_t s);
 hipError_t k_zero_int(int *p, long n, hipStream_t s);
 hipError_t k_copy(const double *src, double *dst, long n, hipStream_t s);
+
+// Ruiz equilibration of every circuit's values (one workgroup per circuit).
+struct RuizArgs {
+    const double *ain;      // values, caller order, all circuits
+    double *aout;           // scaled values R A C, same layout
+    const int *row, *col;   // per entry: circuit-local original row / column
+    const long *aoff;       // ncirc+1 entry offsets
+    const long *voff;       // ncirc+1 vector offsets
+    double *rs, *cs;        // row / column scales (vector layout)
+    unsigned long long *mx; // scratch, vector layout
+    const int *active;      // per-circuit mask (nullptr = all)
+    int sweeps;
+    int pad_;
+};
+hipError_t launch_ruiz(const RuizArgs &R, int ncirc, hipStream_t st);
 
 }  // namespace lsp

@@ -80,8 +80,8 @@ __device__ bool tri_forward(const TriDesc<T> &M, int s, int e, double *y, int *f
 // from the bottom, each block first subtracting the later blocks' rows
 // (U(i, j) = val[dp[j] - (j - i)] within column j's run ru[j], else 0).
 template <typename T>
-__device__ bool tri_backward(const TriDesc<T> &M, int s, int e, double *z, double *x, int *flags, int epoch,
-                             int *abort_word)
+__device__ bool tri_backward(const TriDesc<T> &M, int s, int e, double *z, double *x, const double *cs,
+                             int *flags, int epoch, int *abort_word)
 {
     const int lane = lane_id();
     const LSP_GLOBAL int *dp = gp(M.dp);
@@ -122,7 +122,8 @@ __device__ bool tri_backward(const TriDesc<T> &M, int s, int e, double *z, doubl
         else zt = run_triangle_upper<kWave>(M.val, num, rd, dpc, run, bw);
         if (lane < bw) {
             st_coherent(&z[row], zt);
-            x[M.q[row]] = zt;
+            const int qo = M.q[row];
+            x[qo] = cs ? zt * cs[qo] : zt;
         }
         store_complete();
         if (lane < bw) st_relaxed(&flags[row], epoch);
@@ -138,7 +139,10 @@ __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
     const int lane = lane_id();
     const TriDesc<T> *mats = (const TriDesc<T> *)L.mats;
     const int epoch = read_epoch(L.epoch);
+    // a Krylov iteration after every circuit converged: nothing to solve
+    const bool idle = L.n_active && uniform(ld_relaxed(L.n_active)) == 0;
     for (;;) {
+        if (idle) break;
         const int t = wave_ticket(&L.ctr[0]);
         if (t >= L.ntask) break;
         if (aborted(L.abort_word)) continue;
@@ -154,7 +158,7 @@ __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
             const int s0 = -code - 1;
             const int e0 = M.snend[s0];
             if (kForward) tri_forward<T>(M, s0, e0, y, flags, epoch, L.abort_word);
-            else tri_backward<T>(M, s0, e0, z, L.x + vo, flags, epoch, L.abort_word);
+            else tri_backward<T>(M, s0, e0, z, L.x + vo, L.cs ? L.cs + vo : nullptr, flags, epoch, L.abort_word);
             continue;
         }
         const int i = code >> 1;
@@ -195,13 +199,16 @@ __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
         acc = wave_sum(acc);
         if (lane == 0) {
             if (kForward) {
-                st_coherent(&y[i], L.b[vo + M.rowat[i]] - acc);
+                const int ro = M.rowat[i];
+                const double bi = L.rs ? L.b[vo + ro] * L.rs[vo + ro] : L.b[vo + ro];
+                st_coherent(&y[i], bi - acc);
             } else if (part) {
                 st_coherent(&z[i], y[i] - acc);
             } else {
                 const double zi = (y[i] - acc) / to_f64(M.diag[i]);
                 st_coherent(&z[i], zi);
-                L.x[vo + M.q[i]] = zi;
+                const int qo = M.q[i];
+                L.x[vo + qo] = L.cs ? zi * L.cs[vo + qo] : zi;
             }
             store_complete();
             st_relaxed(&flags[i], part ? -epoch : epoch);

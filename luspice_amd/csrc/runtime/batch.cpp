@@ -12,6 +12,8 @@
 //   word per inner step (the number of circuits still iterating).
 #include "batch.hpp"
 
+#include <type_traits>
+
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -28,6 +30,8 @@ namespace lsp {
 Batch::~Batch()
 {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
     for (auto &m : mem_) if (m.p) (void)hipFree(m.p);
     if (h_pin_) (void)hipHostFree(h_pin_);
     for (auto &e : ev_) if (e) (void)hipEventDestroy(e);
@@ -188,6 +192,15 @@ static int build_circuit(CircuitHost &C, const lsp_circuit_in &in, const lsp_opt
     lsp_coo_to_csc(C.n, C.nnz, C.row.data(), C.col.data(), C.q.data(), Ap, Ai, src);
     std::vector<double> Ax(C.nnz);
     for (long p = 0; p < C.nnz; p++) Ax[p] = C.val0[src[p]];
+    if (opt.pivot_scaled) {
+        // pivot on the equilibrated matrix R A C (the order the reference
+        // uses when Ruiz is on: scale, then order and factor)
+        std::vector<double> R(C.n), Cs(C.n);
+        lsp_ruiz_scales(C.n, C.nnz, C.row.data(), C.col.data(), C.val0.data(),
+                        opt.ruiz_sweeps > 0 ? opt.ruiz_sweeps : 30, R.data(), Cs.data());
+        for (int k = 0; k < C.n; k++)
+            for (int p = Ap[k]; p < Ap[k + 1]; p++) Ax[p] *= R[Ai[p]] * Cs[C.q[k]];
+    }
     int rc = lsp_gp_factor(C.n, Ap.data(), Ai.data(), Ax.data(), C.q.data(), opt.pivot_tol, C.sym);
     if (rc != -1) return LSP_E_SINGULAR;
     if (lsp_build_plan(C.n, Ap.data(), Ai.data(), src.data(), C.sym, C.q.data(), C.plan) != 0)
@@ -227,7 +240,7 @@ int Batch::ensure_factor_set(FactorSet<T> &fs)
         F.aptr = d_aptr_ + C.o_np1;
         F.asrc = d_asrc_ + C.o_a;
         F.aslot = d_aslot_ + C.o_a;
-        F.ain = d_ain_ + C.aoff;
+        F.ain = (std::is_same<T, __half>::value ? d_ains_ : d_ain_) + C.aoff;   // FP16: Ruiz-scaled
         F.relp = d_relp_ + C.o_np1;
         F.rel = d_rel_ + C.o_rel;
         F.snend = d_snend_ + C.o_n;
@@ -332,6 +345,7 @@ size_t Batch::estimate_device_bytes() const
     b += 8 * (n + nc) + 4 * nlu + 24 * n + 2 * rel;                       // lrp urp lci uci rowat q snend ru fcut bcut rel
     b += 8 * (slices + nc) + 8 * n + 8 * sell;                            // SELL structure
     b += 12 * n + 8 * nnz + 12 * sell;                                    // flags, values
+    b += 16 * nnz + 24 * n + 16 * (nc + 1);                               // Ruiz: row/col, scaled values, scales
     b += 8 * T2 + 16 * T + 65536.0 * nc;                                   // dense block work
     b += 24 * n + 8 * slices;                                             // task lists
     b += 64 * n + 8 * (m + 1) * n;                                        // vectors, Krylov basis
@@ -494,6 +508,28 @@ int Batch::upload_all()
         }
     }
     d_ain_ = (double *)dalloc(nnz_tot_ * sizeof(double));
+    // Ruiz equilibration for FP16 factor storage: per-entry circuit-local
+    // row / column, scaled values, row / column scales, scratch
+    {
+        std::vector<int> ar, ac;
+        std::vector<long> ao(nc + 1, 0), vo(nc + 1, 0);
+        ar.reserve(nnz_tot_); ac.reserve(nnz_tot_);
+        for (int m = 0; m < nc; m++) {
+            const CircuitHost &C = circ_[m];
+            ar.insert(ar.end(), C.row.begin(), C.row.end());
+            ac.insert(ac.end(), C.col.begin(), C.col.end());
+            ao[m + 1] = ao[m] + C.nnz;
+            vo[m + 1] = vo[m] + C.n;
+        }
+        d_arow_ = upload(ar); d_acol_ = upload(ac);
+        d_raoff_ = upload(ao); d_rvoff_ = upload(vo);
+        d_ains_ = (double *)dalloc(nnz_tot_ * sizeof(double));
+        d_rscale_ = (double *)dalloc(ntot_ * sizeof(double));
+        d_cscale_ = (double *)dalloc(ntot_ * sizeof(double));
+        d_rmx_ = (unsigned long long *)dalloc(ntot_ * sizeof(unsigned long long));
+        if (!d_arow_ || !d_acol_ || !d_raoff_ || !d_rvoff_ || !d_ains_ || !d_rscale_ || !d_cscale_ || !d_rmx_)
+            return LSP_E_ALLOC;
+    }
     d_sval64_ = (double *)dalloc(nsell_tot_ * sizeof(double));
     d_sval32_ = (float *)dalloc(nsell_tot_ * sizeof(float));
     if (!d_ain_ || !d_sval64_ || !d_sval32_) return LSP_E_ALLOC;
@@ -719,7 +755,7 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
     L.dbg = d_dbg_;
     rc = check(launch_reset_mstat(d_mstat_, (int)circ_.size(), d_abort_, d_dbg_, stream_), "reset");
     if (rc) return rc;
-    (void)hipEventRecord(ev_[0], stream_);
+    if (!capturing_) (void)hipEventRecord(ev_[0], stream_);
     rc = check(launch_refactor<T>(L, fblocks_, stream_), "refactor launch");
     if (!rc && njobs_) {
         rc = check(launch_dense_lu(d_djobs_, njobs_, max_tail_T_, d_active, stream_), "dense lu");
@@ -727,8 +763,26 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
             rc = check(launch_dense_finish<T>(d_djobs_, njobs_, max_tail_T_, fs.fdesc, d_active,
                                               opt_.refactor_ptol, stream_), "dense finish");
     }
-    (void)hipEventRecord(ev_[1], stream_);
+    if (!capturing_) (void)hipEventRecord(ev_[1], stream_);
     return rc;
+}
+
+// The refactor launch sequence of one precision (no host synchronisation).
+int Batch::launch_refactor_prec(int prec, const int *d_active)
+{
+    if (prec == LSP_PREC_FP32) return refactor_t(f32_, d_active);
+    if (prec == LSP_PREC_FP64) return refactor_t(f64_, d_active);
+    if (prec == LSP_PREC_FP16) {
+        // FP16 storage only after Ruiz equilibration: R A C is factored, and
+        // the solves apply R to their input and C to their output
+        RuizArgs R{};
+        R.ain = d_ain_; R.aout = d_ains_; R.row = d_arow_; R.col = d_acol_;
+        R.aoff = d_raoff_; R.voff = d_rvoff_; R.rs = d_rscale_; R.cs = d_cscale_; R.mx = d_rmx_;
+        R.active = d_active; R.sweeps = opt_.ruiz_sweeps > 0 ? opt_.ruiz_sweeps : 30;
+        int rc = check(launch_ruiz(R, (int)circ_.size(), stream_), "ruiz");
+        return rc ? rc : refactor_t(f16_, d_active);
+    }
+    return LSP_E_ARG;
 }
 
 int Batch::refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *subset)
@@ -743,10 +797,7 @@ int Batch::refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *
         if (rc) return rc;
         d_active = d_subset_;
     }
-    if (prec == LSP_PREC_FP32) rc = refactor_t(f32_, d_active);
-    else if (prec == LSP_PREC_FP64) rc = refactor_t(f64_, d_active);
-    else if (prec == LSP_PREC_FP16) rc = refactor_t(f16_, d_active);
-    else return LSP_E_ARG;
+    rc = launch_refactor_prec(prec, d_active);
     if (rc) return rc;
     cur_prec_ = prec;
     const int nc = (int)circ_.size();
@@ -801,7 +852,7 @@ int Batch::refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *
 
 // ---------------------------------------------------------------------------
 template <typename T>
-int Batch::trisolve_t(FactorSet<T> &fs, const double *d_b, double *d_x, const int *d_active)
+int Batch::trisolve_t(FactorSet<T> &fs, const double *d_b, double *d_x, const int *d_active, const int *d_nact)
 {
     if (!fs.ready) return LSP_E_ARG;
     TriLaunch Lf{};
@@ -818,34 +869,40 @@ int Batch::trisolve_t(FactorSet<T> &fs, const double *d_b, double *d_x, const in
     Lf.ctr = d_ctr_ + 2;
     Lf.abort_word = d_abort_ + 1;
     Lf.total_waves = tblocks_f_ * 4;
+    const bool scaled = std::is_same<T, __half>::value;
+    Lf.rs = scaled ? d_rscale_ : nullptr;
+    Lf.cs = scaled ? d_cscale_ : nullptr;
+    Lf.n_active = d_nact;
     TriLaunch Lb = Lf;
     Lb.task = d_bwtask_;
     Lb.ntask = (int)nbw_tot_;
     Lb.epoch = d_epoch_ + 2;
     Lb.ctr = d_ctr_ + 4;
     Lb.total_waves = tblocks_b_ * 4;
-    (void)hipEventRecord(ev_[2], stream_);
+    if (!capturing_) (void)hipEventRecord(ev_[2], stream_);
     int rc = check(launch_trisolve_half<T>(Lf, true, tblocks_f_, stream_), "trisolve fwd");
     if (!rc && njobs_)
         rc = check(launch_tail_solve<T>(fs.tdesc, d_tails_, njobs_, max_tail_T_, d_voff_, d_active, d_b,
-                                        d_ty_, d_tz_, d_x, true, d_epoch_ + 2, d_abort_ + 1, stream_),
+                                        d_ty_, d_tz_, d_x, true, d_epoch_ + 2, d_abort_ + 1, Lf.rs, Lf.cs,
+                                        stream_),
                    "tail fwd");
-    (void)hipEventRecord(ev_[4], stream_);
+    if (!capturing_) (void)hipEventRecord(ev_[4], stream_);
     if (!rc && njobs_)
         rc = check(launch_tail_solve<T>(fs.tdesc, d_tails_, njobs_, max_tail_T_, d_voff_, d_active, d_b,
-                                        d_ty_, d_tz_, d_x, false, d_epoch_ + 2, d_abort_ + 1, stream_),
+                                        d_ty_, d_tz_, d_x, false, d_epoch_ + 2, d_abort_ + 1, Lf.rs, Lf.cs,
+                                        stream_),
                    "tail bwd");
     if (!rc) rc = check(launch_trisolve_half<T>(Lb, false, tblocks_b_, stream_), "trisolve bwd");
-    (void)hipEventRecord(ev_[3], stream_);
+    if (!capturing_) (void)hipEventRecord(ev_[3], stream_);
     tri_pending_ = true;
     return rc;
 }
 
-int Batch::launch_trisolve_prec(int prec, const double *d_b, double *d_x, const int *active)
+int Batch::launch_trisolve_prec(int prec, const double *d_b, double *d_x, const int *active, const int *d_nact)
 {
-    if (prec == LSP_PREC_FP32) return trisolve_t(f32_, d_b, d_x, active);
-    if (prec == LSP_PREC_FP64) return trisolve_t(f64_, d_b, d_x, active);
-    if (prec == LSP_PREC_FP16) return trisolve_t(f16_, d_b, d_x, active);
+    if (prec == LSP_PREC_FP32) return trisolve_t(f32_, d_b, d_x, active, d_nact);
+    if (prec == LSP_PREC_FP64) return trisolve_t(f64_, d_b, d_x, active, d_nact);
+    if (prec == LSP_PREC_FP16) return trisolve_t(f16_, d_b, d_x, active, d_nact);
     return LSP_E_ARG;
 }
 
@@ -1136,6 +1193,160 @@ int Batch::export_factors(int c, int *pinv, int *Lp, int *Li, double *Lx, int *U
     }
     if (Lp) Lp[C.n] = (int)l;
     if (Up) Up[C.n] = (int)u;
+    return LSP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// One Newton step as a replayed HIP graph: refactor (prec) + one GMRES cycle
+// of up to m inner iterations + the true residual, on the resident b / x.
+// Nothing inside synchronises with the host: circuits that converge stop
+// through the device activity mask, so a fixed launch sequence serves every
+// step.  The graph is recorded on first use (and again when prec changes);
+// epochs and queue counters live on the device, so replays need no
+// arguments.  After the replay the host reads the reports once; a circuit
+// still above the tolerance is finished by the eager solve (restart from 0,
+// fp64 fallback as in solve()).
+constexpr int kGraphInner = 8;          // GMRES inner iterations recorded in the step graph
+
+int Batch::record_step(int prec)
+{
+    const KrylovArgs K = kargs();
+    const int nc = (int)circ_.size();
+    const int mm = K.mmax;
+    hipStream_t s = stream_;
+    const double *d_bv = d_b_;
+    double *d_xv = d_x_;
+    int rc;
+#define LSP_TRY(expr) do { rc = (expr); if (rc) return rc; } while (0)
+#define LSP_HIP(expr, what) LSP_TRY(check((expr), what))
+    LSP_TRY(launch_refactor_prec(prec, nullptr));
+    LSP_HIP(k_zero_int(d_abort_ + 1, 1, s), "abort reset");
+    LSP_HIP(k_zero_int(d_conv_, nc, s), "conv");
+    LSP_HIP(k_activate(K, s), "activate");
+    LSP_HIP(k_chunk_dots(K, d_bv, d_bv, 1, s), "dots");
+    LSP_HIP(k_reduce(K, 1, 2, 0, d_nb_, s), "reduce");
+    LSP_HIP(k_chunk_fill(K, d_xv, 0.0, s), "zero x");
+    LSP_HIP(k_activate(K, s), "activate");
+    LSP_TRY(launch_spmv64(d_xv, d_r_, d_bv, d_act_));
+    LSP_HIP(k_chunk_dots(K, d_r_, d_r_, 1, s), "dots");
+    LSP_HIP(k_reduce(K, 1, 2, 0, d_beta_, s), "reduce");
+    LSP_HIP(k_zero_int(d_nact_, 1, s), "zero");
+    LSP_HIP(k_start_cycle(K, d_nact_, s), "start");
+    LSP_HIP(k_chunk_scale(K, d_r_, d_V_, d_beta_, s), "scale");
+    // a replay runs every recorded iteration: record the few a step usually
+    // needs; an idle iteration's solves exit at once (n_active == 0)
+    const int mg = std::min(mm, kGraphInner);
+    for (int j = 0; j < mg; j++) {
+        double *Vj = d_V_ + (long)j * ntot_;
+        LSP_TRY(launch_trisolve_prec(prec, Vj, d_z_, d_act_, d_nact_));
+        LSP_TRY(launch_spmv64(d_z_, d_w_, nullptr, d_act_));
+        for (int pass = 0; pass < 2; pass++) {
+            LSP_HIP(k_chunk_dots(K, d_V_, d_w_, j + 1, s), "dots");
+            LSP_HIP(k_reduce(K, j + 1, 1, j, nullptr, s), "reduce");
+            LSP_HIP(k_chunk_update(K, d_V_, d_w_, j + 1, s), "update");
+        }
+        LSP_HIP(k_chunk_dots(K, d_w_, d_w_, 1, s), "dots");
+        LSP_HIP(k_reduce(K, 1, 2, 0, d_hn_, s), "reduce");
+        LSP_HIP(k_chunk_scale(K, d_w_, d_V_ + (long)(j + 1) * ntot_, d_hn_, s), "scale");
+        LSP_HIP(k_zero_int(d_nact_, 1, s), "zero");
+        LSP_HIP(k_givens(K, j, d_nact_, s), "givens");
+    }
+    LSP_HIP(k_backsolve(K, s), "backsolve");
+    LSP_HIP(k_chunk_combine(K, d_V_, d_t_, s), "combine");
+    LSP_TRY(launch_trisolve_prec(prec, d_t_, d_z_, d_act_));
+    LSP_HIP(k_chunk_axpy(K, d_xv, d_z_, s), "axpy");
+    LSP_HIP(k_set_all_active(K, 1, s), "act");
+    LSP_TRY(launch_spmv64(d_xv, d_r_, d_bv, d_act_));
+    LSP_HIP(k_chunk_dots(K, d_r_, d_r_, 1, s), "dots");
+    LSP_HIP(k_reduce(K, 1, 2, 0, d_beta_, s), "reduce");
+#undef LSP_TRY
+#undef LSP_HIP
+    return LSP_OK;
+}
+
+int Batch::graph_step(int prec, lsp_refactor_report *rrep, lsp_solve_report *srep)
+{
+    if (!stream_ || circ_.empty()) return LSP_E_ARG;
+    int rc;
+    if (!graph_exec_ || graph_prec_ != prec) {
+        // every allocation happens before the capture
+        if (prec == LSP_PREC_FP32) rc = ensure_factor_set(f32_);
+        else if (prec == LSP_PREC_FP64) rc = ensure_factor_set(f64_);
+        else if (prec == LSP_PREC_FP16) rc = ensure_factor_set(f16_);
+        else return LSP_E_ARG;
+        if (rc) return rc;
+        if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+        if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+        rc = check(hipStreamSynchronize(stream_), "sync");
+        if (rc) return rc;
+        rc = check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+        if (rc) return rc;
+        capturing_ = true;
+        const int rc_rec = record_step(prec);
+        capturing_ = false;
+        hipGraph_t g = nullptr;
+        rc = check(hipStreamEndCapture(stream_, &g), "end capture");
+        if (rc_rec || rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc_rec ? rc_rec : rc;
+        }
+        graph_ = g;
+        rc = check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "graph instantiate");
+        if (rc) return rc;
+        graph_prec_ = prec;
+    }
+    rc = check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
+    if (rc) return rc;
+    cur_prec_ = prec;
+    const int nc = (int)circ_.size();
+    std::vector<int> ms(4 * nc), jf(nc);
+    std::vector<double> beta(nc), nb(nc);
+    int ab[2] = {0, 0};
+    rc = check(hipMemcpyAsync(ms.data(), d_mstat_, ms.size() * 4, hipMemcpyDeviceToHost, stream_), "mstat");
+    if (!rc) rc = check(hipMemcpyAsync(beta.data(), d_beta_, nc * 8, hipMemcpyDeviceToHost, stream_), "beta");
+    if (!rc) rc = check(hipMemcpyAsync(nb.data(), d_nb_, nc * 8, hipMemcpyDeviceToHost, stream_), "nb");
+    if (!rc) rc = check(hipMemcpyAsync(jf.data(), d_jfin_, nc * 4, hipMemcpyDeviceToHost, stream_), "jfin");
+    if (!rc) rc = check(hipMemcpyAsync(h_pin_, d_abort_, 8, hipMemcpyDeviceToHost, stream_), "abort");
+    if (!rc) rc = check(hipStreamSynchronize(stream_), "graph sync");
+    if (rc) return rc;
+    ab[0] = h_pin_[0];
+    ab[1] = h_pin_[1];
+    stats_.graph_steps++;
+    lsp_refactor_report r{};
+    r.first_bad_circuit = -1;
+    for (int m = 0; m < nc; m++) {
+        r.bad_pivots += ms[4 * m];
+        if (ms[4 * m + 2] != 0x7fffffff) r.singular_circuits++;
+        if ((ms[4 * m] || ms[4 * m + 2] != 0x7fffffff || ms[4 * m + 3]) && r.first_bad_circuit < 0)
+            r.first_bad_circuit = m;
+    }
+    r.status = ab[0] == kErrSpin ? LSP_E_DEVICE : ab[0] == kErrCapacity ? LSP_E_CAPACITY
+             : r.singular_circuits ? LSP_E_SINGULAR : LSP_OK;
+    if (rrep) *rrep = r;
+    if (r.status == LSP_E_DEVICE || r.status == LSP_E_CAPACITY) return r.status;
+    if (ab[1]) return LSP_E_DEVICE;
+    lsp_solve_report sr{};
+    sr.outer = 1;
+    int conv = 0, jmax = 0;
+    for (int m = 0; m < nc; m++) {
+        double rel = nb[m] > 0 ? beta[m] / nb[m] : beta[m];
+        if (!(rel == rel)) rel = INFINITY;
+        sr.max_rel_res = std::max(sr.max_rel_res, rel);
+        if (rel <= opt_.ir_outer_tol * 10 || rel <= 1e-9) conv++;
+        jmax = std::max(jmax, jf[m]);
+    }
+    sr.inner_total = jmax;
+    sr.converged = conv;
+    sr.status = LSP_OK;
+    if (conv < nc || r.status != LSP_OK) {
+        // finish eagerly (rare): full IR from x = 0 with the fp64 fallback
+        lsp_solve_report s2{};
+        rc = solve(d_b_, d_x_, true, &s2);
+        if (rc) return rc;
+        s2.inner_total += sr.inner_total;
+        sr = s2;
+    }
+    if (srep) *srep = sr;
     return LSP_OK;
 }
 

@@ -60,6 +60,8 @@ public:
     int set_stream(hipStream_t s);
     int set_values(const double *vals, bool on_device);
     int refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *subset = nullptr);
+    int launch_refactor_prec(int prec, const int *d_active);
+    int record_step(int prec);
     int trisolve(const double *b, double *x, bool on_device);
     int solve(const double *b, double *x, bool on_device, lsp_solve_report *rep);
     int spmv(const double *x, double *y, bool on_device, int prec);
@@ -67,6 +69,7 @@ public:
     int export_factors(int c, int *pinv, int *Lp, int *Li, double *Lx, int *Up, int *Ui,
                        double *Ux);
     int device_vectors(double **b, double **x);
+    int graph_step(int prec, lsp_refactor_report *rrep, lsp_solve_report *srep);
     int copy_vector(const double *p, bool to_rhs);
     int last_times(double *rf, double *fw, double *bw) const;
     int kernel_stats(lsp_kernel_stats *st, int reset);
@@ -81,7 +84,8 @@ public:
     // device-resident pieces used by the serving loop and the bench
     int gmres_ir(const double *d_b, double *d_x, int prec, const std::vector<char> *subset,
                  lsp_solve_report *rep);
-    int launch_trisolve_prec(int prec, const double *d_b, double *d_x, const int *active);
+    int launch_trisolve_prec(int prec, const double *d_b, double *d_x, const int *active,
+                             const int *d_nact = nullptr);
     int launch_spmv64(const double *d_x, double *d_y, const double *d_b, const int *active);
     hipStream_t stream() const { return stream_; }
     int cur_prec() const { return cur_prec_; }
@@ -91,7 +95,7 @@ private:
     template <typename T> int ensure_factor_set(FactorSet<T> &fs);
     template <typename T> int refactor_t(FactorSet<T> &fs, const int *d_active);
     template <typename T> int trisolve_t(FactorSet<T> &fs, const double *d_b, double *d_x,
-                                         const int *d_active);
+                                         const int *d_active, const int *d_nact = nullptr);
     void *dalloc(size_t bytes);
     void dfree(void *p);
     int alloc_krylov(int mm);
@@ -120,6 +124,16 @@ private:
     uint16_t *d_rel_ = nullptr;
     int *d_snend_ = nullptr, *d_ru_ = nullptr, *d_fcut_ = nullptr, *d_bcut_ = nullptr;
     double *d_tailW_ = nullptr;
+    // HIP graph of one Newton step (graph_step)
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    int graph_prec_ = -1;
+    bool capturing_ = false;
+    // Ruiz equilibration (FP16 factor storage)
+    int *d_arow_ = nullptr, *d_acol_ = nullptr;
+    long *d_raoff_ = nullptr, *d_rvoff_ = nullptr;
+    double *d_ains_ = nullptr, *d_rscale_ = nullptr, *d_cscale_ = nullptr;
+    unsigned long long *d_rmx_ = nullptr;
     double *d_ain_ = nullptr, *d_sval64_ = nullptr;
     float *d_sval32_ = nullptr;
     SellDesc<double> *d_sdesc64_ = nullptr;

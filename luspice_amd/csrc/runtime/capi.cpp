@@ -115,6 +115,11 @@ int lsp_batch_export_factors(lsp_batch *b, int c, int *pinv, int *Lp, int *Li, d
     return b ? b->impl.export_factors(c, pinv, Lp, Li, Lx, Up, Ui, Ux) : LSP_E_ARG;
 }
 
+int lsp_batch_graph_step(lsp_batch *b, int prec, lsp_refactor_report *rrep, lsp_solve_report *srep)
+{
+    return b ? b->impl.graph_step(prec, rrep, srep) : LSP_E_ARG;
+}
+
 int lsp_batch_device_vectors(lsp_batch *b, double **rhs, double **x)
 {
     return b ? b->impl.device_vectors(rhs, x) : LSP_E_ARG;
@@ -258,6 +263,9 @@ int lsp_mna_generate(int kind, int n_req, double lr, uint64_t seed, int *n, long
     else if (kind == 2) {
         if (n_req < 3) return LSP_E_ARG;
         lsp_spec_sized(&sp, n_req, lr, seed);
+    } else if (kind == 3) {
+        if (n_req < 3) return LSP_E_ARG;
+        lsp_spec_c2(&sp, n_req, seed);
     } else {
         return LSP_E_ARG;
     }

@@ -57,10 +57,14 @@ class NewtonBatched:
         self.dev_vals = [DeviceArray(v) for v, _ in self.host]
         self.dev_rhs = [DeviceArray(b) for _, b in self.host]
 
-    def run_step(self, s: int, prec: str = "fp32"):
+    def run_step(self, s: int, prec: str = "fp32", graph: bool = False):
+        """Values and b from HBM, refactor + GMRES-IR; graph=True replays the
+        step as one HIP graph (recorded on first use)."""
         B = self.B
         B.set_values_device(self.dev_vals[s].ptr)
         B.set_rhs(self.dev_rhs[s].ptr)
+        if graph:
+            return B.graph_step(prec)
         rr = B.refactor(prec)
         sr = B.solve_device()
         return rr, sr

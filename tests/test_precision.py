@@ -1,0 +1,114 @@
+"""FP16 factor storage after Ruiz equilibration, and the precision-sweep
+generator.
+
+GPU: on a well-conditioned matrix hidden behind row / column scales of
+1e-4..1e4, Ruiz equilibrates it and one FP16 solve is accurate to ~1e-2
+(R and C are applied correctly); on a precision-sweep circuit, the FP16
+factors precondition fp64 GMRES-IR, a converged solve is within 1e-6 of the
+fp64 oracle and the automatic fp64 fallback always reaches it.
+CPU: the generator is seeded and has the intended shape.
+"""
+import numpy as np
+import pytest
+
+import luspice_amd as L
+from oracle import oracle as O
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_c2_generator_shape():
+    a = L.mna("c2", n=1500, seed=2005)
+    b = L.mna("c2", n=1500, seed=2005)
+    assert np.array_equal(a.val, b.val) and np.array_equal(a.rhs, b.rhs)
+    assert a.n == 1500 and 5.5 <= a.nnz / a.n <= 6.5
+    tiny = float(np.mean(np.abs(a.val) < 6e-5))
+    assert 0.04 <= tiny <= 0.45
+    assert np.abs(a.val).max() <= 1e3 * 12 and np.abs(a.val[a.val != 0]).min() < 1e-5
+
+
+def _scaled_grid(k=30, seed=11):
+    """Diagonally dominant 5-point grid matrix (well conditioned) with row and
+    column scales 10^U(-4, 4): unscaled its entries span ~1e-9..1e9, far
+    beyond half precision; Ruiz equilibration brings them to [0, 1]."""
+    rng = np.random.default_rng(seed)
+    n = k * k
+    rows, cols = [], []
+    for r in range(k):
+        for c in range(k):
+            v = r * k + c
+            for dr, dc in ((0, 1), (1, 0), (0, -1), (-1, 0)):
+                rr, cc = r + dr, c + dc
+                if 0 <= rr < k and 0 <= cc < k:
+                    rows.append(v); cols.append(rr * k + cc)
+    rows = np.array(rows, np.int32); cols = np.array(cols, np.int32)
+    off = rng.uniform(-1.0, 1.0, rows.size)
+    dsum = np.zeros(n); np.add.at(dsum, rows, np.abs(off))
+    row = np.concatenate([rows, np.arange(n, dtype=np.int32)])
+    col = np.concatenate([cols, np.arange(n, dtype=np.int32)])
+    val = np.concatenate([off, 1.0 + dsum])
+    rs = 10.0 ** rng.uniform(-4, 4, n)
+    cs = 10.0 ** rng.uniform(-4, 4, n)
+    val = val * rs[row] * cs[col]
+    return L.Circuit(n, row, col, val, rng.uniform(-1, 1, n))
+
+
+@pytest.mark.gpu
+def test_fp16_ruiz_scaling_one_solve():
+    print("case: scaled 30x30 grid, one FP16 solve after Ruiz", flush=True)
+    c = _scaled_grid()
+    q = L.colamd(c)
+    rc, xr, _, _ = O.cpu_lu_ref(c.n, c.row, c.col, c.val, q, c.rhs)
+    assert rc == -1
+    B = L.Batch(L.Circuit(c.n, c.row, c.col, c.val, c.rhs, q=q), fallback_fp64=0, pivot_scaled=1)
+    rr = B.refactor("fp16")
+    assert rr["rc"] == 0 and rr["first_bad_circuit"] == -1, rr
+    xt = B.trisolve(c.rhs)                       # R and C applied around the FP16 solve
+    assert np.all(np.isfinite(xt)) and _rel(xt, xr) < 5e-2          # CPU emulation of FP16 factors: 4.6e-3
+    # the residual stop (||b - A x|| <= 1e-10 ||b|| on the unscaled A) may be
+    # out of fp64 reach for such scales; the error is what must be small
+    x, sr = B.solve(c.rhs)
+    assert _rel(x, xr) <= 1e-6, sr
+    B.close()
+
+
+@pytest.mark.gpu
+def test_fp16_ruiz_gmres_ir():
+    print("case: c2 n=1500 seed 2005, FP16 factors after Ruiz", flush=True)
+    c = L.mna("c2", n=1500, seed=2005)
+    q = L.colamd(c)
+    rc, xr, _, _ = O.cpu_lu_ref(c.n, c.row, c.col, c.val, q, c.rhs)
+    assert rc == -1
+    circ = L.Circuit(c.n, c.row, c.col, c.val, c.rhs, q=q)
+    B = L.Batch(circ, fallback_fp64=0, pivot_scaled=1)
+    rr = B.refactor("fp16")
+    assert rr["rc"] == 0, rr
+    xt = B.trisolve(c.rhs)                       # one preconditioner application: finite
+    assert np.all(np.isfinite(xt))
+    x, sr = B.solve(c.rhs)
+    if sr["converged"]:
+        assert _rel(x, xr) <= 1e-6, sr
+    B.close()
+    B2 = L.Batch(circ, pivot_scaled=1)           # fp64 fallback enabled (default)
+    B2.refactor("fp16")
+    x2, sr2 = B2.solve(c.rhs)
+    assert _rel(x2, xr) <= 1e-6, sr2
+    B2.close()
+
+
+@pytest.mark.gpu
+def test_fp32_and_fp64_on_c2():
+    print("case: c2 n=1200 seed 2007, fp64 and fp32 factors", flush=True)
+    c = L.mna("c2", n=1200, seed=2007)
+    q = L.colamd(c)
+    rc, xr, _, _ = O.cpu_lu_ref(c.n, c.row, c.col, c.val, q, c.rhs)
+    circ = L.Circuit(c.n, c.row, c.col, c.val, c.rhs, q=q)
+    B = L.Batch(circ)
+    for prec in ("fp64", "fp32"):
+        rr = B.refactor(prec)
+        assert rr["rc"] == 0
+        x, sr = B.solve(c.rhs)
+        assert _rel(x, xr) <= 1e-6, (prec, sr)
+    B.close()

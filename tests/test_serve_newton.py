@@ -50,6 +50,23 @@ def test_newton_batched_matches_oracle():
 
 
 @pytest.mark.gpu
+def test_graph_step_matches_eager():
+    print("case: 12-circuit Newton batch, graph replay vs eager, 4 steps", flush=True)
+    circ = W.newton_mix(12, 100, 2000, seed=5)
+    NB = W.NewtonBatched(circ, 4, seed=5)
+    for s in range(4):
+        rr, sr = NB.run_step(s)                      # eager, with its host synchronisation
+        assert rr["rc"] == 0 and sr["rc"] == 0, (rr, sr)
+        xe = NB.solution().copy()
+        rg, sg = NB.run_step(s, graph=True)          # graph replay of the same step
+        assert rg["rc"] == 0 and sg["rc"] == 0, (rg, sg)
+        xg = NB.solution()
+        assert sg["converged"] == len(circ)
+        assert _rel(xg, xe) <= 1e-13, s
+    assert NB.B.kernel_stats()["graph_steps"] == 4
+
+
+@pytest.mark.gpu
 def test_serve_same_alone_and_mixed():
     print("case: 6 concurrent requests vs alone", flush=True)
     reqs = S.make_requests(6, rate=1e6, nmin=200, nmax=3000, smin=3, smax=4, seed=4)
