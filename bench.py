@@ -50,6 +50,19 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter runs
+    (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this command, calibrated by
+    the known-size copy), or None when no such measurement is committed."""
+    path = os.path.join(ROOT, "profiles", "traffic_c1.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t.get(kernel, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
 def algorithmic_bytes(inf, nnz, n, vF):
     """Per-launch algorithmic bytes (compulsory traffic) of each kernel."""
     nLU = inf["lnnz"] + inf["unnz"]
@@ -64,7 +77,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-sample", type=int, default=3, help="CPU baseline factor+solve runs")
+    ap.add_argument("--cpu-sample", type=int, default=5, help="CPU baseline factor+solve runs")
+    ap.add_argument("--calib-mb", type=int, default=64,
+                    help="size of the device copy issued before warm-up (counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -89,6 +104,17 @@ def main():
     # the right-hand side lives in the batch's resident arena for the whole run
     B.set_rhs(c.rhs)
     L.device_sync()
+    # one plain device-to-device copy of known size before anything is timed:
+    # in a `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` run of this command it
+    # calibrates the counters (bytes counted per byte copied)
+    if args.calib_mb > 0:
+        nb = args.calib_mb << 20
+        ca = L.DeviceArray(np.zeros(nb // 8))
+        cb = L.DeviceArray(np.zeros(nb // 8))
+        L.device_copy(cb.ptr, ca.ptr, nb)
+        L.device_sync()
+        ca.close(); cb.close()
+        log(f"[bench] calibration copy {nb} B")
 
     # ---- reference fp64 solution (CPU baseline leg doubles as the checker) ----
     cpu = None
@@ -148,9 +174,9 @@ def main():
     per_step_fac = st["refactor_ms"] / args.steps
     per_step_tri = (st["fwd_ms"] + st["bwd_ms"]) / args.steps
     if per_step_fac >= per_step_tri:
-        dom, ach_bytes, dur = "refactor_kernel<float>", fac_b, rf_ms
+        dom, ach_bytes, dur = "lsp::refactor_kernel<float>", fac_b, rf_ms
     else:
-        dom, ach_bytes, dur = "trisolve_kernel<float> (fwd+bwd)", tri_b, tri_ms
+        dom, ach_bytes, dur = "trisolve (fwd+bwd launches)", tri_b, tri_ms
     achieved = ach_bytes / (dur * 1e-3) / 1e9 if dur > 0 else 0.0
     lat = np.array(m32["lat"])
     value = args.steps / m32["wall"]
@@ -182,11 +208,14 @@ def main():
                       "ms_per_step": round(1e3 * m64["wall"] / args.steps, 3),
                       "rel_err": m64["err"]},
         "mixed_over_fp64": round(value / (args.steps / m64["wall"]), 3),
-        "kernel_ms": {"refactor": round(rf_ms, 4), "trisolve_pair": round(tri_ms, 4),
+        "kernel_ms": {"refactor_kernel": round(rf_ms, 4),
+                      "dense_block": round(st["dense_ms"] / max(1, st["refactor_count"]), 4),
+                      "trisolve_pair": round(tri_ms, 4),
                       "trisolves_per_step": round(st["tri_count"] / args.steps, 2)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(dom),
+                     "algorithmic_bytes": int(ach_bytes), "duration_ms": round(dur, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -96,11 +96,12 @@ typedef struct {
 } lsp_circuit_info;
 
 typedef struct {
-    double refactor_ms;    /* summed device time of refactor launches */
+    double refactor_ms;    /* summed device time of the persistent refactor kernel */
     long refactor_count;
     double fwd_ms, bwd_ms; /* summed device time of forward / backward solves */
     long tri_count;        /* trisolve pairs timed */
     long graph_steps;      /* Newton steps replayed as a HIP graph */
+    double dense_ms;       /* summed device time of the dense trailing-block LU */
 } lsp_kernel_stats;
 
 typedef struct lsp_batch lsp_batch;

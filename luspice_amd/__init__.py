@@ -113,6 +113,11 @@ class DeviceArray:
         self.close()
 
 
+def device_copy(dst_ptr: int, src_ptr: int, nbytes: int):
+    """Device-to-device copy of nbytes (synchronous)."""
+    check(N.lib.lsp_dev_copy(N.C.c_void_p(dst_ptr), N.C.c_void_p(src_ptr), int(nbytes)), "dev_copy")
+
+
 def device_sync():
     check(N.lib.lsp_dev_sync(), "dev_sync")
 

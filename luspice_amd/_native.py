@@ -63,7 +63,8 @@ class CircuitInfo(C.Structure):
 
 class KernelStats(C.Structure):
     _fields_ = [("refactor_ms", C.c_double), ("refactor_count", C.c_long),
-                ("fwd_ms", C.c_double), ("bwd_ms", C.c_double), ("tri_count", C.c_long), ("graph_steps", C.c_long)]
+                ("fwd_ms", C.c_double), ("bwd_ms", C.c_double), ("tri_count", C.c_long), ("graph_steps", C.c_long),
+                ("dense_ms", C.c_double)]
 
 
 def _sig(name, res, *args):

@@ -757,13 +757,14 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
     if (rc) return rc;
     if (!capturing_) (void)hipEventRecord(ev_[0], stream_);
     rc = check(launch_refactor<T>(L, fblocks_, stream_), "refactor launch");
+    if (!capturing_) (void)hipEventRecord(ev_[1], stream_);      // the persistent kernel alone
     if (!rc && njobs_) {
         rc = check(launch_dense_lu(d_djobs_, njobs_, max_tail_T_, d_active, stream_), "dense lu");
         if (!rc)
             rc = check(launch_dense_finish<T>(d_djobs_, njobs_, max_tail_T_, fs.fdesc, d_active,
                                               opt_.refactor_ptol, stream_), "dense finish");
     }
-    if (!capturing_) (void)hipEventRecord(ev_[1], stream_);
+    if (!capturing_) (void)hipEventRecord(ev_[5], stream_);      // + dense trailing block
     return rc;
 }
 
@@ -806,8 +807,12 @@ int Batch::refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *
     if (!rc) rc = check(hipMemcpyAsync(h_pin_, d_abort_, 4, hipMemcpyDeviceToHost, stream_), "abort");
     if (!rc) rc = check(hipStreamSynchronize(stream_), "refactor sync");
     if (rc) return rc;
-    (void)hipEventElapsedTime(&t_refactor_, ev_[0], ev_[1]);
-    stats_.refactor_ms += t_refactor_;
+    float t_kernel = 0.0f, t_all = 0.0f;
+    (void)hipEventElapsedTime(&t_kernel, ev_[0], ev_[1]);
+    (void)hipEventElapsedTime(&t_all, ev_[0], ev_[5]);
+    t_refactor_ = t_all;
+    stats_.refactor_ms += t_kernel;
+    stats_.dense_ms += t_all - t_kernel;
     stats_.refactor_count++;
     const int abort_code = h_pin_[0];
     lsp_refactor_report r{};
