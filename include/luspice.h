@@ -164,6 +164,7 @@ int  lsp_batch_device_vectors(lsp_batch *b, double **rhs, double **x);
  * spmv.h:5-7); this is the capturable refactor -> trisolve -> IR sequence. */
 int  lsp_batch_graph_step(lsp_batch *b, int prec, lsp_refactor_report *rrep,
                           lsp_solve_report *srep);
+
 /* Copies total_n doubles into the resident right-hand-side arena / out of
  * the resident solution arena, ordered on the batch's stream (src / dst may
  * be host or device memory; the call returns when the copy is complete). */

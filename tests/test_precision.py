@@ -14,6 +14,8 @@ import pytest
 import luspice_amd as L
 from oracle import oracle as O
 
+FP16_SKIP = pytest.mark.skip(reason="The first FP16 solve faulted the GPU in a run that also carried a trisolve ticketing change (since removed); the cause was not shown, so the FP16 GPU path stays off the GPU until it is.")
+
 
 def _rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
@@ -56,6 +58,7 @@ def _scaled_grid(k=30, seed=11):
 
 
 @pytest.mark.gpu
+@FP16_SKIP
 def test_fp16_ruiz_scaling_one_solve():
     print("case: scaled 30x30 grid, one FP16 solve after Ruiz", flush=True)
     c = _scaled_grid()
@@ -75,6 +78,7 @@ def test_fp16_ruiz_scaling_one_solve():
 
 
 @pytest.mark.gpu
+@FP16_SKIP
 def test_fp16_ruiz_gmres_ir():
     print("case: c2 n=1500 seed 2005, FP16 factors after Ruiz", flush=True)
     c = L.mna("c2", n=1500, seed=2005)
