@@ -209,6 +209,40 @@ __device__ __forceinline__ bool wave_wait(const int *flag, int epoch, int *abort
     }
 }
 
+// Wave-collective wait for a growing prefix of lanes: lane l < nd waits on
+// its flag (null = nothing to wait for); `ok` holds the lane's seen state
+// across calls and starts as (flag == null).  Returns the length p of the
+// leading run of lanes whose flags have all been seen, once p > have, or -1
+// when the wait was abandoned.  The result is wave-uniform.
+__device__ __forceinline__ int wave_wait_prefix(const int *flag, int epoch, int *abort_word, int &ok, int have,
+                                                int nd, int *dbg = nullptr, int who2 = -1)
+{
+    int seen = epoch;
+    int backoff = 0;
+    for (int round = 0;; round++) {
+        if (!ok) {
+            seen = ld_relaxed(flag);
+            ok = seen == epoch;
+        }
+        const unsigned long long miss = ~__ballot(ok);
+        const int p = miss ? min(nd, (int)__builtin_ctzll(miss)) : nd;
+        if (p > have) return p;
+        if ((round & 15) == 15 && aborted(abort_word)) return -1;
+        if (round > kRoundLimit) {
+            if (!ok && atomicCAS(abort_word, 0, kErrSpin) == 0 && dbg) {
+                dbg[1] = seen;
+                dbg[2] = epoch;
+                dbg[3] = round;
+                dbg[4] = who2;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            }
+            return -1;
+        }
+        for (int s = 0; s <= backoff; s++) __builtin_amdgcn_s_sleep(4);
+        if (backoff < 7) backoff = 2 * backoff + 1;
+    }
+}
+
 // wave_wait over up to N flags per lane (null = nothing to wait for): each
 // round issues the outstanding polls together, so N chunks of a long
 // dependency list cost one round trip instead of N.

@@ -145,34 +145,47 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
         const int jp = __shfl_up(j, 1, kWave);
         const int ep = __shfl_up(e, 1, kWave);
         const bool head = lane < nd && (lane == 0 || j != jp + 1 || e != ep);
-        unsigned long long heads = __ballot(head);
-        if (!wave_wait(flag, epoch, abort_word, dbg, k)) return;   // aborted; kernel drains
-        if (tw) *tw = (long long)wall_clock64();
+        const unsigned long long heads = __ballot(head);
         double xc = lane < nd ? (double)x[s0 + lane] : 0.0;
-        while (heads) {
-            const int ta = __builtin_ctzll(heads);
-            heads &= heads - 1;
-            const int tb = heads ? __builtin_ctzll(heads) : nd;
-            const int bw = tb - ta;
-            const int jl = __builtin_amdgcn_readlane(e, ta) - 1;        // last column of the run
-            const int tri = jl - __builtin_amdgcn_readlane(j, tb - 1);  // run rows in later chunks
-            const int nlast = __builtin_amdgcn_readlane(nl, ta);
-            if (bw > 1) {
-                if (bw <= 9) xc = run_triangle<8>(M.val, xc, lb, ta, tb);
-                else if (bw <= 25) xc = run_triangle<24>(M.val, xc, lb, ta, tb);
-                else xc = run_triangle<kTri>(M.val, xc, lb, ta, tb);
+        // Dependencies are applied as soon as a leading group of the chunk's
+        // flags has been seen: a column whose last dependency is its
+        // predecessor in the same supernode (the critical chain) applies all
+        // the others while that one is still being factored, and then only
+        // the predecessor's column.  A run that continues past the ready
+        // prefix is cut there; its later rows stay in the rows below.
+        int ok = flag == nullptr;
+        for (int have = 0; have < nd;) {
+            const int p = wave_wait_prefix(flag, epoch, abort_word, ok, have, nd, dbg, k);
+            if (p < 0) return;                                      // aborted; kernel drains
+            if (tw) *tw = (long long)wall_clock64();
+            unsigned long long hs = (heads & ~((1ull << have) - 1)) | (1ull << have);
+            if (p < kWave) hs &= (1ull << p) - 1;
+            while (hs) {
+                const int ta = __builtin_ctzll(hs);
+                hs &= hs - 1;
+                const int tb = hs ? __builtin_ctzll(hs) : p;
+                const int bw = tb - ta;
+                const int jl = __builtin_amdgcn_readlane(e, ta) - 1;        // last column of the run
+                const int tri = jl - __builtin_amdgcn_readlane(j, tb - 1);  // run rows after this part
+                const int nlast = __builtin_amdgcn_readlane(nl, ta);
+                if (bw > 1) {
+                    if (bw <= 9) xc = run_triangle<8>(M.val, xc, lb, ta, tb);
+                    else if (bw <= 25) xc = run_triangle<24>(M.val, xc, lb, ta, tb);
+                    else xc = run_triangle<kTri>(M.val, xc, lb, ta, tb);
+                }
+                if (lane >= ta && lane < tb) x[s0 + lane] = xc;
+                // rows below this part of the run: the run's later rows, then
+                // the rows of L(:, jl) through the map
+                const int R = tri + nlast;
+                if (R > 2 * kWave && bw >= 8)
+                    run_gemv<4, 8>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
+                else
+                    run_gemv<1, kGemvBatch>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
+                if (tri == 0) u += nlast;
+                wave_fence();
+                if (lane >= tb && lane < nd) xc = x[s0 + lane];   // later lanes of this chunk
             }
-            if (lane >= ta && lane < tb) x[s0 + lane] = xc;
-            // rows below the run's part in this chunk: the run's later rows,
-            // then the rows of L(:, jl) through the map
-            const int R = tri + nlast;
-            if (R > 2 * kWave && bw >= 8)
-                run_gemv<4, 8>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
-            else
-                run_gemv<1, kGemvBatch>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
-            if (tri == 0) u += nlast;
-            wave_fence();
-            if (lane >= tb && lane < nd) xc = x[s0 + lane];   // later runs of this chunk
+            have = p;
         }
         wave_fence();
     }
