@@ -15,17 +15,17 @@
 //     by running waves: the schedule cannot deadlock whatever the occupancy;
 //   * the column's working vector x (fp64) lives in LDS when it fits
 //     (cap doubles per wave), else in a per-wave global scratch slice;
-//   * dependencies are checked 64 at a time (one lane per dependency), then
-//     applied in order; each dependency is a gather of fp32/fp64 L values and
-//     16-bit local slot indices (precomputed by the plan), so no search and no
+//   * dependencies are fetched 64 at a time (one lane per dependency) and
+//     applied in order, by supernode segment, as soon as a leading group of
+//     their flags has been seen; rows below a segment are reached through
+//     16-bit local slot indices precomputed by the plan, so no search and no
 //     atomics touch x;
-//   * the pivot test (|pivot| >= ptol * max |L(:,k)|, in fp64) uses a wave
-//     max-reduction;
-//   * dependencies are applied with a one-dependency software pipeline (the
-//     next L column's values and slots load while the current one updates x);
 //   * the finished column's L part is stored with agent-coherent stores and
 //     the column's flag is published once they are acknowledged, so no L2
-//     write-back / invalidate is needed on either side.
+//     write-back / invalidate is needed on either side;
+//   * the pivot test (|pivot| >= ptol * max |L(:,k)|, in fp64, wave
+//     max-reduction) and the stores read only after the launch (U, diagonal,
+//     row-wise copies) follow the flag.
 #include "common.hpp"
 #include "launch.hpp"
 
@@ -176,11 +176,22 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
                 if (lane >= ta && lane < tb) x[s0 + lane] = xc;
                 // rows below this part of the run: the run's later rows, then
                 // the rows of L(:, jl) through the map
+                // (about 32 loads in flight: a narrow part, such as the single
+                // column that ends a critical chain step, covers more rows
+                // per round trip)
                 const int R = tri + nlast;
-                if (R > 2 * kWave && bw >= 8)
+                if (bw >= 8) {
+                    if (R > 2 * kWave) run_gemv<4, 8>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
+                    else run_gemv<1, kGemvBatch>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
+                } else if (R <= kWave) {
+                    run_gemv<1, 8>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
+                } else if (bw == 1) {
+                    run_gemv<8, 1>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
+                } else if (bw <= 4) {
+                    run_gemv<8, 4>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
+                } else {
                     run_gemv<4, 8>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
-                else
-                    run_gemv<1, kGemvBatch>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
+                }
                 if (tri == 0) u += nlast;
                 wave_fence();
                 if (lane >= tb && lane < nd) xc = x[s0 + lane];   // later lanes of this chunk
@@ -207,7 +218,15 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
         return;
     }
 
+    // L(:, k) is read by later columns of this launch: coherent stores, then
+    // the flag once they are acknowledged.  The pivot test, U, the diagonal
+    // and the row-wise copies are needed only after the launch, so they come
+    // after the flag, off the dependency chain.
     const double piv = x[dl];
+    for (int i = dl + 1 + lane; i < m; i += kWave) st_coherent(&M.val[c0 + i], from_f64<T>(x[i] / piv));
+    store_complete();
+    if (lane == 0) st_relaxed(&M.done[k], epoch);
+
     // a stored value must be finite in the storage type (65504 for FP16)
     const double vmax = sizeof(T) == 2 ? 65504.0 : sizeof(T) == 4 ? 3.4028234663852886e38 : 1.0e308;
     double amax = 0.0;
@@ -229,9 +248,6 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
         }
         if (nonfinite) atomicAdd(&M.mstat[3], 1);
     }
-    // L(:, k) is read by later columns of this launch: coherent stores, then
-    // the flag once they are acknowledged.  U, the diagonal and the row-wise
-    // copies are read only after the launch.
     const LSP_GLOBAL int *dst = gp(M.dst);
     for (int i = lane; i < dl; i += kWave) {
         const T v = from_f64<T>(x[i]);
@@ -243,13 +259,7 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
         st_g(&M.val[c0 + dl], v);
         st_g(&M.diag[k], v);
     }
-    for (int i = dl + 1 + lane; i < m; i += kWave) {
-        const T v = from_f64<T>(x[i] / piv);
-        st_coherent(&M.val[c0 + i], v);
-        st_g(&M.lrow[dst[c0 + i]], v);
-    }
-    store_complete();
-    if (lane == 0) st_relaxed(&M.done[k], epoch);
+    for (int i = dl + 1 + lane; i < m; i += kWave) st_g(&M.lrow[dst[c0 + i]], from_f64<T>(x[i] / piv));
 }
 
 template <typename T>
