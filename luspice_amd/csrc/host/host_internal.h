@@ -47,6 +47,8 @@ void lsp_gp_solve(const LspGpLu &F, const int *q, const double *b, double *x);
 
 // ---------------------------------------------------------------------------
 // Static structure for the GPU numeric refactorization and trisolve.
+constexpr int kTriPacketBase = 1 << 30;   // trisolve task codes at or above: packets
+
 struct LspPlan {
     int n = 0;
     long nnz_a = 0;
@@ -90,8 +92,13 @@ struct LspPlan {
     // row's entries outside D, left of s forward / right of e backward, into
     // a partial sum) and one triangle task (code -(s+1)) that solves D's
     // dense diagonal block for all its rows.
+    // Row tasks of one level whose entries fit 32 lanes or fewer are packed:
+    // code kTriPacketBase + o names a packet at offset o of fpk / bpk, laid
+    // out as [g, count, row code x count]: g lanes (4, 8, 16 or 32, at least
+    // the row's entry count) per row, up to 64 / g rows.
     std::vector<int> fwd_task, bwd_task;
     std::vector<int> fwd_key, bwd_key;    // ordering keys (merge across circuits)
+    std::vector<int> fpk, bpk;            // packets (see above)
     std::vector<int> fcut;                // n: end of row i's L entries left of its supernode
     std::vector<int> bcut;                // n: start of row i's U entries right of its supernode
     std::vector<int> ru;                  // n: U(j - ru[j] .. j-1, j) is column j's run in its supernode

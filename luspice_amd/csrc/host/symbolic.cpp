@@ -351,6 +351,7 @@ int lsp_plan_build_segments(LspPlan &P)
 // ---------------------------------------------------------------------------
 static void plan_tri_supernodes(LspPlan &P)
 {
+    constexpr int kWave64 = 64;             // lanes per wave of the solve kernel
     const int n = P.n;
     const int k0 = P.tail_k0;
     std::vector<int> sst(n);
@@ -371,7 +372,7 @@ static void plan_tri_supernodes(LspPlan &P)
         P.ru[i] = r;
     }
     // F(D) = 1 + max F over the supernodes D's rows read outside D
-    auto build = [&](bool fwd, std::vector<int> &task, std::vector<int> &key, int &nlv) {
+    auto build = [&](bool fwd, std::vector<int> &task, std::vector<int> &key, std::vector<int> &pk, int &nlv) {
         std::vector<int> F(k0, 0);
         int top = 0;
         auto visit = [&](int i) {
@@ -406,13 +407,53 @@ static void plan_tri_supernodes(LspPlan &P)
         }
         std::stable_sort(items.begin(), items.end(),
                          [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first < b.first; });
-        task.resize(items.size());
-        key.resize(items.size());
-        for (size_t t = 0; t < items.size(); t++) { key[t] = items[t].first; task[t] = items[t].second; }
+        // Pack the short row tasks of each key: rows with at most g entries
+        // (g = 4, 8, 16, 32) go 64 / g to a packet, so one ticket serves up
+        // to 16 rows.  Tasks of one key are independent, so any grouping
+        // inside a key keeps the order a valid schedule.
+        auto entries = [&](int code) {
+            const int i = code >> 1;
+            const bool part = code & 1;
+            return fwd ? (part ? P.fcut[i] : P.lrp[i + 1]) - P.lrp[i]
+                       : P.urp[i + 1] - (part ? P.bcut[i] : P.urp[i]);
+        };
+        std::vector<std::pair<int, int>> out;
+        out.reserve(items.size());
+        pk.clear();
+        std::vector<int> cls[4];
+        for (size_t a = 0; a < items.size();) {
+            size_t b = a;
+            while (b < items.size() && items[b].first == items[a].first) b++;
+            for (auto &c : cls) c.clear();
+            for (size_t t = a; t < b; t++) {
+                const int code = items[t].second;
+                if (code >= 0) {
+                    const int ne = entries(code);
+                    const int c = ne <= 4 ? 0 : ne <= 8 ? 1 : ne <= 16 ? 2 : ne <= 32 ? 3 : -1;
+                    if (c >= 0) { cls[c].push_back(code); continue; }
+                }
+                out.push_back(items[t]);
+            }
+            for (int c = 0; c < 4; c++) {
+                const int g = 4 << c, per = kWave64 / g;
+                for (size_t x = 0; x < cls[c].size(); x += per) {
+                    const int cnt = (int)std::min<size_t>(per, cls[c].size() - x);
+                    if (cnt == 1) { out.push_back({items[a].first, cls[c][x]}); continue; }
+                    out.push_back({items[a].first, kTriPacketBase + (int)pk.size()});
+                    pk.push_back(g);
+                    pk.push_back(cnt);
+                    for (int r = 0; r < cnt; r++) pk.push_back(cls[c][x + r]);
+                }
+            }
+            a = b;
+        }
+        task.resize(out.size());
+        key.resize(out.size());
+        for (size_t t = 0; t < out.size(); t++) { key[t] = out[t].first; task[t] = out[t].second; }
         nlv = k0 > 0 ? top + 1 : 0;
     };
-    build(true, P.fwd_task, P.fwd_key, P.nlv_f);
-    build(false, P.bwd_task, P.bwd_key, P.nlv_b);
+    build(true, P.fwd_task, P.fwd_key, P.fpk, P.nlv_f);
+    build(false, P.bwd_task, P.bwd_key, P.bpk, P.nlv_b);
 }
 
 // ---------------------------------------------------------------------------

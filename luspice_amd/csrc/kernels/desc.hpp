@@ -16,6 +16,9 @@ enum : int {
     kErrCapacity = 2,    // a column/row exceeded its scratch capacity
 };
 
+// Trisolve task codes at or above this value name a packet of short rows.
+constexpr int kTriPacket = 1 << 30;
+
 // Numeric refactorization over the static column patterns.
 template <typename T>
 struct FactorDesc {
@@ -64,6 +67,7 @@ struct TriDesc {
     const int *ru;          // n: U(j - ru[j] .. j-1, j) is column j's run in its supernode
     const int *fcut;        // n: end of row i's L entries left of its supernode
     const int *bcut;        // n: start of row i's U entries right of its supernode
+    const int *fpk, *bpk;   // row packets forward / backward: [g, count, row codes]
     // dense trailing block: rows [tail_k0, n) are solved with tail_F
     const T *tail_F;        // tail_T^2, column-major, unit-L below / U on and above
     const int *tail_lcut;   // tail_T: end of the sparse part of L row tail_k0 + r

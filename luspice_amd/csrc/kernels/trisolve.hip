@@ -131,8 +131,63 @@ __device__ bool tri_backward(const TriDesc<T> &M, int s, int e, double *z, doubl
     return true;
 }
 
+// Packet task (code kTriPacket + o): up to 64 / g row tasks of one level,
+// each with at most g entries (g = 4, 8, 16 or 32); lanes [r g, (r + 1) g)
+// gather row r.  The rows' flags are polled together, each lane group sums
+// with g-lane shuffles, and every row is written as its own task would be.
+template <typename T, bool kForward>
+__device__ void tri_packet(const TriLaunch &L, const TriDesc<T> &M, int o, long vo, int *flags, int epoch)
+{
+    const int lane = lane_id();
+    const LSP_GLOBAL int *pk = gp(kForward ? M.fpk : M.bpk) + o;
+    const int g = uniform(pk[0]);
+    const int cnt = uniform(pk[1]);
+    const int grp = lane >> __builtin_ctz(g);
+    const int l = lane & (g - 1);
+    const int rc = grp < cnt ? pk[2 + grp] : -1;
+    const int i = rc >> 1;
+    const bool part = rc & 1;
+    int e0 = 0, e1 = 0;
+    if (rc >= 0) {
+        if (kForward) {
+            e0 = M.lrp[i];
+            e1 = part ? M.fcut[i] : M.lrp[i + 1];
+        } else {
+            e0 = part ? M.bcut[i] : M.urp[i];
+            e1 = M.urp[i + 1];
+        }
+    }
+    const int e = e0 + l;
+    const bool on = e < e1;
+    const int jj = on ? (kForward ? M.lci : M.uci)[e] : 0;
+    const T vv = on ? (kForward ? M.lrow : M.urow)[e] : T(0);
+    if (!wave_wait(on ? &flags[jj] : nullptr, epoch, L.abort_word)) return;
+    double *y = L.y + vo;
+    double *z = L.z + vo;
+    double acc = on ? to_f64(vv) * ld_coherent_val(&(kForward ? y : z)[jj]) : 0.0;
+    for (int off = g >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+    const bool lead = rc >= 0 && l == 0;
+    if (lead) {
+        if (kForward) {
+            const int ro = M.rowat[i];
+            const double bi = L.rs ? L.b[vo + ro] * L.rs[vo + ro] : L.b[vo + ro];
+            st_coherent(&y[i], bi - acc);
+        } else if (part) {
+            st_coherent(&z[i], y[i] - acc);
+        } else {
+            const double zi = (y[i] - acc) / to_f64(M.diag[i]);
+            st_coherent(&z[i], zi);
+            const int qo = M.q[i];
+            L.x[vo + qo] = L.cs ? zi * L.cs[vo + qo] : zi;
+        }
+    }
+    store_complete();
+    if (lead) st_relaxed(&flags[i], part ? -epoch : epoch);
+}
+
 // Task codes (see the plan): 2i = row i solved whole, 2i+1 = row i's entries
-// outside its supernode into a partial sum, -(s+1) = supernode triangle.
+// outside its supernode into a partial sum, -(s+1) = supernode triangle,
+// kTriPacket + o = packet of short rows.
 template <typename T, bool kForward>
 __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
 {
@@ -154,6 +209,10 @@ __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
         double *z = L.z + vo;
         int *flags = kForward ? M.fdone : M.bdone;
         const int code = tk.y;
+        if (code >= kTriPacket) {
+            tri_packet<T, kForward>(L, M, code - kTriPacket, vo, flags, epoch);
+            continue;
+        }
         if (code < 0) {
             const int s0 = -code - 1;
             const int e0 = M.snend[s0];

@@ -26,6 +26,8 @@
 
 namespace lsp {
 
+static_assert(kTriPacket == ::kTriPacketBase, "packet codes of the plan and the solve kernel");
+
 
 Batch::~Batch()
 {
@@ -278,6 +280,8 @@ int Batch::ensure_factor_set(FactorSet<T> &fs)
         D.ru = d_ru_ + C.o_n;
         D.fcut = d_fcut_ + C.o_n;
         D.bcut = d_bcut_ + C.o_n;
+        D.fpk = d_fpk_ + C.o_fpk;
+        D.bpk = d_bpk_ + C.o_bpk;
         D.tail_lcut = d_tail_lcut_ + C.o_t;
         D.tail_flag = d_tail_flag_ + C.o_tb;
         D.tail_ctr = d_tail_ctr_ + 4 * m;
@@ -347,7 +351,7 @@ size_t Batch::estimate_device_bytes() const
     b += 12 * n + 8 * nnz + 12 * sell;                                    // flags, values
     b += 16 * nnz + 24 * n + 16 * (nc + 1);                               // Ruiz: row/col, scaled values, scales
     b += 8 * T2 + 16 * T + 65536.0 * nc;                                   // dense block work
-    b += 24 * n + 8 * slices;                                             // task lists
+    b += 24 * n + 8 * slices + 16 * n;                                    // task lists, row packets
     b += 64 * n + 8 * (m + 1) * n;                                        // vectors, Krylov basis
     b += (4 + 8) * (slots + nlu + n + T2);                                 // fp32 + fp64 factor sets
     if (maxcol > std::max(1, opt_.lds_cap)) b += 256.0 * 8 * 4 * maxcol * 8;   // global column scratch
@@ -379,8 +383,10 @@ int Batch::upload_all()
 
     // ---- concatenate structures ----
     long o_np1 = 0, o_n = 0, o_slot = 0, o_a = 0, o_rel = 0, o_l = 0, o_u = 0, o_sl = 0,
-         o_slp1 = 0, o_ss = 0, o_t = 0, o_t2 = 0, o_tb = 0;
+         o_slp1 = 0, o_ss = 0, o_t = 0, o_t2 = 0, o_tb = 0, o_fpk = 0, o_bpk = 0;
     for (auto &C : circ_) {
+        C.o_fpk = o_fpk; C.o_bpk = o_bpk;
+        o_fpk += (long)C.plan.fpk.size(); o_bpk += (long)C.plan.bpk.size();
         C.o_np1 = o_np1; C.o_n = o_n; C.o_slot = o_slot; C.o_a = o_a; C.o_rel = o_rel;
         C.o_l = o_l; C.o_u = o_u; C.o_sl = o_sl; C.o_slp1 = o_slp1; C.o_ss = o_ss;
         C.o_t = o_t; C.o_t2 = o_t2; C.o_tb = o_tb;
@@ -397,7 +403,7 @@ int Batch::upload_all()
     nt_tot_ = o_t; nt2_tot_ = o_t2; ntb_tot_ = o_tb;
     {
         std::vector<int64_t> cp, relp, soff;
-        std::vector<int> dp, pos, aptr, asrc, aslot, dst, lrp, lci, urp, uci, rowat, q, snend, ru, fcut, bcut;
+        std::vector<int> dp, pos, aptr, asrc, aslot, dst, lrp, lci, urp, uci, rowat, q, snend, ru, fcut, bcut, fpk, bpk;
         std::vector<int> srlen, scol, sperm, ssrc;
         std::vector<uint16_t> rel;
         cp.reserve(o_np1); relp.reserve(o_np1); rel.reserve(o_rel);
@@ -421,6 +427,8 @@ int Batch::upload_all()
             ru.insert(ru.end(), P.ru.begin(), P.ru.end());
             fcut.insert(fcut.end(), P.fcut.begin(), P.fcut.end());
             bcut.insert(bcut.end(), P.bcut.begin(), P.bcut.end());
+            fpk.insert(fpk.end(), P.fpk.begin(), P.fpk.end());
+            bpk.insert(bpk.end(), P.bpk.begin(), P.bpk.end());
             rel.insert(rel.end(), P.rel.begin(), P.rel.end());
             const LspSell &S = C.sell;
             soff.insert(soff.end(), S.off.begin(), S.off.end());
@@ -443,11 +451,12 @@ int Batch::upload_all()
         d_lrp_ = upload(lrp); d_lci_ = upload(lci); d_urp_ = upload(urp); d_uci_ = upload(uci);
         d_rowat_ = upload(rowat); d_q_ = upload(q); d_rel_ = upload(rel); d_snend_ = upload(snend);
         d_ru_ = upload(ru); d_fcut_ = upload(fcut); d_bcut_ = upload(bcut);
+        d_fpk_ = upload(fpk); d_bpk_ = upload(bpk);
         d_srlen_ = upload(srlen); d_scol_ = upload(scol); d_sperm_ = upload(sperm);
         d_ssrc_ = upload(ssrc);
         if (!d_cp_ || !d_relp_ || !d_soff_ || !d_dp_ || !d_pos_ || !d_aptr_ || !d_asrc_ ||
             !d_aslot_ || !d_dst_ || !d_lrp_ || !d_lci_ || !d_urp_ || !d_uci_ || !d_rowat_ ||
-            !d_q_ || !d_rel_ || !d_snend_ || !d_ru_ || !d_fcut_ || !d_bcut_ || !d_srlen_ || !d_scol_ || !d_sperm_ || !d_ssrc_)
+            !d_q_ || !d_rel_ || !d_snend_ || !d_ru_ || !d_fcut_ || !d_bcut_ || !d_fpk_ || !d_bpk_ || !d_srlen_ || !d_scol_ || !d_sperm_ || !d_ssrc_)
             return LSP_E_ALLOC;
     }
     // flags start at 0; epochs start at 1

@@ -38,6 +38,7 @@ struct CircuitHost {
     long o_l = 0, o_u = 0;               // strict L / U entry arrays
     long o_sl = 0, o_slp1 = 0, o_ss = 0; // SELL slices, slices+1, slots
     long o_t = 0, o_t2 = 0, o_tb = 0;    // dense block: T, T^2, 64-row blocks
+    long o_fpk = 0, o_bpk = 0;           // trisolve packets forward / backward
 };
 
 template <typename T>
@@ -123,6 +124,7 @@ private:
         *d_scol_ = nullptr, *d_sperm_ = nullptr, *d_ssrc_ = nullptr;
     uint16_t *d_rel_ = nullptr;
     int *d_snend_ = nullptr, *d_ru_ = nullptr, *d_fcut_ = nullptr, *d_bcut_ = nullptr;
+    int *d_fpk_ = nullptr, *d_bpk_ = nullptr;
     double *d_tailW_ = nullptr;
     // HIP graph of one Newton step (graph_step)
     hipGraph_t graph_ = nullptr;
