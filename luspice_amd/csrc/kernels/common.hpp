@@ -182,7 +182,7 @@ __device__ __forceinline__ void wave_sync()
 constexpr int kRoundLimit = 1 << 21;
 
 __device__ __forceinline__ bool wave_wait(const int *flag, int epoch, int *abort_word,
-                                          int *dbg = nullptr, int who2 = -1)
+                                          int *dbg = nullptr, int who2 = -1, int max_backoff = 7)
 {
     int seen = flag ? -1 : epoch;
     int ok = flag == nullptr;
@@ -205,7 +205,7 @@ __device__ __forceinline__ bool wave_wait(const int *flag, int epoch, int *abort
             return false;
         }
         for (int s = 0; s <= backoff; s++) __builtin_amdgcn_s_sleep(4);
-        if (backoff < 7) backoff = 2 * backoff + 1;
+        if (backoff < max_backoff) backoff = 2 * backoff + 1;
     }
 }
 

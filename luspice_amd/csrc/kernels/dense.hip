@@ -505,10 +505,15 @@ __global__ void __launch_bounds__(256) tail_fwd_gather_kernel(const void *mats, 
 // Dense triangular solve of the block rows (see the file comment).
 //   lower: y_T <- L22^-1 y_T (unit diagonal), in place
 //   upper: z_T <- U22^-1 y_T, x[q] = z, bdone[row] = epoch of the sparse solve
-// Per contributing block the 64 x 64 factor slab (one row per lane, in
-// registers) is loaded before its flag is awaited, so the wait overlaps the
-// loads; the solved values arrive with one coherent load and are broadcast by
-// readlane.  The diagonal block is loaded in bulk before the 64-step solve.
+// Waves 1..3 accumulate the contributions of the blocks solved before this
+// one: per block the 64 x 64 factor slab (one row per lane, in registers) is
+// loaded before its flag is awaited, so the wait overlaps the loads, and the
+// solved values arrive with one coherent load and are broadcast by readlane.
+// Meanwhile wave 0 holds the diagonal block (and, upper, the reciprocals of
+// its diagonal), so after the last contribution only the 64-step solve and
+// the stores remain on the chain from block to block.
+constexpr int kTailBackoff = 1;    // few waiters: poll every ~0.1-0.2 us
+
 template <typename T, bool kLower>
 __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const int *tails,
                                                         const long *voff, const int *active,
@@ -540,33 +545,42 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
             const int rr = r0 + min(lane, bi - 1);               // clamped row of this lane
             double acc = 0.0;
             bool ok = true;
-            for (int s = wv; s < t && ok; s += 4) {
-                const int jb = kLower ? s : nblk - 1 - s;
-                const int c0 = jb * kNb;
-                const int bj = min(kNb, M.tail_T - c0);
-                T fv[kNb];
+            T dg[kNb];                                           // wave 0: row lane of the diagonal block
+            double rd = 1.0;                                     // wave 0, upper: 1 / U(rr, rr)
+            double yb = 0.0;                                     // wave 0: right-hand side of row rr
+            int qo = 0;                                          // wave 0, upper: output index of row rr
+            if (wv == 0) {
 #pragma unroll
-                for (int c = 0; c < kNb; c++) fv[c] = M.tail_F[(size_t)(c0 + min(c, bj - 1)) * ld + rr];
-                ok = wave_wait(lane == 0 ? &M.tail_flag[jb] : nullptr, epoch, abort_word);
-                if (!ok) break;
-                const double sv = lane < bj ? ld_coherent_val(&src[vo + k0 + c0 + lane]) : 0.0;
+                for (int c = 0; c < kNb; c++) dg[c] = M.tail_F[(size_t)(r0 + min(c, bi - 1)) * ld + rr];
+                if (!kLower) {
+                    rd = 1.0 / to_f64(M.tail_F[(size_t)rr * ld + rr]);
+                    qo = M.q[k0 + rr];
+                }
+                yb = y[vo + k0 + rr];
+            } else {
+                for (int s = wv - 1; s < t && ok; s += 3) {
+                    const int jb = kLower ? s : nblk - 1 - s;
+                    const int c0 = jb * kNb;
+                    const int bj = min(kNb, M.tail_T - c0);
+                    T fv[kNb];
 #pragma unroll
-                for (int c = 0; c < kNb; c++) {
-                    const double yc = bcast(sv, c);              // zero beyond bj
-                    acc += to_f64(fv[c]) * yc;
+                    for (int c = 0; c < kNb; c++) fv[c] = M.tail_F[(size_t)(c0 + min(c, bj - 1)) * ld + rr];
+                    ok = wave_wait(lane == 0 ? &M.tail_flag[jb] : nullptr, epoch, abort_word, nullptr, -1,
+                                   kTailBackoff);
+                    if (!ok) break;
+                    const double sv = lane < bj ? ld_coherent_val(&src[vo + k0 + c0 + lane]) : 0.0;
+#pragma unroll
+                    for (int c = 0; c < kNb; c++) {
+                        const double yc = bcast(sv, c);              // zero beyond bj
+                        acc += to_f64(fv[c]) * yc;
+                    }
                 }
             }
             red[wv][lane] = lane < bi ? acc : 0.0;
-            __syncthreads();
             const bool all_ok = __syncthreads_and(ok);
             if (all_ok && wv == 0) {
-                // row lane of the diagonal block: dg[c] = F(r0 + lane, r0 + c)
-                T dg[kNb];
-#pragma unroll
-                for (int c = 0; c < kNb; c++) dg[c] = M.tail_F[(size_t)(r0 + min(c, bi - 1)) * ld + rr];
                 double tr = 0.0;
-                if (lane < bi)
-                    tr = y[vo + k0 + r0 + lane] - (red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane]);
+                if (lane < bi) tr = yb - (red[1][lane] + red[2][lane] + red[3][lane]);
                 if (kLower) {
 #pragma unroll
                     for (int c = 0; c < kNb; c++) {
@@ -578,15 +592,13 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
 #pragma unroll
                     for (int c = kNb - 1; c >= 0; c--) {
                         if (c < bi) {
-                            const double zc = bcast(tr, c) / bcast(to_f64(dg[c]), c);
+                            const double zc = bcast(tr, c) * bcast(rd, c);
                             if (lane == c) tr = zc;
                             if (lane < c) tr -= to_f64(dg[c]) * zc;
                         }
                     }
                     if (lane < bi) {
-                        const int row = k0 + r0 + lane;
-                        st_coherent(&z[vo + row], tr);
-                        const int qo = M.q[row];
+                        st_coherent(&z[vo + k0 + rr], tr);
                         x[vo + qo] = cs ? tr * cs[vo + qo] : tr;
                     }
                 }

@@ -72,10 +72,11 @@ void Batch::dfree(void *p)
 int Batch::alloc_krylov(int mm)
 {
     const int nc = (int)circ_.size();
-    for (void *p : {(void *)d_V_, (void *)d_partial_, (void *)d_H_, (void *)d_cs_, (void *)d_sn_,
+    for (void *p : {(void *)d_V_, (void *)d_Z_, (void *)d_partial_, (void *)d_H_, (void *)d_cs_, (void *)d_sn_,
                     (void *)d_g_, (void *)d_yv_, (void *)d_hcol_})
         dfree(p);
     d_V_ = (double *)dalloc((size_t)(mm + 1) * ntot_ * 8);
+    d_Z_ = (double *)dalloc((size_t)mm * ntot_ * 8);
     d_partial_ = (double *)dalloc((size_t)nchunks_ * (mm + 1) * 8);
     d_H_ = (double *)dalloc((size_t)nc * (mm + 1) * mm * 8);
     d_cs_ = (double *)dalloc((size_t)nc * mm * 8);
@@ -83,7 +84,7 @@ int Batch::alloc_krylov(int mm)
     d_g_ = (double *)dalloc((size_t)nc * (mm + 1) * 8);
     d_yv_ = (double *)dalloc((size_t)nc * mm * 8);
     d_hcol_ = (double *)dalloc((size_t)nc * (mm + 1) * 8);
-    if (!d_V_ || !d_partial_ || !d_H_ || !d_cs_ || !d_sn_ || !d_g_ || !d_yv_ || !d_hcol_)
+    if (!d_V_ || !d_Z_ || !d_partial_ || !d_H_ || !d_cs_ || !d_sn_ || !d_g_ || !d_yv_ || !d_hcol_)
         return LSP_E_ALLOC;
     mm_alloc_ = mm;
     return LSP_OK;
@@ -352,7 +353,7 @@ size_t Batch::estimate_device_bytes() const
     b += 16 * nnz + 24 * n + 16 * (nc + 1);                               // Ruiz: row/col, scaled values, scales
     b += 8 * T2 + 16 * T + 65536.0 * nc;                                   // dense block work
     b += 24 * n + 8 * slices + 16 * n;                                    // task lists, row packets
-    b += 64 * n + 8 * (m + 1) * n;                                        // vectors, Krylov basis
+    b += 64 * n + 8 * (2 * m + 1) * n;                                    // vectors, Krylov bases V and Z
     b += (4 + 8) * (slots + nlu + n + T2);                                 // fp32 + fp64 factor sets
     if (maxcol > std::max(1, opt_.lds_cap)) b += 256.0 * 8 * 4 * maxcol * 8;   // global column scratch
     return (size_t)(1.05 * b) + (1u << 20);
@@ -1018,10 +1019,13 @@ int Batch::gmres_ir(const double *d_bv, double *d_xv, int prec, const std::vecto
         if (h_pin_[0] == 0) break;
         outer_used++;
         LSP_HIP(k_chunk_scale(K, d_r_, d_V_, d_beta_, s), "scale");
+        // Z_j = M^-1 V_j is kept, so the update x += M^-1 (V y) = Z y needs
+        // no further solve (the flexible-GMRES form of right preconditioning)
         for (int j = 0; j < mm; j++) {
             double *Vj = d_V_ + (long)j * ntot_;
-            LSP_TRY(launch_trisolve_prec(prec, Vj, d_z_, d_act_));
-            LSP_TRY(launch_spmv64(d_z_, d_w_, nullptr, d_act_));
+            double *Zj = d_Z_ + (long)j * ntot_;
+            LSP_TRY(launch_trisolve_prec(prec, Vj, Zj, d_act_));
+            LSP_TRY(launch_spmv64(Zj, d_w_, nullptr, d_act_));
             for (int pass = 0; pass < 2; pass++) {
                 LSP_HIP(k_chunk_dots(K, d_V_, d_w_, j + 1, s), "dots");
                 LSP_HIP(k_reduce(K, j + 1, 1, j, nullptr, s), "reduce");
@@ -1037,9 +1041,8 @@ int Batch::gmres_ir(const double *d_bv, double *d_xv, int prec, const std::vecto
             if (h_pin_[0] == 0) break;
         }
         LSP_HIP(k_backsolve(K, s), "backsolve");
-        LSP_HIP(k_chunk_combine(K, d_V_, d_t_, s), "combine");
-        LSP_TRY(launch_trisolve_prec(prec, d_t_, d_z_, d_act_));
-        LSP_HIP(k_chunk_axpy(K, d_xv, d_z_, s), "axpy");
+        LSP_HIP(k_chunk_combine(K, d_Z_, d_t_, s), "combine");
+        LSP_HIP(k_chunk_axpy(K, d_xv, d_t_, s), "axpy");
     }
     // final true residual of every circuit in the subset
     {
@@ -1252,8 +1255,9 @@ int Batch::record_step(int prec)
     const int mg = std::min(mm, kGraphInner);
     for (int j = 0; j < mg; j++) {
         double *Vj = d_V_ + (long)j * ntot_;
-        LSP_TRY(launch_trisolve_prec(prec, Vj, d_z_, d_act_, d_nact_));
-        LSP_TRY(launch_spmv64(d_z_, d_w_, nullptr, d_act_));
+        double *Zj = d_Z_ + (long)j * ntot_;
+        LSP_TRY(launch_trisolve_prec(prec, Vj, Zj, d_act_, d_nact_));
+        LSP_TRY(launch_spmv64(Zj, d_w_, nullptr, d_act_));
         for (int pass = 0; pass < 2; pass++) {
             LSP_HIP(k_chunk_dots(K, d_V_, d_w_, j + 1, s), "dots");
             LSP_HIP(k_reduce(K, j + 1, 1, j, nullptr, s), "reduce");
@@ -1266,9 +1270,8 @@ int Batch::record_step(int prec)
         LSP_HIP(k_givens(K, j, d_nact_, s), "givens");
     }
     LSP_HIP(k_backsolve(K, s), "backsolve");
-    LSP_HIP(k_chunk_combine(K, d_V_, d_t_, s), "combine");
-    LSP_TRY(launch_trisolve_prec(prec, d_t_, d_z_, d_act_));
-    LSP_HIP(k_chunk_axpy(K, d_xv, d_z_, s), "axpy");
+    LSP_HIP(k_chunk_combine(K, d_Z_, d_t_, s), "combine");
+    LSP_HIP(k_chunk_axpy(K, d_xv, d_t_, s), "axpy");
     LSP_HIP(k_set_all_active(K, 1, s), "act");
     LSP_TRY(launch_spmv64(d_xv, d_r_, d_bv, d_act_));
     LSP_HIP(k_chunk_dots(K, d_r_, d_r_, 1, s), "dots");

@@ -168,7 +168,8 @@ private:
     FactorSet<__half> f16_;
     // vectors and Krylov state
     double *d_b_ = nullptr, *d_x_ = nullptr, *d_r_ = nullptr, *d_w_ = nullptr,
-           *d_z_ = nullptr, *d_t_ = nullptr, *d_ty_ = nullptr, *d_tz_ = nullptr, *d_V_ = nullptr;
+           *d_z_ = nullptr, *d_t_ = nullptr, *d_ty_ = nullptr, *d_tz_ = nullptr, *d_V_ = nullptr,
+           *d_Z_ = nullptr;
     long *d_voff_ = nullptr;
     int4 *d_chunks_ = nullptr;
     int *d_cbeg_ = nullptr;
