@@ -61,7 +61,7 @@ typedef struct {
     int refactor_blocks;   /* persistent refactor grid (0 = auto) */
     int trisolve_blocks;   /* persistent trisolve grid (0 = auto) */
     int tail_max;          /* largest dense trailing block (0 = 8192, <0 = none) */
-    double tail_density;   /* min pattern density of that block (0 = 0.3) */
+    double tail_density;   /* min pattern density of that block (0 = 0.5) */
     int ruiz_sweeps;       /* Ruiz sweeps ahead of FP16 factor storage (0 = 30) */
     int pivot_scaled;      /* 1: choose the pivot sequence on the Ruiz-scaled matrix */
 } lsp_options;

@@ -48,6 +48,7 @@ void lsp_gp_solve(const LspGpLu &F, const int *q, const double *b, double *x);
 // ---------------------------------------------------------------------------
 // Static structure for the GPU numeric refactorization and trisolve.
 constexpr int kTriPacketBase = 1 << 30;   // trisolve task codes at or above: packets
+constexpr int kTriPacketPerLane = 4;      // entries per lane of a packet row
 
 struct LspPlan {
     int n = 0;
@@ -92,10 +93,11 @@ struct LspPlan {
     // row's entries outside D, left of s forward / right of e backward, into
     // a partial sum) and one triangle task (code -(s+1)) that solves D's
     // dense diagonal block for all its rows.
-    // Row tasks of one level whose entries fit 32 lanes or fewer are packed:
-    // code kTriPacketBase + o names a packet at offset o of fpk / bpk, laid
-    // out as [g, count, row code x count]: g lanes (4, 8, 16 or 32, at least
-    // the row's entry count) per row, up to 64 / g rows.
+    // Row tasks of one level with at most 32 kTriPacketPerLane entries are
+    // packed: code kTriPacketBase + o names a packet at offset o of fpk /
+    // bpk, laid out as [g, count, row code x count]: g lanes per row (4, 8,
+    // 16 or 32, with g kTriPacketPerLane >= the row's entry count), up to
+    // 64 / g rows.
     std::vector<int> fwd_task, bwd_task;
     std::vector<int> fwd_key, bwd_key;    // ordering keys (merge across circuits)
     std::vector<int> fpk, bpk;            // packets (see above)

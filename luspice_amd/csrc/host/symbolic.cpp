@@ -407,10 +407,11 @@ static void plan_tri_supernodes(LspPlan &P)
         }
         std::stable_sort(items.begin(), items.end(),
                          [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first < b.first; });
-        // Pack the short row tasks of each key: rows with at most g entries
-        // (g = 4, 8, 16, 32) go 64 / g to a packet, so one ticket serves up
-        // to 16 rows.  Tasks of one key are independent, so any grouping
-        // inside a key keeps the order a valid schedule.
+        // Pack the short row tasks of each key: rows with at most
+        // g kTriPacketPerLane entries (g = 4, 8, 16, 32 lanes) go 64 / g to a
+        // packet, so one ticket serves up to 16 rows.  Tasks of one key are
+        // independent, so any grouping inside a key keeps the order a valid
+        // schedule.
         auto entries = [&](int code) {
             const int i = code >> 1;
             const bool part = code & 1;
@@ -429,7 +430,8 @@ static void plan_tri_supernodes(LspPlan &P)
                 const int code = items[t].second;
                 if (code >= 0) {
                     const int ne = entries(code);
-                    const int c = ne <= 4 ? 0 : ne <= 8 ? 1 : ne <= 16 ? 2 : ne <= 32 ? 3 : -1;
+                    const int k = kTriPacketPerLane;
+                    const int c = ne <= 4 * k ? 0 : ne <= 8 * k ? 1 : ne <= 16 * k ? 2 : ne <= 32 * k ? 3 : -1;
                     if (c >= 0) { cls[c].push_back(code); continue; }
                 }
                 out.push_back(items[t]);

@@ -150,6 +150,23 @@ template <> __device__ __forceinline__ __half ld_coherent_val<__half>(const __ha
         __hip_atomic_load(gp((const unsigned short *)p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
+// Element i of a wave-uniform base: agent-coherent (data written during this
+// launch) or plain (data written by an earlier launch, which may be cached).
+template <typename T> __device__ __forceinline__ T ld_plain(const LSP_GLOBAL T *base, uint32_t i)
+{
+    return base[i];
+}
+template <> __device__ __forceinline__ __half ld_plain<__half>(const LSP_GLOBAL __half *base, uint32_t i)
+{
+    return __ushort_as_half(((const LSP_GLOBAL unsigned short *)base)[i]);
+}
+template <bool kCoherent, typename T>
+__device__ __forceinline__ T ld_val(const LSP_GLOBAL T *base, uint32_t i)
+{
+    if constexpr (kCoherent) return ld_coherent_val(base, i);
+    else return ld_plain(base, i);
+}
+
 // Orders this wave's accesses in the compiler only: the memory operations of
 // one wave complete in order, so LDS (and same-address global) traffic between
 // lanes of a wave needs no wait.
@@ -320,7 +337,7 @@ constexpr int kTri = kWave - 1;         // triangle steps held in registers (64-
 // every later lane t of the run.  The NT triangle columns are all loaded
 // before the first step (columns past the run load a valid duplicate and are
 // masked), so the chain waits on registers only.
-template <int NT, typename T>
+template <int NT, bool kCoherent = true, typename T>
 __device__ __forceinline__ double run_triangle(const T *val, double xc, int lb, int ta, int tb)
 {
     const int lane = lane_id();
@@ -332,7 +349,7 @@ __device__ __forceinline__ double run_triangle(const T *val, double xc, int lb, 
         const int tp = ta + qq;
         const int base = __builtin_amdgcn_readlane(lb, tp);
         const int off = min(max(lane - tp - 1, 0), bw - qq - 2);
-        tv[q] = ld_coherent_val(gp(val) + (uint32_t)base, (uint32_t)off);
+        tv[q] = ld_val<kCoherent>(gp(val) + (uint32_t)base, (uint32_t)off);
     }
 #pragma unroll
     for (int q = 0; q < NT; q++) {
@@ -349,7 +366,7 @@ __device__ __forceinline__ double run_triangle(const T *val, double xc, int lb, 
 // rd_t = 1 / U(j_t, j_t).  Steps go from the last column to the first: lane
 // q' finalises z_q' = num_q' rd_q' and every earlier lane subtracts
 // U(j_t, j_q') z_q'.  Returns z_t.
-template <int NT, typename T>
+template <int NT, bool kCoherent = true, typename T>
 __device__ __forceinline__ double run_triangle_upper(const T *val, double num, double rd, int dpc, int ru, int bw)
 {
     const int lane = lane_id();
@@ -359,7 +376,7 @@ __device__ __forceinline__ double run_triangle_upper(const T *val, double num, d
         const int qq = min(q, bw - 1);
         const int base = __builtin_amdgcn_readlane(dpc, qq);
         const int off = min(max(qq - lane, 0), qq);
-        tv[q] = ld_coherent_val(gp(val) + (uint32_t)(base - off), 0u);
+        tv[q] = ld_val<kCoherent>(gp(val) + (uint32_t)(base - off), 0u);
     }
 #pragma unroll
     for (int q = NT - 1; q >= 0; q--) {

@@ -16,8 +16,10 @@ enum : int {
     kErrCapacity = 2,    // a column/row exceeded its scratch capacity
 };
 
-// Trisolve task codes at or above this value name a packet of short rows.
+// Trisolve task codes at or above this value name a packet of short rows;
+// each lane of a packet row gathers up to kTriPacketLane of its entries.
 constexpr int kTriPacket = 1 << 30;
+constexpr int kTriPacketLane = 4;
 
 // Numeric refactorization over the static column patterns.
 template <typename T>

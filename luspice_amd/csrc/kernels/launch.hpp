@@ -37,6 +37,7 @@ struct TriLaunch {
     double *y, *z;          // work vectors (position order)
     const double *rs, *cs;  // optional row / column scales (Ruiz): b_s = R b, x = C z
     const int *n_active;    // optional: when *n_active == 0 no circuit is active, tasks are skipped
+    long long *timeline;    // optional diagnostic, 2 per task: ticket granted, task finished
     int ntask;
     int *epoch;             // device epoch counter (see FactorLaunch)
     int *ctr;

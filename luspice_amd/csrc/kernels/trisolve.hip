@@ -28,8 +28,9 @@
 namespace lsp {
 
 
-constexpr int kSolveBatch = 16;         // dense-part values in flight per pass
 constexpr int kGather = 4;              // 64-entry chunks of a row per wait round
+
+constexpr int kSolveBatch = 32;         // dense-part values in flight per pass of a triangle task
 
 // Forward triangle task of supernode D = [s, e): its rows hold partial sums
 // p_i = b_i - sum over L entries left of s (flag -epoch); y_D = L_DD^-1 p in
@@ -48,26 +49,31 @@ __device__ bool tri_forward(const TriDesc<T> &M, int s, int e, double *y, int *f
         if (!wave_wait(lane < bw ? &flags[row] : nullptr, -epoch, abort_word)) return false;
         const int rr = s + b0 + min(lane, bw - 1);
         double p = lane < bw ? ld_coherent_val(&y[row]) : 0.0;
+        // earlier blocks, 64 columns at a time: lane q holds y of column q of
+        // the chunk (one load), broadcast by readlane
+        constexpr int SB = kSolveBatch;
         double acc = 0.0;
-        for (int q0 = 0; q0 < b0; q0 += kSolveBatch) {
-            T v[kSolveBatch];
-            double yv[kSolveBatch];
+        for (int q0 = 0; q0 < b0; q0 += kWave) {
+            const int qn = min(kWave, b0 - q0);
+            const double yq = ld_coherent_val(&y[s + q0 + min(lane, qn - 1)]);
+            for (int h0 = 0; h0 < qn; h0 += SB) {
+                T v[SB];
 #pragma unroll
-            for (int h = 0; h < kSolveBatch; h++) {
-                const int jq = s + min(q0 + h, b0 - 1);
-                v[h] = ld_coherent_val(gp(M.val) + (uint32_t)(dp[jq] + (rr - jq)), 0u);
-                yv[h] = ld_coherent_val(&y[jq]);
+                for (int h = 0; h < SB; h++) {
+                    const int jq = s + q0 + min(h0 + h, qn - 1);
+                    v[h] = ld_plain(gp(M.val), (uint32_t)(dp[jq] + (rr - jq)));   // written by the refactor
+                }
+#pragma unroll
+                for (int h = 0; h < SB; h++)
+                    if (h0 + h < qn) acc = fma(to_f64(v[h]), bcast_f64(yq, h0 + h), acc);
             }
-#pragma unroll
-            for (int h = 0; h < kSolveBatch; h++)
-                if (q0 + h < b0) acc = fma(to_f64(v[h]), yv[h], acc);
         }
         p -= acc;
         const int lb = dp[rr] + 1;
         if (bw == 1) {
-        } else if (bw <= 9) p = run_triangle<8>(M.val, p, lb, 0, bw);
-        else if (bw <= 25) p = run_triangle<24>(M.val, p, lb, 0, bw);
-        else p = run_triangle<kTri>(M.val, p, lb, 0, bw);
+        } else if (bw <= 9) p = run_triangle<8, false>(M.val, p, lb, 0, bw);
+        else if (bw <= 25) p = run_triangle<24, false>(M.val, p, lb, 0, bw);
+        else p = run_triangle<kTri, false>(M.val, p, lb, 0, bw);
         if (lane < bw) st_coherent(&y[row], p);
         store_complete();
         if (lane < bw) st_relaxed(&flags[row], epoch);
@@ -94,32 +100,39 @@ __device__ bool tri_backward(const TriDesc<T> &M, int s, int e, double *z, doubl
         if (!wave_wait(lane < bw ? &flags[row] : nullptr, -epoch, abort_word)) return false;
         const int rr = base + min(lane, bw - 1);
         double num = lane < bw ? ld_coherent_val(&z[row]) : 0.0;
-        const int cnt = e - (base + bw);                 // rows of later blocks
+        // later blocks, 64 columns at a time: lane q holds z of column q of
+        // the chunk (one load), broadcast by readlane
+        constexpr int SB = kSolveBatch;
+        const int c0 = base + bw;                        // first column of the later blocks
+        const int cnt = e - c0;
         double acc = 0.0;
-        for (int q0 = 0; q0 < cnt; q0 += kSolveBatch) {
-            T v[kSolveBatch];
-            double zv[kSolveBatch];
-            int in[kSolveBatch];
+        for (int q0 = 0; q0 < cnt; q0 += kWave) {
+            const int qn = min(kWave, cnt - q0);
+            const double zq = ld_coherent_val(&z[c0 + q0 + min(lane, qn - 1)]);
+            for (int h0 = 0; h0 < qn; h0 += SB) {
+                T v[SB];
+                bool in[SB];
 #pragma unroll
-            for (int h = 0; h < kSolveBatch; h++) {
-                const int jq = base + bw + min(q0 + h, cnt - 1);
-                const int d = jq - rr;                   // >= 1
-                in[h] = d <= ru[jq];
-                v[h] = ld_coherent_val(gp(M.val) + (uint32_t)(dp[jq] - min(d, ru[jq])), 0u);
-                zv[h] = ld_coherent_val(&z[jq]);
+                for (int h = 0; h < SB; h++) {
+                    const int jq = c0 + q0 + min(h0 + h, qn - 1);
+                    const int d = jq - rr;               // >= 1
+                    const int rj = ru[jq];
+                    in[h] = d <= rj;
+                    v[h] = ld_plain(gp(M.val), (uint32_t)(dp[jq] - min(d, rj)));
+                }
+#pragma unroll
+                for (int h = 0; h < SB; h++)
+                    if (h0 + h < qn && in[h]) acc = fma(to_f64(v[h]), bcast_f64(zq, h0 + h), acc);
             }
-#pragma unroll
-            for (int h = 0; h < kSolveBatch; h++)
-                if (q0 + h < cnt && in[h]) acc = fma(to_f64(v[h]), zv[h], acc);
         }
         num -= acc;
         const double rd = 1.0 / to_f64(M.diag[rr]);
         const int dpc = dp[rr];
         const int run = ru[rr];
         double zt;
-        if (bw <= 8) zt = run_triangle_upper<8>(M.val, num, rd, dpc, run, bw);
-        else if (bw <= 24) zt = run_triangle_upper<24>(M.val, num, rd, dpc, run, bw);
-        else zt = run_triangle_upper<kWave>(M.val, num, rd, dpc, run, bw);
+        if (bw <= 8) zt = run_triangle_upper<8, false>(M.val, num, rd, dpc, run, bw);
+        else if (bw <= 24) zt = run_triangle_upper<24, false>(M.val, num, rd, dpc, run, bw);
+        else zt = run_triangle_upper<kWave, false>(M.val, num, rd, dpc, run, bw);
         if (lane < bw) {
             st_coherent(&z[row], zt);
             const int qo = M.q[row];
@@ -132,9 +145,10 @@ __device__ bool tri_backward(const TriDesc<T> &M, int s, int e, double *z, doubl
 }
 
 // Packet task (code kTriPacket + o): up to 64 / g row tasks of one level,
-// each with at most g entries (g = 4, 8, 16 or 32); lanes [r g, (r + 1) g)
-// gather row r.  The rows' flags are polled together, each lane group sums
-// with g-lane shuffles, and every row is written as its own task would be.
+// each with at most g kTriPacketLane entries (g = 4, 8, 16 or 32); lanes
+// [r g, (r + 1) g) gather row r, lane l taking entries l, l + g, ...  The
+// rows' flags are polled together, each lane group sums with g-lane
+// shuffles, and every row is written as its own task would be.
 template <typename T, bool kForward>
 __device__ void tri_packet(const TriLaunch &L, const TriDesc<T> &M, int o, long vo, int *flags, int epoch)
 {
@@ -157,14 +171,29 @@ __device__ void tri_packet(const TriLaunch &L, const TriDesc<T> &M, int o, long 
             e1 = M.urp[i + 1];
         }
     }
-    const int e = e0 + l;
-    const bool on = e < e1;
-    const int jj = on ? (kForward ? M.lci : M.uci)[e] : 0;
-    const T vv = on ? (kForward ? M.lrow : M.urow)[e] : T(0);
-    if (!wave_wait(on ? &flags[jj] : nullptr, epoch, L.abort_word)) return;
+    const int *ci = kForward ? M.lci : M.uci;
+    const T *rv = kForward ? M.lrow : M.urow;
+    int jj[kTriPacketLane];
+    T vv[kTriPacketLane];
+    const int *fl[kTriPacketLane];
+#pragma unroll
+    for (int h = 0; h < kTriPacketLane; h++) {
+        const int e = e0 + l + h * g;
+        const bool on = e < e1;
+        jj[h] = on ? ci[e] : 0;
+        vv[h] = on ? rv[e] : T(0);
+        fl[h] = on ? &flags[jj[h]] : nullptr;
+    }
+    if (!wave_wait_n<kTriPacketLane>(fl, epoch, L.abort_word)) return;
     double *y = L.y + vo;
     double *z = L.z + vo;
-    double acc = on ? to_f64(vv) * ld_coherent_val(&(kForward ? y : z)[jj]) : 0.0;
+    const double *src = kForward ? y : z;
+    double sv[kTriPacketLane];
+#pragma unroll
+    for (int h = 0; h < kTriPacketLane; h++) sv[h] = fl[h] ? ld_coherent_val(&src[jj[h]]) : 0.0;
+    double acc = 0.0;
+#pragma unroll
+    for (int h = 0; h < kTriPacketLane; h++) acc += to_f64(vv[h]) * sv[h];
     for (int off = g >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
     const bool lead = rc >= 0 && l == 0;
     if (lead) {
@@ -185,14 +214,91 @@ __device__ void tri_packet(const TriLaunch &L, const TriDesc<T> &M, int o, long 
     if (lead) st_relaxed(&flags[i], part ? -epoch : epoch);
 }
 
+// One task of the solve (ticket t).
+template <typename T, bool kForward>
+__device__ void tri_task(const TriLaunch &L, int t, int epoch)
+{
+    const int lane = lane_id();
+    const TriDesc<T> *mats = (const TriDesc<T> *)L.mats;
+    const int2 tk = L.task[t];
+    if (L.active && !L.active[tk.x]) return;
+    const TriDesc<T> M = mats[tk.x];
+    const long vo = L.voff[tk.x];
+    double *y = L.y + vo;
+    double *z = L.z + vo;
+    int *flags = kForward ? M.fdone : M.bdone;
+    const int code = tk.y;
+    if (code >= kTriPacket) {
+        tri_packet<T, kForward>(L, M, code - kTriPacket, vo, flags, epoch);
+        return;
+    }
+    if (code < 0) {
+        const int s0 = -code - 1;
+        const int e0 = M.snend[s0];
+        if (kForward) tri_forward<T>(M, s0, e0, y, flags, epoch, L.abort_word);
+        else tri_backward<T>(M, s0, e0, z, L.x + vo, L.cs ? L.cs + vo : nullptr, flags, epoch, L.abort_word);
+        return;
+    }
+    const int i = code >> 1;
+    const bool part = code & 1;
+    const int *ci = kForward ? M.lci : M.uci;
+    const T *rv = kForward ? M.lrow : M.urow;
+    const double *src = kForward ? y : z;
+    int e0, e1;
+    if (kForward) {
+        e0 = M.lrp[i];
+        e1 = part ? M.fcut[i] : M.lrp[i + 1];
+    } else {
+        e0 = part ? M.bcut[i] : M.urp[i];
+        e1 = M.urp[i + 1];
+    }
+    double acc = 0.0;
+    bool ok = true;
+    for (int b0 = e0; b0 < e1 && ok; b0 += kGather * kWave) {
+        int jj[kGather];
+        const int *fl[kGather];
+        T vv[kGather];
+#pragma unroll
+        for (int g = 0; g < kGather; g++) {
+            const int e = b0 + g * kWave + lane;
+            jj[g] = e < e1 ? ci[e] : 0;
+            vv[g] = e < e1 ? rv[e] : T(0);
+            fl[g] = e < e1 ? &flags[jj[g]] : nullptr;
+        }
+        ok = wave_wait_n<kGather>(fl, epoch, L.abort_word);
+        if (!ok) break;
+        double sv[kGather];
+#pragma unroll
+        for (int g = 0; g < kGather; g++) sv[g] = fl[g] ? ld_coherent_val(&src[jj[g]]) : 0.0;
+#pragma unroll
+        for (int g = 0; g < kGather; g++) acc += to_f64(vv[g]) * sv[g];
+    }
+    if (!ok) return;
+    acc = wave_sum(acc);
+    if (lane == 0) {
+        if (kForward) {
+            const int ro = M.rowat[i];
+            const double bi = L.rs ? L.b[vo + ro] * L.rs[vo + ro] : L.b[vo + ro];
+            st_coherent(&y[i], bi - acc);
+        } else if (part) {
+            st_coherent(&z[i], y[i] - acc);
+        } else {
+            const double zi = (y[i] - acc) / to_f64(M.diag[i]);
+            st_coherent(&z[i], zi);
+            const int qo = M.q[i];
+            L.x[vo + qo] = L.cs ? zi * L.cs[vo + qo] : zi;
+        }
+        store_complete();
+        st_relaxed(&flags[i], part ? -epoch : epoch);
+    }
+}
+
 // Task codes (see the plan): 2i = row i solved whole, 2i+1 = row i's entries
 // outside its supernode into a partial sum, -(s+1) = supernode triangle,
 // kTriPacket + o = packet of short rows.
 template <typename T, bool kForward>
 __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
 {
-    const int lane = lane_id();
-    const TriDesc<T> *mats = (const TriDesc<T> *)L.mats;
     const int epoch = read_epoch(L.epoch);
     // a Krylov iteration after every circuit converged: nothing to solve
     const bool idle = L.n_active && uniform(ld_relaxed(L.n_active)) == 0;
@@ -201,79 +307,15 @@ __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
         const int t = wave_ticket(&L.ctr[0]);
         if (t >= L.ntask) break;
         if (aborted(L.abort_word)) continue;
-        const int2 tk = L.task[t];
-        if (L.active && !L.active[tk.x]) continue;
-        const TriDesc<T> M = mats[tk.x];
-        const long vo = L.voff[tk.x];
-        double *y = L.y + vo;
-        double *z = L.z + vo;
-        int *flags = kForward ? M.fdone : M.bdone;
-        const int code = tk.y;
-        if (code >= kTriPacket) {
-            tri_packet<T, kForward>(L, M, code - kTriPacket, vo, flags, epoch);
-            continue;
-        }
-        if (code < 0) {
-            const int s0 = -code - 1;
-            const int e0 = M.snend[s0];
-            if (kForward) tri_forward<T>(M, s0, e0, y, flags, epoch, L.abort_word);
-            else tri_backward<T>(M, s0, e0, z, L.x + vo, L.cs ? L.cs + vo : nullptr, flags, epoch, L.abort_word);
-            continue;
-        }
-        const int i = code >> 1;
-        const bool part = code & 1;
-        const int *ci = kForward ? M.lci : M.uci;
-        const T *rv = kForward ? M.lrow : M.urow;
-        const double *src = kForward ? y : z;
-        int e0, e1;
-        if (kForward) {
-            e0 = M.lrp[i];
-            e1 = part ? M.fcut[i] : M.lrp[i + 1];
-        } else {
-            e0 = part ? M.bcut[i] : M.urp[i];
-            e1 = M.urp[i + 1];
-        }
-        double acc = 0.0;
-        bool ok = true;
-        for (int b0 = e0; b0 < e1 && ok; b0 += kGather * kWave) {
-            int jj[kGather];
-            const int *fl[kGather];
-            T vv[kGather];
-#pragma unroll
-            for (int g = 0; g < kGather; g++) {
-                const int e = b0 + g * kWave + lane;
-                jj[g] = e < e1 ? ci[e] : 0;
-                vv[g] = e < e1 ? rv[e] : T(0);
-                fl[g] = e < e1 ? &flags[jj[g]] : nullptr;
-            }
-            ok = wave_wait_n<kGather>(fl, epoch, L.abort_word);
-            if (!ok) break;
-            double sv[kGather];
-#pragma unroll
-            for (int g = 0; g < kGather; g++) sv[g] = fl[g] ? ld_coherent_val(&src[jj[g]]) : 0.0;
-#pragma unroll
-            for (int g = 0; g < kGather; g++) acc += to_f64(vv[g]) * sv[g];
-        }
-        if (!ok) continue;
-        acc = wave_sum(acc);
-        if (lane == 0) {
-            if (kForward) {
-                const int ro = M.rowat[i];
-                const double bi = L.rs ? L.b[vo + ro] * L.rs[vo + ro] : L.b[vo + ro];
-                st_coherent(&y[i], bi - acc);
-            } else if (part) {
-                st_coherent(&z[i], y[i] - acc);
-            } else {
-                const double zi = (y[i] - acc) / to_f64(M.diag[i]);
-                st_coherent(&z[i], zi);
-                const int qo = M.q[i];
-                L.x[vo + qo] = L.cs ? zi * L.cs[vo + qo] : zi;
-            }
-            store_complete();
-            st_relaxed(&flags[i], part ? -epoch : epoch);
+        const long long tg = L.timeline ? (long long)wall_clock64() : 0;
+        tri_task<T, kForward>(L, t, epoch);
+        if (L.timeline && lane_id() == 0) {
+            // diagnostic: ticket granted, task finished (wall clock)
+            L.timeline[2L * t] = tg;
+            L.timeline[2L * t + 1] = (long long)wall_clock64();
         }
     }
-    if (lane == 0) {
+    if (lane_id() == 0) {
         const int fin = atomicAdd(&L.ctr[1], 1);
         if (fin == L.total_waves - 1) {
             st_relaxed(&L.ctr[0], 0);

@@ -26,7 +26,8 @@
 
 namespace lsp {
 
-static_assert(kTriPacket == ::kTriPacketBase, "packet codes of the plan and the solve kernel");
+static_assert(kTriPacket == ::kTriPacketBase && kTriPacketLane == ::kTriPacketPerLane,
+              "packet layout of the plan and the solve kernel");
 
 
 Batch::~Batch()
@@ -211,7 +212,7 @@ static int build_circuit(CircuitHost &C, const lsp_circuit_in &in, const lsp_opt
     if (C.plan.max_col > 65535) return LSP_E_CAPACITY;
     {
         const int tmax = opt.tail_max == 0 ? 8192 : opt.tail_max;
-        const double tden = opt.tail_density > 0 ? opt.tail_density : 0.3;
+        const double tden = opt.tail_density > 0 ? opt.tail_density : 0.5;
         if (tmax > 0 && lsp_plan_set_tail(C.plan, 64, tmax, tden) < 0) return LSP_E_STRUCTURE;
     }
     std::vector<int> rp, ci, rsrc;
@@ -888,7 +889,13 @@ int Batch::trisolve_t(FactorSet<T> &fs, const double *d_b, double *d_x, const in
     Lf.rs = scaled ? d_rscale_ : nullptr;
     Lf.cs = scaled ? d_cscale_ : nullptr;
     Lf.n_active = d_nact;
+    Lf.timeline = nullptr;
+    if (getenv("LSP_TRISOLVE_TIMELINE")) {                   // diagnostic dump (Batch::trisolve)
+        if (!d_tri_tl_) d_tri_tl_ = (long long *)dalloc((size_t)(nfw_tot_ + nbw_tot_) * 2 * sizeof(long long));
+        Lf.timeline = d_tri_tl_;
+    }
     TriLaunch Lb = Lf;
+    Lb.timeline = Lf.timeline ? Lf.timeline + 2 * nfw_tot_ : nullptr;
     Lb.task = d_bwtask_;
     Lb.ntask = (int)nbw_tot_;
     Lb.epoch = d_epoch_ + 2;
@@ -951,6 +958,25 @@ int Batch::trisolve(const double *b, double *x, bool on_device)
     if (!rc) rc = check(hipStreamSynchronize(stream_), "sync");
     if (rc) return rc;
     account_trisolve();
+    const char *tl_path = getenv("LSP_TRISOLVE_TIMELINE");
+    if (tl_path && d_tri_tl_) {
+        // [nfw, nbw] (int64), forward then backward task lists (int32 x 2 per
+        // task), then per task: ticket granted, task finished (int64 ticks)
+        const size_t nt = (size_t)(nfw_tot_ + nbw_tot_);
+        std::vector<long long> tl(2 * nt);
+        std::vector<int2> tf(nfw_tot_), tb(nbw_tot_);
+        (void)hipMemcpy(tl.data(), d_tri_tl_, tl.size() * sizeof(long long), hipMemcpyDeviceToHost);
+        (void)hipMemcpy(tf.data(), d_fwtask_, tf.size() * sizeof(int2), hipMemcpyDeviceToHost);
+        (void)hipMemcpy(tb.data(), d_bwtask_, tb.size() * sizeof(int2), hipMemcpyDeviceToHost);
+        if (FILE *f = fopen(tl_path, "wb")) {
+            const long long hdr[2] = {nfw_tot_, nbw_tot_};
+            fwrite(hdr, sizeof(long long), 2, f);
+            fwrite(tf.data(), sizeof(int2), tf.size(), f);
+            fwrite(tb.data(), sizeof(int2), tb.size(), f);
+            fwrite(tl.data(), sizeof(long long), tl.size(), f);
+            fclose(f);
+        }
+    }
     return h_pin_[0] ? LSP_E_DEVICE : LSP_OK;
 }
 

@@ -150,6 +150,7 @@ private:
     int *d_dbg_ = nullptr;               // first stuck wait of the refactor
     int next_epoch_peek_ = 0;
     long long *d_timeline_ = nullptr;    // optional refactor timeline (LSP_REFACTOR_TIMELINE)
+    long long *d_tri_tl_ = nullptr;      // optional trisolve timeline (LSP_TRISOLVE_TIMELINE)
     double *d_scratch_ = nullptr;
     long scratch_stride_ = 0;
     int fblocks_ = 0, tblocks_f_ = 0, tblocks_b_ = 0;
