@@ -106,7 +106,11 @@ __global__ void __launch_bounds__(256) dense_diag_kernel(const DenseJob *jobs, c
     // constants); the steps inside a block stay a loop.  Masks are applied to
     // the broadcast operands (row multipliers zero for rows <= c, row values
     // zero outside their columns), so each element costs one fma; column c is
-    // then overwritten with l and -l.
+    // then overwritten with l and -l.  The reciprocal of the next pivot is
+    // taken one step ahead: its owner updates that element first and divides
+    // while the rest of its update runs, so no division sits between a
+    // step's barrier and the next.
+    if (tid == 0) rpiv[0] = 1.0 / A[0][0];
 #pragma unroll
     for (int ca = 0; ca < 4; ca++)
 #pragma unroll 1
@@ -119,7 +123,6 @@ __global__ void __launch_bounds__(256) dense_diag_kernel(const DenseJob *jobs, c
                 bc[par][0][s + 16 * bb] = A[ca][bb];
                 bc[par][1][s + 16 * bb] = X[ca][bb];
             }
-            if (s == (c & 15)) rpiv[par] = 1.0 / A[ca][ca];
         }
         if (s == (c & 15)) {
 #pragma unroll
@@ -138,6 +141,12 @@ __global__ void __launch_bounds__(256) dense_diag_kernel(const DenseJob *jobs, c
         for (int a = 0; a < 4; a++) {
             const int i = r + 16 * a;
             l[a] = i > c ? bc[par][2][i] * rp : 0.0;
+        }
+        // next pivot A(c+1, c+1): register (cn, cn) of thread ((c+1) % 16, (c+1) % 16)
+        if (c + 1 < kNb && r == ((c + 1) & 15) && s == ((c + 1) & 15)) {
+            const int cn1 = ca < 3 ? ca + 1 : 3;
+            const double dn = cl < 15 ? fma(-l[ca], ur[ca], A[ca][ca]) : fma(-l[cn1], ur[cn1], A[cn1][cn1]);
+            rpiv[par ^ 1] = 1.0 / dn;
         }
 #pragma unroll
         for (int a = 0; a < 4; a++)

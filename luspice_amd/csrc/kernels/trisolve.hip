@@ -297,7 +297,7 @@ __device__ void tri_task(const TriLaunch &L, int t, int epoch)
 // outside its supernode into a partial sum, -(s+1) = supernode triangle,
 // kTriPacket + o = packet of short rows.
 template <typename T, bool kForward>
-__global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
+__device__ __forceinline__ void trisolve_body(const TriLaunch &L)
 {
     const int epoch = read_epoch(L.epoch);
     // a Krylov iteration after every circuit converged: nothing to solve
@@ -326,12 +326,23 @@ __global__ void __launch_bounds__(256) trisolve_kernel(TriLaunch L)
 }
 
 template <typename T>
+__global__ void __launch_bounds__(256) trisolve_fwd_kernel(TriLaunch L)
+{
+    trisolve_body<T, true>(L);
+}
+template <typename T>
+__global__ void __launch_bounds__(256) trisolve_bwd_kernel(TriLaunch L)
+{
+    trisolve_body<T, false>(L);
+}
+
+template <typename T>
 hipError_t launch_trisolve_half(const TriLaunch &L, bool forward, int blocks, hipStream_t st)
 {
     if (forward)
-        hipLaunchKernelGGL((trisolve_kernel<T, true>), dim3(blocks), dim3(256), 0, st, L);
+        hipLaunchKernelGGL(trisolve_fwd_kernel<T>, dim3(blocks), dim3(256), 0, st, L);
     else
-        hipLaunchKernelGGL((trisolve_kernel<T, false>), dim3(blocks), dim3(256), 0, st, L);
+        hipLaunchKernelGGL(trisolve_bwd_kernel<T>, dim3(blocks), dim3(256), 0, st, L);
     return hipGetLastError();
 }
 

@@ -613,12 +613,14 @@ int Batch::upload_all()
     }
     // persistent grids
     {
-        // LDS doubles per refactor wave: the requested cap, raised to hold the
-        // longest column when that fits 4 waves in the 160 KB of a CU
+        // LDS doubles per refactor wave: the requested cap, raised towards the
+        // longest column up to 2048 (two 4-wave blocks per CU's 160 KB; on
+        // c1 that beats one block holding every column: 4.65 vs 4.91 ms).  A
+        // negative request is taken as is.  Longer columns use global scratch.
         int maxcol_all = 0;
         for (auto &C : circ_) maxcol_all = std::max(maxcol_all, C.plan.max_col);
-        int cap = opt_.lds_cap > 0 ? opt_.lds_cap : 1024;
-        if (maxcol_all > cap && maxcol_all <= 4096) cap = (maxcol_all + 255) / 256 * 256;
+        int cap = opt_.lds_cap > 0 ? opt_.lds_cap : opt_.lds_cap < 0 ? -opt_.lds_cap : 1024;
+        if (opt_.lds_cap >= 0 && maxcol_all > cap) cap = std::min(2048, (maxcol_all + 255) / 256 * 256);
         opt_.lds_cap = cap;
         const size_t lds = (size_t)cap * 8 * 4;
         if (lds > 65536) {
