@@ -64,6 +64,7 @@ typedef struct {
     double tail_density;   /* min pattern density of that block (0 = 0.5) */
     int ruiz_sweeps;       /* Ruiz sweeps ahead of FP16 factor storage (0 = 30) */
     int pivot_scaled;      /* 1: choose the pivot sequence on the Ruiz-scaled matrix */
+    int poll_backoff;      /* refactor dependency polls: at most this + 1 sleeps apart (0 = 7) */
 } lsp_options;
 
 typedef struct {

@@ -232,7 +232,7 @@ __device__ __forceinline__ bool wave_wait(const int *flag, int epoch, int *abort
 // leading run of lanes whose flags have all been seen, once p > have, or -1
 // when the wait was abandoned.  The result is wave-uniform.
 __device__ __forceinline__ int wave_wait_prefix(const int *flag, int epoch, int *abort_word, int &ok, int have,
-                                                int nd, int *dbg = nullptr, int who2 = -1)
+                                                int nd, int *dbg = nullptr, int who2 = -1, int max_backoff = 7)
 {
     int seen = epoch;
     int backoff = 0;
@@ -256,7 +256,7 @@ __device__ __forceinline__ int wave_wait_prefix(const int *flag, int epoch, int 
             return -1;
         }
         for (int s = 0; s <= backoff; s++) __builtin_amdgcn_s_sleep(4);
-        if (backoff < 7) backoff = 2 * backoff + 1;
+        if (backoff < max_backoff) backoff = 2 * backoff + 1;
     }
 }
 

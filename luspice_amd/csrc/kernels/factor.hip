@@ -99,7 +99,7 @@ __device__ __forceinline__ void run_gemv(const T *val, XP x, const LSP_GLOBAL ui
 // a time; each lane fetches the metadata and waits on the flag of one slot.
 template <typename T, typename XP>
 __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
-                              int epoch, int *abort_word, double ptol, int *dbg, long long *tw)
+                              int epoch, int *abort_word, double ptol, int *dbg, long long *tw, int backoff)
 {
     const int lane = lane_id();
     const LSP_GLOBAL int64_t *cp = gp(M.cp);
@@ -155,9 +155,9 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
         // prefix is cut there; its later rows stay in the rows below.
         int ok = flag == nullptr;
         for (int have = 0; have < nd;) {
-            const int p = wave_wait_prefix(flag, epoch, abort_word, ok, have, nd, dbg, k);
+            const int p = wave_wait_prefix(flag, epoch, abort_word, ok, have, nd, dbg, k, backoff);
             if (p < 0) return;                                      // aborted; kernel drains
-            if (tw) *tw = (long long)wall_clock64();
+            if (tw) tw[0] = (long long)wall_clock64();
             unsigned long long hs = (heads & ~((1ull << have) - 1)) | (1ull << have);
             if (p < kWave) hs &= (1ull << p) - 1;
             while (hs) {
@@ -225,6 +225,7 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
     const double piv = x[dl];
     for (int i = dl + 1 + lane; i < m; i += kWave) st_coherent(&M.val[c0 + i], from_f64<T>(x[i] / piv));
     store_complete();
+    if (tw) tw[1] = (long long)wall_clock64();  // timeline: flag published
     if (lane == 0) st_relaxed(&M.done[k], epoch);
 
     // a stored value must be finite in the storage type (65504 for FP16)
@@ -282,21 +283,21 @@ __global__ void __launch_bounds__(256) refactor_kernel(FactorLaunch L)
         const FactorDesc<T> M = mats[tk.x];
         const int k = tk.y;
         const int m = (int)(M.cp[k + 1] - M.cp[k]);
-        long long tw = 0;
-        long long *wp = L.timeline ? &tw : nullptr;
+        long long tw[2] = {0, 0};               // end of the last wait, flag published
+        long long *wp = L.timeline ? tw : nullptr;
         const long long t0 = L.timeline ? (long long)wall_clock64() : 0;
         if (m <= L.cap) {
-            factor_column<T>(M, k, xl, epoch, L.abort_word, L.ptol, L.dbg, wp);
+            factor_column<T>(M, k, xl, epoch, L.abort_word, L.ptol, L.dbg, wp, L.poll_backoff);
         } else if (m <= L.scratch_stride) {
-            factor_column<T>(M, k, xg, epoch, L.abort_word, L.ptol, L.dbg, wp);
+            factor_column<T>(M, k, xg, epoch, L.abort_word, L.ptol, L.dbg, wp, L.poll_backoff);
         } else {
             if (lane_id() == 0) st_relaxed(L.abort_word, kErrCapacity);
         }
         if (L.timeline && lane_id() == 0) {
             long long *r = L.timeline + 4L * t;
             r[0] = t0;
-            r[1] = (long long)wall_clock64();
-            r[2] = tw;
+            r[1] = tw[1] ? tw[1] : (long long)wall_clock64();
+            r[2] = tw[0];
             r[3] = k;
         }
     }

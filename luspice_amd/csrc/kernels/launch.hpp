@@ -23,8 +23,10 @@ struct FactorLaunch {
     int total_waves;
     double ptol;
     int *dbg;               // 8 ints: first stuck wait (dep, seen, epoch, spins, column)
-    long long *timeline;    // optional, 4 per task: start, end (100 MHz wall clock),
-                            // shader clocks spent waiting on dependencies, column
+    long long *timeline;    // optional, 4 per task (100 MHz wall clock): start, flag
+                            // published (end for a dense-block column), end of the
+                            // last dependency wait, column
+    int poll_backoff;       // dependency polls back off to at most poll_backoff + 1 sleeps
 };
 
 struct TriLaunch {

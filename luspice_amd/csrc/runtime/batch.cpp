@@ -764,6 +764,7 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
     L.scratch_stride = scratch_stride_;
     L.cap = opt_.lds_cap;
     L.total_waves = fblocks_ * 4;
+    L.poll_backoff = opt_.poll_backoff > 0 ? opt_.poll_backoff : 7;
     L.ptol = opt_.refactor_ptol;
     L.dbg = d_dbg_;
     rc = check(launch_reset_mstat(d_mstat_, (int)circ_.size(), d_abort_, d_dbg_, stream_), "reset");
