@@ -97,7 +97,8 @@ struct LspPlan {
     // packed: code kTriPacketBase + o names a packet at offset o of fpk /
     // bpk, laid out as [g, count, row code x count]: g lanes per row (4, 8,
     // 16 or 32, with g kTriPacketPerLane >= the row's entry count), up to
-    // 64 / g rows.
+    // 64 / g rows.  Triangle tasks of supernodes of at most 16 rows are
+    // packed as [-g, count, supernode start x count], g >= the width.
     std::vector<int> fwd_task, bwd_task;
     std::vector<int> fwd_key, bwd_key;    // ordering keys (merge across circuits)
     std::vector<int> fpk, bpk;            // packets (see above)

@@ -418,14 +418,32 @@ static void plan_tri_supernodes(LspPlan &P)
             return fwd ? (part ? P.fcut[i] : P.lrp[i + 1]) - P.lrp[i]
                        : P.urp[i + 1] - (part ? P.bcut[i] : P.urp[i]);
         };
+        // Triangle tasks of supernodes of at most g rows (g = 4, 8, 16) are
+        // packed the same way, 64 / g to a packet laid out as
+        // [-g, count, supernode start x count].
         std::vector<std::pair<int, int>> out;
         out.reserve(items.size());
         pk.clear();
-        std::vector<int> cls[4];
+        std::vector<int> cls[4], tcls[3];
+        auto emit = [&](int key_v, std::vector<int> &v, int g, bool tri) {
+            const int per = kWave64 / g;
+            for (size_t x = 0; x < v.size(); x += per) {
+                const int cnt = (int)std::min<size_t>(per, v.size() - x);
+                if (cnt == 1) {
+                    out.push_back({key_v, tri ? -(v[x] + 1) : v[x]});
+                    continue;
+                }
+                out.push_back({key_v, kTriPacketBase + (int)pk.size()});
+                pk.push_back(tri ? -g : g);
+                pk.push_back(cnt);
+                for (int r = 0; r < cnt; r++) pk.push_back(v[x + r]);
+            }
+        };
         for (size_t a = 0; a < items.size();) {
             size_t b = a;
             while (b < items.size() && items[b].first == items[a].first) b++;
             for (auto &c : cls) c.clear();
+            for (auto &c : tcls) c.clear();
             for (size_t t = a; t < b; t++) {
                 const int code = items[t].second;
                 if (code >= 0) {
@@ -433,20 +451,16 @@ static void plan_tri_supernodes(LspPlan &P)
                     const int k = kTriPacketPerLane;
                     const int c = ne <= 4 * k ? 0 : ne <= 8 * k ? 1 : ne <= 16 * k ? 2 : ne <= 32 * k ? 3 : -1;
                     if (c >= 0) { cls[c].push_back(code); continue; }
+                } else {
+                    const int s0 = -code - 1;
+                    const int w = P.snend[s0] - s0;
+                    const int c = w <= 4 ? 0 : w <= 8 ? 1 : w <= 16 ? 2 : -1;
+                    if (c >= 0) { tcls[c].push_back(s0); continue; }
                 }
                 out.push_back(items[t]);
             }
-            for (int c = 0; c < 4; c++) {
-                const int g = 4 << c, per = kWave64 / g;
-                for (size_t x = 0; x < cls[c].size(); x += per) {
-                    const int cnt = (int)std::min<size_t>(per, cls[c].size() - x);
-                    if (cnt == 1) { out.push_back({items[a].first, cls[c][x]}); continue; }
-                    out.push_back({items[a].first, kTriPacketBase + (int)pk.size()});
-                    pk.push_back(g);
-                    pk.push_back(cnt);
-                    for (int r = 0; r < cnt; r++) pk.push_back(cls[c][x + r]);
-                }
-            }
+            for (int c = 0; c < 4; c++) emit(items[a].first, cls[c], 4 << c, false);
+            for (int c = 0; c < 3; c++) emit(items[a].first, tcls[c], 4 << c, true);
             a = b;
         }
         task.resize(out.size());

@@ -214,6 +214,85 @@ __device__ void tri_packet(const TriLaunch &L, const TriDesc<T> &M, int o, long 
     if (lead) st_relaxed(&flags[i], part ? -epoch : epoch);
 }
 
+// Packet of small supernode triangles (header -g, g = 4, 8 or 16): up to
+// 64 / g supernodes of one level with at most g rows each; lanes
+// [r g, (r + 1) g) hold the rows of supernode r.  Every row's partial sum is
+// awaited, the dense diagonal block's entries are loaded before the first
+// step, and the triangle is solved with g-lane shuffles.
+constexpr int kTriPacketRows = 16;      // widest supernode in a triangle packet
+
+template <typename T, bool kForward>
+__device__ void tri_tpacket(const TriLaunch &L, const TriDesc<T> &M, int o, long vo, int *flags, int epoch)
+{
+    const int lane = lane_id();
+    const LSP_GLOBAL int *pk = gp(kForward ? M.fpk : M.bpk) + o;
+    const int g = uniform(-pk[0]);
+    const int cnt = uniform(pk[1]);
+    const int lg = __builtin_ctz(g);
+    const int grp = lane >> lg;
+    const int l = lane & (g - 1);
+    const int s = grp < cnt ? pk[2 + grp] : 0;
+    const int w = grp < cnt ? M.snend[s] - s : 0;
+    const bool on = l < w;
+    const int row = s + min(l, max(w - 1, 0));
+    const int src0 = grp << lg;                         // first lane of this group
+    const LSP_GLOBAL int *dp = gp(M.dp);
+    if (kForward) {
+        // L(s + l, s + q) = val[dp[s + q] + (l - q)] for q < l
+        T tv[kTriPacketRows - 1];
+#pragma unroll
+        for (int q = 0; q < kTriPacketRows - 1; q++) {
+            const int j = s + min(q, max(w - 1, 0));
+            const int off = on && q < l ? l - q : 0;
+            tv[q] = ld_plain(gp(M.val), (uint32_t)(dp[j] + off));
+        }
+        if (!wave_wait(on ? &flags[row] : nullptr, -epoch, L.abort_word)) return;
+        double *y = L.y + vo;
+        double p = on ? ld_coherent_val(&y[row]) : 0.0;
+#pragma unroll
+        for (int q = 0; q < kTriPacketRows - 1; q++) {
+            if (q < g - 1) {
+                const double yq = __shfl(p, src0 + q, kWave);
+                if (on && q < l) p = fma(-to_f64(tv[q]), yq, p);
+            }
+        }
+        if (on) st_coherent(&y[row], p);
+        store_complete();
+        if (on) st_relaxed(&flags[row], epoch);
+    } else {
+        // U(s + l, s + q) = val[dp[s + q] - (q - l)] for l < q within column
+        // s + q's run ru, zero above it
+        const LSP_GLOBAL int *ru = gp(M.ru);
+        T tv[kTriPacketRows];
+        bool in[kTriPacketRows];
+#pragma unroll
+        for (int q = 0; q < kTriPacketRows; q++) {
+            const int j = s + min(q, max(w - 1, 0));
+            in[q] = on && q < w && l < q && q - l <= ru[j];
+            tv[q] = ld_plain(gp(M.val), (uint32_t)(dp[j] - (in[q] ? q - l : 0)));
+        }
+        const double rd = on ? 1.0 / to_f64(M.diag[row]) : 1.0;
+        if (!wave_wait(on ? &flags[row] : nullptr, -epoch, L.abort_word)) return;
+        double *z = L.z + vo;
+        double num = on ? ld_coherent_val(&z[row]) : 0.0;
+#pragma unroll
+        for (int q = kTriPacketRows - 1; q >= 0; q--) {
+            if (q < g) {
+                const double zq = __shfl(num, src0 + q, kWave) * __shfl(rd, src0 + q, kWave);
+                if (in[q]) num = fma(-to_f64(tv[q]), zq, num);
+            }
+        }
+        const double zt = num * rd;
+        if (on) {
+            st_coherent(&z[row], zt);
+            const int qo = M.q[row];
+            L.x[vo + qo] = L.cs ? zt * L.cs[vo + qo] : zt;
+        }
+        store_complete();
+        if (on) st_relaxed(&flags[row], epoch);
+    }
+}
+
 // One task of the solve (ticket t).
 template <typename T, bool kForward>
 __device__ void tri_task(const TriLaunch &L, int t, int epoch)
@@ -229,7 +308,9 @@ __device__ void tri_task(const TriLaunch &L, int t, int epoch)
     int *flags = kForward ? M.fdone : M.bdone;
     const int code = tk.y;
     if (code >= kTriPacket) {
-        tri_packet<T, kForward>(L, M, code - kTriPacket, vo, flags, epoch);
+        const int o = code - kTriPacket;
+        if (uniform(gp(kForward ? M.fpk : M.bpk)[o]) < 0) tri_tpacket<T, kForward>(L, M, o, vo, flags, epoch);
+        else tri_packet<T, kForward>(L, M, o, vo, flags, epoch);
         return;
     }
     if (code < 0) {
