@@ -167,6 +167,25 @@ __device__ __forceinline__ T ld_val(const LSP_GLOBAL T *base, uint32_t i)
     else return ld_plain(base, i);
 }
 
+// "Not yet written" marker of a factor value: a signaling-NaN bit pattern,
+// which no arithmetic result has (hardware NaN results are quiet NaNs).
+template <typename T> struct Unset;
+template <> struct Unset<float> {
+    static __device__ __forceinline__ bool is(float v) { return __float_as_uint(v) == 0x7FA5A5A5u; }
+    static __device__ __forceinline__ float value() { return __uint_as_float(0x7FA5A5A5u); }
+};
+template <> struct Unset<double> {
+    static __device__ __forceinline__ bool is(double v)
+    {
+        return (unsigned long long)__double_as_longlong(v) == 0x7FF4A5A5A5A5A5A5ull;
+    }
+    static __device__ __forceinline__ double value() { return __longlong_as_double(0x7FF4A5A5A5A5A5A5ll); }
+};
+template <> struct Unset<__half> {
+    static __device__ __forceinline__ bool is(__half v) { return __half_as_ushort(v) == 0x7D2Au; }
+    static __device__ __forceinline__ __half value() { return __ushort_as_half((unsigned short)0x7D2Au); }
+};
+
 // Orders this wave's accesses in the compiler only: the memory operations of
 // one wave complete in order, so LDS (and same-address global) traffic between
 // lanes of a wave needs no wait.

@@ -26,6 +26,8 @@
 //   * the pivot test (|pivot| >= ptol * max |L(:,k)|, in fp64, wave
 //     max-reduction) and the stores read only after the launch (U, diagonal,
 //     row-wise copies) follow the flag.
+#include <algorithm>
+
 #include "common.hpp"
 #include "launch.hpp"
 
@@ -80,6 +82,58 @@ __device__ __forceinline__ void run_gemv(const T *val, XP x, const LSP_GLOBAL ui
             }
         }
     }
+}
+
+// The rows below a single-column part whose column j has not been seen
+// published: the values of L(:, j) are loaded until none is unset (the
+// values array holds the unset marker before the launch, and every value is
+// written once, as a whole), then applied as run_gemv would.  Used for the
+// last dependency of a column when it is the column just before it (the
+// critical chain of a supernode), which saves the producer's store
+// acknowledgement and the flag round trip.  Returns false when abandoned.
+template <int G, typename T, typename XP>
+__device__ __forceinline__ bool run_gemv_unset(const T *val, XP x, const LSP_GLOBAL uint16_t *map, int64_t u,
+                                               double xj, int base, int R, int *abort_word)
+{
+    const int lane = lane_id();
+    for (int i0 = 0; i0 < R; i0 += G * kWave) {
+        int i[G];
+        T v[G];
+#pragma unroll
+        for (int g = 0; g < G; g++) i[g] = min(i0 + g * kWave + lane, R - 1);
+        for (int round = 0;; round++) {
+#pragma unroll
+            for (int g = 0; g < G; g++) v[g] = ld_coherent_val(gp(val) + (uint32_t)base, (uint32_t)i[g]);
+            bool miss = false;
+#pragma unroll
+            for (int g = 0; g < G; g++) miss |= Unset<T>::is(v[g]);
+            if (!__any(miss)) break;
+            if ((round & 15) == 15 && aborted(abort_word)) return false;
+            if (round > kRoundLimit) {
+                atomicCAS(abort_word, 0, kErrSpin);
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            if (i0 + g * kWave + lane < R) x[(int)map[u + i[g]]] -= to_f64(v[g]) * xj;
+        }
+    }
+    return true;
+}
+
+// Marks the factor values of every active matrix unset (before a refactor);
+// blockIdx.y is the matrix.
+template <typename T>
+__global__ void __launch_bounds__(256) fill_unset_kernel(const FactorDesc<T> *mats, const int *active)
+{
+    const int m = blockIdx.y;
+    if (active && !active[m]) return;
+    const FactorDesc<T> M = mats[m];
+    const long n = (long)M.cp[M.n];
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) M.val[i] = Unset<T>::value();
 }
 
 // One column of the left-looking refactorization.  XP is the work vector's
@@ -154,7 +208,21 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
         // the predecessor's column.  A run that continues past the ready
         // prefix is cut there; its later rows stay in the rows below.
         int ok = flag == nullptr;
+        // the last dependency is the column just before this one, alone in
+        // its part (the chain step inside a supernode): its values are
+        // checked directly instead of its flag
+        const bool chain_last = s0 + nd == ndep && __builtin_amdgcn_readlane(j, nd - 1) == k - 1;
         for (int have = 0; have < nd;) {
+            if (chain_last && have == nd - 1) {
+                const int lbj = __builtin_amdgcn_readlane(lb, nd - 1);
+                const int nlj = __builtin_amdgcn_readlane(nl, nd - 1);
+                const double xj = bcast_f64(xc, nd - 1);
+                if (!run_gemv_unset<8>(M.val, x, map, u, xj, lbj, nlj, abort_word)) return;
+                u += nlj;
+                wave_fence();
+                have = nd;
+                break;
+            }
             const int p = wave_wait_prefix(flag, epoch, abort_word, ok, have, nd, dbg, k, backoff);
             if (p < 0) return;                                      // aborted; kernel drains
             if (tw) tw[0] = (long long)wall_clock64();
@@ -345,6 +413,19 @@ hipError_t launch_refactor(const FactorLaunch &L, int blocks, hipStream_t st)
 template hipError_t launch_refactor<float>(const FactorLaunch &, int, hipStream_t);
 template hipError_t launch_refactor<double>(const FactorLaunch &, int, hipStream_t);
 template hipError_t launch_refactor<__half>(const FactorLaunch &, int, hipStream_t);
+
+template <typename T>
+hipError_t launch_fill_unset(const void *mats, const int *active, int nmat, long max_slots, hipStream_t st)
+{
+    const long want = (max_slots + 255) / 256;
+    const int bx = (int)std::max(1L, std::min(1024L, want));
+    hipLaunchKernelGGL(fill_unset_kernel<T>, dim3(bx, nmat), dim3(256), 0, st, (const FactorDesc<T> *)mats,
+                       active);
+    return hipGetLastError();
+}
+template hipError_t launch_fill_unset<float>(const void *, const int *, int, long, hipStream_t);
+template hipError_t launch_fill_unset<double>(const void *, const int *, int, long, hipStream_t);
+template hipError_t launch_fill_unset<__half>(const void *, const int *, int, long, hipStream_t);
 
 hipError_t launch_reset_mstat(int *mstat, int nmat, int *abort_word, int *dbg, hipStream_t st)
 {

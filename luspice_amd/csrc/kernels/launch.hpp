@@ -76,6 +76,8 @@ struct KrylovArgs {
 template <typename T> hipError_t launch_refactor(const FactorLaunch &L, int blocks, hipStream_t st);
 template <typename T> hipError_t refactor_set_lds(int cap);
 hipError_t launch_reset_mstat(int *mstat, int nmat, int *abort_word, int *dbg, hipStream_t st);
+template <typename T>
+hipError_t launch_fill_unset(const void *mats, const int *active, int nmat, long max_slots, hipStream_t st);
 
 // dense trailing blocks (dense.hip)
 hipError_t dense_set_lds();
