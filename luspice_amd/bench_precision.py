@@ -1,6 +1,7 @@
 """Precision sweep over ill-conditioned circuits (fp64 / fp32 / FP16+Ruiz).
 
     python -m luspice_amd.bench_precision [--matrices 64] [--nmin 1000] [--nmax 8000]
+                                          [--precs fp64 fp32 fp16]
 
 Workload: ``matrices`` circuits from the precision-sweep generator (mna kind
 'c2', seed 2): n log-uniform in [nmin, nmax], ~6 nonzeros per row,
@@ -53,12 +54,14 @@ def main(argv=None):
     ap.add_argument("--nmin", type=int, default=1000)
     ap.add_argument("--nmax", type=int, default=8000)
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--precs", nargs="+", default=["fp64", "fp32", "fp16"],
+                    choices=["fp64", "fp32", "fp16"], help="factor storage precisions to run")
     args = ap.parse_args(argv)
     import luspice_amd as L
 
     rng = np.random.default_rng(args.seed)
     ns = np.rint(np.exp(rng.uniform(math.log(args.nmin), math.log(args.nmax), args.matrices))).astype(int)
-    precs = ["fp64", "fp32", "fp16"]
+    precs = list(args.precs)
     rows = []
     t_all = time.time()
     for i, n in enumerate(ns):
@@ -73,7 +76,7 @@ def main(argv=None):
             # fp64 / fp32 keep the default pivot sequence; FP16 pivots on the
             # Ruiz-scaled matrix it stores
             Bd = L.Batch(circ, fallback_fp64=fb)
-            Bs = L.Batch(circ, fallback_fp64=fb, pivot_scaled=1)
+            Bs = L.Batch(circ, fallback_fp64=fb, pivot_scaled=1) if "fp16" in precs else None
             for prec in precs:
                 B = Bs if prec == "fp16" else Bd
                 rr = B.refactor(prec)
@@ -86,7 +89,8 @@ def main(argv=None):
                 if fb == 0:
                     rec[key]["backward_error"] = backward_error(c, x)
             Bd.close()
-            Bs.close()
+            if Bs is not None:
+                Bs.close()
         rows.append(rec)
         print(json.dumps(rec), flush=True)
     summ = {"metric": "precision sweep: pass rate at rel err <= 1e-6 and fp64-fallback rate",
