@@ -769,11 +769,12 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
     L.dbg = d_dbg_;
     rc = check(launch_reset_mstat(d_mstat_, (int)circ_.size(), d_abort_, d_dbg_, stream_), "reset");
     if (rc) return rc;
-    if (!capturing_) (void)hipEventRecord(ev_[0], stream_);
     long max_slots = 0;
     for (const auto &C : circ_) max_slots = std::max(max_slots, (long)C.plan.cp[C.n]);
     rc = check(launch_fill_unset<T>(fs.fdesc, d_active, (int)circ_.size(), max_slots, stream_), "unset");
-    if (!rc) rc = check(launch_refactor<T>(L, fblocks_, stream_), "refactor launch");
+    if (rc) return rc;
+    if (!capturing_) (void)hipEventRecord(ev_[0], stream_);   // the persistent kernel alone, from here
+    rc = check(launch_refactor<T>(L, fblocks_, stream_), "refactor launch");
     if (!capturing_) (void)hipEventRecord(ev_[1], stream_);      // the persistent kernel alone
     if (!rc && njobs_) {
         rc = check(launch_dense_lu(d_djobs_, njobs_, max_tail_T_, d_active, stream_), "dense lu");
