@@ -38,7 +38,7 @@ SPACE = {
     "refactor_blocks": [0, 64, 128, 256, 512, 1024],
     "trisolve_blocks": [0, 64, 128, 256, 512, 1024],
     "lds_cap": [512, 1024, 2048],
-    "tail_density": [0.2, 0.3, 0.45, 0.6],
+    "tail_density": [0.3, 0.4, 0.5, 0.6, 0.7],
 }
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "luspice_amd", "autotune.json")
