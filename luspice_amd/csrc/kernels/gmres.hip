@@ -58,13 +58,18 @@ __global__ void __launch_bounds__(256) chunk_dots_kernel(KrylovArgs K, const dou
 
 // out[m*(mmax+1)+i] = sum of partials (mode 0: store, mode 1: add into H
 // column j and hcol, mode 2: sqrt into hn)
-__global__ void reduce_partials_kernel(KrylovArgs K, int nv, int mode, int j, double *out)
+// One wave per (matrix, i): lanes stride over the matrix's chunks, then a
+// wave sum (a single thread walking hundreds of chunks took ~46 us on c1).
+__global__ void __launch_bounds__(64) reduce_partials_kernel(KrylovArgs K, int nv, int mode, int j, double *out)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x;
     const int m = t / nv, i = t % nv;
     if (m >= K.nmat || !K.act[m]) return;
     double s = 0.0;
-    for (int c = K.cbeg[m]; c < K.cbeg[m + 1]; c++) s += K.partial[(long)c * (K.mmax + 1) + i];
+    for (int c = K.cbeg[m] + (int)threadIdx.x; c < K.cbeg[m + 1]; c += kWave)
+        s += K.partial[(long)c * (K.mmax + 1) + i];
+    s = wave_sum(s);
+    if (threadIdx.x != 0) return;
     if (mode == 0) {
         out[(long)m * (K.mmax + 1) + i] = s;
     } else if (mode == 1) {
@@ -103,26 +108,29 @@ __global__ void __launch_bounds__(256) chunk_scale_kernel(KrylovArgs K, const do
 }
 
 // Cycle start: outer convergence test on beta, reset H and g.
-__global__ void start_cycle_kernel(KrylovArgs K, int *n_active)
+// One workgroup per matrix: every thread reads the same state and takes the
+// same branch; the Hessenberg and g are cleared by all threads together.
+__global__ void __launch_bounds__(128) start_cycle_kernel(KrylovArgs K, int *n_active)
 {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.x;
     if (m >= K.nmat) return;
-    K.cycled[m] = 0;
-    K.jfin[m] = 0;
-    if (K.conv[m]) { K.act[m] = 0; return; }
+    const bool lead = threadIdx.x == 0;
+    const bool conv = K.conv[m] != 0;
     const double rel = K.beta[m] / K.nb[m];
-    if (rel <= K.outer_tol || !(K.beta[m] > 0.0)) {
-        K.conv[m] = 1;
-        K.act[m] = 0;
-        return;
+    const bool done = !conv && (rel <= K.outer_tol || !(K.beta[m] > 0.0));
+    __syncthreads();                            // every thread has read conv before it changes
+    if (lead) {
+        K.cycled[m] = 0;
+        K.jfin[m] = 0;
+        K.act[m] = conv || done ? 0 : 1;
+        if (done) K.conv[m] = 1;
     }
-    K.act[m] = 1;
+    if (conv || done) return;
     double *H = K.H + (long)m * (K.mmax + 1) * K.mmax;
-    for (long t = 0; t < (long)(K.mmax + 1) * K.mmax; t++) H[t] = 0.0;
+    for (long t = threadIdx.x; t < (long)(K.mmax + 1) * K.mmax; t += blockDim.x) H[t] = 0.0;
     double *g = K.g + (long)m * (K.mmax + 1);
-    for (int t = 0; t <= K.mmax; t++) g[t] = 0.0;
-    g[0] = K.beta[m];
-    atomicAdd(n_active, 1);
+    for (int t = threadIdx.x; t <= K.mmax; t += blockDim.x) g[t] = t == 0 ? K.beta[m] : 0.0;
+    if (lead) atomicAdd(n_active, 1);
 }
 
 // Givens step for column j; sets act = 0 when the cycle ends for matrix m.
@@ -255,7 +263,7 @@ hipError_t k_chunk_dots(const KrylovArgs &K, const double *V, const double *w, i
 hipError_t k_reduce(const KrylovArgs &K, int nv, int mode, int j, double *out, hipStream_t s)
 {
     const int tot = K.nmat * nv;
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(nb_for(tot, 128)), dim3(128), 0, s, K, nv, mode, j, out);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(tot > 0 ? tot : 1), dim3(kWave), 0, s, K, nv, mode, j, out);
     return hipGetLastError();
 }
 hipError_t k_chunk_update(const KrylovArgs &K, const double *V, double *w, int nv, hipStream_t s)
@@ -270,7 +278,7 @@ hipError_t k_chunk_scale(const KrylovArgs &K, const double *src, double *dst, co
 }
 hipError_t k_start_cycle(const KrylovArgs &K, int *n_active, hipStream_t s)
 {
-    hipLaunchKernelGGL(start_cycle_kernel, dim3(nb_for(K.nmat, 128)), dim3(128), 0, s, K, n_active);
+    hipLaunchKernelGGL(start_cycle_kernel, dim3(K.nmat > 0 ? K.nmat : 1), dim3(128), 0, s, K, n_active);
     return hipGetLastError();
 }
 hipError_t k_givens(const KrylovArgs &K, int j, int *n_active, hipStream_t s)
