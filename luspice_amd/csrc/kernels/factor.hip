@@ -291,7 +291,8 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
     // and the row-wise copies are needed only after the launch, so they come
     // after the flag, off the dependency chain.
     const double piv = x[dl];
-    for (int i = dl + 1 + lane; i < m; i += kWave) st_coherent(&M.val[c0 + i], from_f64<T>(x[i] / piv));
+    const double rpiv = 1.0 / piv;             // one division; L = x * (1 / pivot)
+    for (int i = dl + 1 + lane; i < m; i += kWave) st_coherent(&M.val[c0 + i], from_f64<T>(x[i] * rpiv));
     store_complete();
     if (tw) tw[1] = (long long)wall_clock64();  // timeline: flag published
     if (lane == 0) st_relaxed(&M.done[k], epoch);
@@ -328,7 +329,7 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
         st_g(&M.val[c0 + dl], v);
         st_g(&M.diag[k], v);
     }
-    for (int i = dl + 1 + lane; i < m; i += kWave) st_g(&M.lrow[dst[c0 + i]], from_f64<T>(x[i] / piv));
+    for (int i = dl + 1 + lane; i < m; i += kWave) st_g(&M.lrow[dst[c0 + i]], from_f64<T>(x[i] * rpiv));
 }
 
 template <typename T>
