@@ -65,6 +65,9 @@ typedef struct {
     int ruiz_sweeps;       /* Ruiz sweeps ahead of FP16 factor storage (0 = 30) */
     int pivot_scaled;      /* 1: choose the pivot sequence on the Ruiz-scaled matrix */
     int poll_backoff;      /* refactor dependency polls: at most this + 1 sleeps apart (0 = 7) */
+    double factor_shift;   /* factor-only diagonal shift: the first diagonal entry of each row
+                              is factored as a + shift*|a| (0 = none; the residual and SpMV keep
+                              the caller's values; test_lu_gpu_f32.cu:354-364) */
 } lsp_options;
 
 typedef struct {

@@ -40,7 +40,8 @@ class Options(C.Structure):
                 ("sell_sigma", C.c_int), ("fallback_fp64", C.c_int),
                 ("refactor_blocks", C.c_int), ("trisolve_blocks", C.c_int),
                 ("tail_max", C.c_int), ("tail_density", C.c_double), ("ruiz_sweeps", C.c_int),
-                ("pivot_scaled", C.c_int), ("poll_backoff", C.c_int)]
+                ("pivot_scaled", C.c_int), ("poll_backoff", C.c_int),
+                ("factor_shift", C.c_double)]
 
 
 class RefactorReport(C.Structure):

@@ -102,6 +102,8 @@ hipError_t launch_sell_spmv(const SellDesc<V> *mats, const int2 *tasks, int ntas
                             const int *active, hipStream_t st);
 hipError_t launch_csr_spmv(int n, const int *rowptr, const int *colidx, const double *values,
                            const double *x, double *y, hipStream_t st);
+hipError_t launch_csr_spmv_f32(int n, const int *rowptr, const int *colidx, const float *values,
+                               const double *x, double *y, hipStream_t st);
 template <typename Vout>
 hipError_t launch_gather_cast(long count, const int *idx, const double *in, Vout *out,
                               hipStream_t st);
@@ -138,5 +140,8 @@ struct RuizArgs {
     int pad_;
 };
 hipError_t launch_ruiz(const RuizArgs &R, int ncirc, hipStream_t st);
+// Factor-only diagonal shift: dst[e] = src[e] + shift*|src[e]| for e in idx[0..nidx).
+hipError_t launch_shift(const double *src, double *dst, const long *idx, long nidx, double shift,
+                        hipStream_t st);
 
 }  // namespace lsp

@@ -58,4 +58,28 @@ hipError_t launch_ruiz(const RuizArgs &R, int ncirc, hipStream_t st)
     return hipGetLastError();
 }
 
+// Factor-only diagonal shift (test_lu_gpu_f32.cu:354-364, test_lu_gpu_ir64.cu:
+// 206-212): for each row i the first value entry on the diagonal, in caller
+// order, becomes a + shift*|a| in the copy that is factored; the residual keeps
+// the unshifted values.  idx lists those entries (global value indices), at
+// most one per row; src and dst may be the same buffer.
+__global__ void __launch_bounds__(256) shift_kernel(const double *src, double *dst, const long *idx,
+                                                    long nidx, double shift)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nidx; t += (long)gridDim.x * blockDim.x) {
+        const long e = idx[t];
+        const double a = src[e];
+        dst[e] = a + shift * fabs(a);
+    }
+}
+
+hipError_t launch_shift(const double *src, double *dst, const long *idx, long nidx, double shift,
+                        hipStream_t st)
+{
+    if (nidx <= 0) return hipSuccess;
+    const long want = (nidx + 255) / 256, blocks = want < 4096 ? want : 4096;
+    hipLaunchKernelGGL(shift_kernel, dim3((unsigned)blocks), dim3(256), 0, st, src, dst, idx, nidx, shift);
+    return hipGetLastError();
+}
+
 }  // namespace lsp

@@ -63,6 +63,22 @@ csr_spmv_kernel(int n, const int *__restrict__ rowptr, const int *__restrict__ c
     }
 }
 
+// CSR with fp32 values: the same loop reading float values, fp64 x and sum
+// (the reference's k_csr32, src/mp/mp_spmv_bench.cu:80-88, keeps x and y in
+// fp32; here only the stored values are narrowed).
+__global__ void __launch_bounds__(256)
+csr_spmv_f32_kernel(int n, const int *__restrict__ rowptr, const int *__restrict__ colidx,
+                    const float *__restrict__ values, const double *__restrict__ x,
+                    double *__restrict__ y)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row < n) {
+        double s = 0.0;
+        for (int e = rowptr[row]; e < rowptr[row + 1]; e++) s += (double)values[e] * x[colidx[e]];
+        y[row] = s;
+    }
+}
+
 // out[i] = idx[i] >= 0 ? (Vout) in[idx[i]] : 0   (value repack per refactor)
 template <typename Vout>
 __global__ void gather_cast_kernel(long count, const int *idx, const double *in, Vout *out)
@@ -101,6 +117,16 @@ hipError_t launch_csr_spmv(int n, const int *rowptr, const int *colidx, const do
     if (n <= 0) return hipSuccess;
     const int blocks = (n + 255) / 256;
     hipLaunchKernelGGL(csr_spmv_kernel, dim3(blocks), dim3(256), 0, st, n, rowptr, colidx,
+                       values, x, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_csr_spmv_f32(int n, const int *rowptr, const int *colidx, const float *values,
+                               const double *x, double *y, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const int blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(csr_spmv_f32_kernel, dim3(blocks), dim3(256), 0, st, n, rowptr, colidx,
                        values, x, y);
     return hipGetLastError();
 }

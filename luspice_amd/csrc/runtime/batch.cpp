@@ -244,7 +244,9 @@ int Batch::ensure_factor_set(FactorSet<T> &fs)
         F.aptr = d_aptr_ + C.o_np1;
         F.asrc = d_asrc_ + C.o_a;
         F.aslot = d_aslot_ + C.o_a;
-        F.ain = (std::is_same<T, __half>::value ? d_ains_ : d_ain_) + C.aoff;   // FP16: Ruiz-scaled
+        // FP16 reads the Ruiz-scaled copy; a factor-only shift also factors that copy
+        const bool own_copy = std::is_same<T, __half>::value || opt_.factor_shift > 0.0;
+        F.ain = (own_copy ? d_ains_ : d_ain_) + C.aoff;
         F.relp = d_relp_ + C.o_np1;
         F.rel = d_rel_ + C.o_rel;
         F.snend = d_snend_ + C.o_n;
@@ -352,6 +354,7 @@ size_t Batch::estimate_device_bytes() const
     b += 8 * (slices + nc) + 8 * n + 8 * sell;                            // SELL structure
     b += 12 * n + 8 * nnz + 12 * sell;                                    // flags, values
     b += 16 * nnz + 24 * n + 16 * (nc + 1);                               // Ruiz: row/col, scaled values, scales
+    if (opt_.factor_shift > 0.0) b += 8 * n;                               // shifted diagonal entries
     b += 8 * T2 + 16 * T + 65536.0 * nc;                                   // dense block work
     b += 24 * n + 8 * slices + 16 * n;                                    // task lists, row packets
     b += 64 * n + 8 * (2 * m + 1) * n;                                    // vectors, Krylov bases V and Z
@@ -540,6 +543,23 @@ int Batch::upload_all()
         d_rmx_ = (unsigned long long *)dalloc(ntot_ * sizeof(unsigned long long));
         if (!d_arow_ || !d_acol_ || !d_raoff_ || !d_rvoff_ || !d_ains_ || !d_rscale_ || !d_cscale_ || !d_rmx_)
             return LSP_E_ALLOC;
+        if (opt_.factor_shift < 0.0) return LSP_E_ARG;
+        if (opt_.factor_shift > 0.0) {
+            // the first value entry of each diagonal, in caller order
+            // (test_lu_gpu_f32.cu:356-361 stops at the first match per row)
+            std::vector<long> sidx;
+            for (int m = 0; m < nc; m++) {
+                const CircuitHost &C = circ_[m];
+                std::vector<char> seen(C.n, 0);
+                for (long e = 0; e < C.nnz; e++)
+                    if (C.row[e] == C.col[e] && !seen[C.row[e]]) {
+                        seen[C.row[e]] = 1;
+                        sidx.push_back(C.aoff + e);
+                    }
+            }
+            nsidx_ = (long)sidx.size();
+            if (nsidx_ && !(d_sidx_ = upload(sidx))) return LSP_E_ALLOC;
+        }
     }
     d_sval64_ = (double *)dalloc(nsell_tot_ * sizeof(double));
     d_sval32_ = (float *)dalloc(nsell_tot_ * sizeof(float));
@@ -789,6 +809,13 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
 // The refactor launch sequence of one precision (no host synchronisation).
 int Batch::launch_refactor_prec(int prec, const int *d_active)
 {
+    const bool shift = opt_.factor_shift > 0.0;
+    if (shift && (prec == LSP_PREC_FP32 || prec == LSP_PREC_FP64)) {
+        // the factored copy: caller values with the diagonal shifted
+        int rc = check(k_copy(d_ain_, d_ains_, nnz_tot_, stream_), "shift copy");
+        if (!rc) rc = check(launch_shift(d_ain_, d_ains_, d_sidx_, nsidx_, opt_.factor_shift, stream_), "shift");
+        if (rc) return rc;
+    }
     if (prec == LSP_PREC_FP32) return refactor_t(f32_, d_active);
     if (prec == LSP_PREC_FP64) return refactor_t(f64_, d_active);
     if (prec == LSP_PREC_FP16) {
@@ -799,6 +826,8 @@ int Batch::launch_refactor_prec(int prec, const int *d_active)
         R.aoff = d_raoff_; R.voff = d_rvoff_; R.rs = d_rscale_; R.cs = d_cscale_; R.mx = d_rmx_;
         R.active = d_active; R.sweeps = opt_.ruiz_sweeps > 0 ? opt_.ruiz_sweeps : 30;
         int rc = check(launch_ruiz(R, (int)circ_.size(), stream_), "ruiz");
+        if (!rc && shift)     // the shift is relative, so it commutes with the diagonal scaling
+            rc = check(launch_shift(d_ains_, d_ains_, d_sidx_, nsidx_, opt_.factor_shift, stream_), "shift");
         return rc ? rc : refactor_t(f16_, d_active);
     }
     return LSP_E_ARG;

@@ -135,6 +135,8 @@ private:
     int *d_arow_ = nullptr, *d_acol_ = nullptr;
     long *d_raoff_ = nullptr, *d_rvoff_ = nullptr;
     double *d_ains_ = nullptr, *d_rscale_ = nullptr, *d_cscale_ = nullptr;
+    long *d_sidx_ = nullptr;             // factor-only shift: first diagonal entry per row
+    long nsidx_ = 0;
     unsigned long long *d_rmx_ = nullptr;
     double *d_ain_ = nullptr, *d_sval64_ = nullptr;
     float *d_sval32_ = nullptr;

@@ -137,6 +137,12 @@ int main(int argc, char **argv)
            M.n, M.nnz, pin_val, flt ? flt : "-", max_n_st, reps);
     lsp_flt_pin(&M, flt, pin_val);
     const int n = M.n;
+    // every entry must index the n x n matrix before it reaches a kernel
+    for (long i = 0; i < M.nnz; i++)
+        if (M.row[i] < 1 || M.row[i] > n || M.col[i] < 1 || M.col[i] > n) {
+            fprintf(stderr, "entry %ld (%d, %d) outside 1..%d\n", i + 1, M.row[i], M.col[i], n);
+            return 1;
+        }
 
     // COLAMD on the 0-based pattern
     std::vector<int> r0(M.nnz), c0(M.nnz), q(n);

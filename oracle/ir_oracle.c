@@ -26,6 +26,11 @@
  *                     every sweep, so its returned R, C do not describe the
  *                     matrix it factors; this restatement keeps
  *                     A_s = diag(R) A diag(C) exact.
+ *   orc_diag_shift    bench/test_lu_gpu_f32.cu:354-364 and
+ *                     bench/test_lu_gpu_ir64.cu:206-212 (factor-only shift: for
+ *                     each row i, the first COO entry in file order with
+ *                     row == col == i becomes a + shift*|a|; later duplicates and
+ *                     every other entry are copied unchanged)
  *   orc_sell_order    src/sellc_spmv.cu:199-257 (rows sorted by length,
  *                     descending, by counting sort -- equal lengths come out in
  *                     reverse index order; slice length = max row length in the
@@ -197,6 +202,23 @@ int orc_gmres_ir(int n, long nnz, const int *row1, const int *col1,
     free(V); free(H); free(cs); free(sn); free(g); free(yv);
     free(w); free(z); free(r);
     return rel <= outer_tol ? 0 : 1;
+}
+
+void orc_diag_shift(int n, long nnz, const int *row1, const int *col1,
+                    const double *val, double shift, double *out)
+{
+    /* the reference scans the entries once per row i and stops at the first
+     * (i, i) entry; one pass with a per-row "seen" mark visits the same entry */
+    char *seen = (char *)calloc((size_t)n + 1, 1);
+    for (long e = 0; e < nnz; e++) {
+        out[e] = val[e];
+        const int i = row1[e];
+        if (i == col1[e] && i >= 1 && i <= n && !seen[i]) {
+            seen[i] = 1;
+            out[e] = val[e] + shift * fabs(val[e]);
+        }
+    }
+    free(seen);
 }
 
 void orc_ruiz(int n, long nnz, const int *row1, const int *col1, double *val,

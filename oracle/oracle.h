@@ -57,6 +57,8 @@ int  orc_gmres_ir(int n, long nnz, const int *row1, const int *col1,
                   int max_outer, int m, double inner_tol, double outer_tol,
                   double *x, int *total_inner, int *outer_done,
                   double *final_res);
+void orc_diag_shift(int n, long nnz, const int *row1, const int *col1,
+                    const double *val, double shift, double *out);
 void orc_ruiz(int n, long nnz, const int *row1, const int *col1, double *val,
               int sweeps, double *R, double *C);
 long orc_sell_order(int n, const int *rowptr, int C, int *order, int *lens,

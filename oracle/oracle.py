@@ -62,6 +62,7 @@ def lib():
                                    ip, ip, C.c_int, C.c_int, C.c_double, C.c_double, dp, ip, ip,
                                    dp]
         L.orc_ruiz.argtypes = [C.c_int, C.c_long, ip, ip, dp, C.c_int, dp, dp]
+        L.orc_diag_shift.argtypes = [C.c_int, C.c_long, ip, ip, dp, C.c_double, dp]
         L.orc_sell_order.restype = C.c_long
         L.orc_sell_order.argtypes = [C.c_int, ip, C.c_int, ip, ip, ip]
         _lib = L
@@ -221,6 +222,17 @@ def gmres_ir(n, row, col, val, b, F32, q, max_outer=7, m=50, inner_tol=1e-10, ou
                             _p(F32["pinv"]), _p(qa), max_outer, m, inner_tol, outer_tol, _p(x),
                             C.byref(ti), C.byref(od), C.byref(fr))
     return x, dict(rc=rc, inner=ti.value, outer=od.value, res=fr.value)
+
+
+def diag_shift(n, row, col, val, shift):
+    """Factor-only diagonal shift (orc_diag_shift): the first (i, i) entry of
+    each row, in entry order, becomes a + shift*|a|."""
+    r1 = np.ascontiguousarray(np.asarray(row) + 1, np.int32)
+    c1 = np.ascontiguousarray(np.asarray(col) + 1, np.int32)
+    v = np.ascontiguousarray(val, np.float64)
+    out = np.empty_like(v)
+    lib().orc_diag_shift(n, len(v), _p(r1), _p(c1), _p(v), shift, _p(out))
+    return out
 
 
 def ruiz(n, row, col, val, sweeps=30):

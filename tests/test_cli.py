@@ -1,0 +1,54 @@
+"""The two stage CLIs on the device: mp_lu_bench runs the reference-shaped
+kernels of lu_gpu.h / lu_gpu_par.h (<<<1,1>>> fp64 and fp32 factor and
+trisolve, <<<1,256>>> single-workgroup factor and trisolves) and the
+device-resident path on config 0, checking each against the CPU cpu_lu_ref
+solution; mp_spmv_bench runs CSR and SELL-C-64 SpMV on the same matrix.
+
+Tolerances: fp64 factors give the CPU solution to 1e-10 relative L2; the fp32
+single-thread factors (fp32 work vector, as the reference kernel) only to
+fp32 rounding times the growth, so their direct solve is held to 1e-3; the
+GMRES-IR result to the 1e-6 pass line.  SELL fp64 rows equal CSR fp64 rows
+bit for bit (same per-row order); fp32 values stay within 1e-4 of the
+fp64 product relative to max|y| (value rounding only).
+"""
+import json
+import os
+import re
+import subprocess
+
+import pytest
+
+import luspice_amd as L
+from luspice_amd import s0tools as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(args, timeout):
+    print("case:", " ".join(os.path.basename(a) for a in args), flush=True)
+    p = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=timeout)
+    print(p.stdout[-3000:], flush=True)
+    assert p.returncode == 0, p.stdout[-2000:]
+    rec = [ln for ln in p.stdout.splitlines() if ln.startswith("JSON:")]
+    return p.stdout, json.loads(rec[-1][5:])
+
+
+def test_mp_lu_bench_reference_kernels_c0(tmp_path):
+    c = L.mna("c0", seed=0)
+    mtx, rhs = S.write_s0(str(tmp_path / "c0"), c.n, c.row, c.col, c.val, c.rhs)
+    out, rec = _run([os.path.join(S.BIN, "mp_lu_bench"), mtx, rhs, "0", "0", "15000", "2"], 110)
+    errs = dict(re.findall(r"^(fp64 ST|fp32 ST|fp64 PAR|fp32 PAR trisolve):.*L2\(vs CPU\)=([0-9.e+-]+)",
+                           out, re.M))
+    assert set(errs) == {"fp64 ST", "fp32 ST", "fp64 PAR", "fp32 PAR trisolve"}, out
+    assert float(errs["fp64 ST"]) <= 1e-10 and float(errs["fp64 PAR"]) <= 1e-10
+    assert float(errs["fp32 ST"]) <= 1e-3 and float(errs["fp32 PAR trisolve"]) <= 1e-3
+    assert rec["lnnz64"] == rec["lnnz_cpu"] and rec["unnz64"] == rec["unnz_cpu"]
+    assert rec["ir_rel_err"] <= 1e-6
+
+
+def test_mp_spmv_bench_c0(tmp_path):
+    c = L.mna("c0", seed=0)
+    mtx, _ = S.write_s0(str(tmp_path / "c0"), c.n, c.row, c.col, c.val, c.rhs)
+    _, rec = _run([os.path.join(S.BIN, "mp_spmv_bench"), mtx, "20", "2"], 60)
+    assert rec["n"] == c.n and rec["nnz"] == c.nnz
+    assert rec["sell_csr_rows_differing"] == 0 and rec["rel_err_fp32"] <= 1e-4

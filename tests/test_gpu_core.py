@@ -42,6 +42,52 @@ def test_refactor_fp64_matches_oracle_factors(c0, c0_q):
         np.testing.assert_allclose(fu, R["Ux"][b][oj], rtol=1e-10, atol=1e-12 * np.abs(R["Ux"][b]).max())
 
 
+def test_factor_shift_matches_oracle(c0, c0_q):
+    """Factor-only diagonal shift: fp64 factors equal the oracle's factors of
+    the shifted copy (a duplicate diagonal entry shows that only the first one
+    is shifted), the direct solve inverts the shifted matrix, and GMRES-IR on
+    the unshifted values still reaches the fp64 solution."""
+    print("case: c0 factor shift", flush=True)
+    s = 1e-3
+    has = np.zeros(c0.n, bool)
+    has[c0.row[c0.row == c0.col]] = True
+    i_dup = int(np.flatnonzero(has)[11])
+    first = int(np.flatnonzero((c0.row == i_dup) & (c0.col == i_dup))[0])
+    row = np.append(c0.row, i_dup).astype(np.int32)
+    col = np.append(c0.col, i_dup).astype(np.int32)
+    val = np.append(c0.val, 2.0 * c0.val[first])
+    B = L.Batch(L.Circuit(c0.n, row, col, val, c0.rhs, q=c0_q), factor_shift=s)
+    rep = B.refactor("fp64")
+    assert rep["rc"] == 0, rep
+    F = B.export_factors(0)
+    R = O.lu_factor(c0.n, row, col, O.diag_shift(c0.n, row, col, val, s), c0_q)
+    assert np.array_equal(F["pinv"], R["pinv"])
+    assert np.array_equal(F["Lp"], R["Lp"]) and np.array_equal(F["Up"], R["Up"])
+    for k in range(c0.n):
+        a, b = slice(R["Lp"][k], R["Lp"][k + 1]), slice(R["Up"][k], R["Up"][k + 1])
+        fl = F["Lx"][a][np.argsort(F["Li"][a], kind="stable")]
+        fu = F["Ux"][b][np.argsort(F["Ui"][b], kind="stable")]
+        rl = R["Lx"][a][np.argsort(R["Li"][a], kind="stable")]
+        ru = R["Ux"][b][np.argsort(R["Ui"][b], kind="stable")]
+        np.testing.assert_allclose(fl, rl, rtol=1e-10, atol=1e-12 * np.abs(rl).max())
+        np.testing.assert_allclose(fu, ru, rtol=1e-10, atol=1e-12 * np.abs(ru).max())
+    # no duplicates from here: the solve of the shifted matrix, then GMRES-IR
+    B2 = L.Batch(L.Circuit(c0.n, c0.row, c0.col, c0.val, c0.rhs, q=c0_q), factor_shift=s)
+    assert B2.refactor("fp64")["rc"] == 0
+    vs = O.diag_shift(c0.n, c0.row, c0.col, c0.val, s)
+    v = np.random.default_rng(9).uniform(-1, 1, c0.n)
+    As_v = np.zeros(c0.n)
+    np.add.at(As_v, c0.row, vs * v[c0.col])
+    assert O.rel_l2(B2.trisolve(As_v), v) < 1e-9
+    xo = _oracle_x(c0, c0_q)
+    assert O.rel_l2(B2.trisolve(c0.rhs), xo) > 1e-6      # the shift took effect
+    B3 = L.Batch(L.Circuit(c0.n, c0.row, c0.col, c0.val, c0.rhs, q=c0_q), factor_shift=1e-7)
+    assert B3.refactor("fp32")["rc"] == 0
+    x, sr = B3.solve(c0.rhs)
+    print("shift 1e-7 fp32 GMRES-IR", O.rel_l2(x, xo), sr, flush=True)
+    assert sr["rc"] == 0 and O.rel_l2(x, xo) <= 1e-6
+
+
 def test_trisolve_fp64_direct(c0, c0_q):
     print("case: c0 trisolve fp64", flush=True)
     B = L.Batch(L.Circuit(c0.n, c0.row, c0.col, c0.val, c0.rhs, q=c0_q))

@@ -154,6 +154,52 @@ def test_oracle_ruiz_invariants():
     np.testing.assert_allclose(Cs * xs, np.linalg.solve(A, c.rhs), rtol=1e-8, atol=1e-12)
 
 
+def test_oracle_diag_shift_pins():
+    """Factor-only shift (bench/test_lu_gpu_f32.cu:354-364): zero shift is the
+    identity; the first (i, i) entry of each row moves away from zero by the
+    relative amount s with its sign kept; a later duplicate of that diagonal
+    and every off-diagonal entry are untouched; and factors of the shifted
+    copy still precondition the unshifted system to the fp64 solution."""
+    c = L.mna("c0", seed=0)
+    # a duplicate diagonal entry appended for row 7 (rows that already have one)
+    has = np.zeros(c.n, bool)
+    has[c.row[c.row == c.col]] = True
+    i_dup = int(np.flatnonzero(has)[7])
+    row = np.append(c.row, i_dup).astype(np.int32)
+    col = np.append(c.col, i_dup).astype(np.int32)
+    val = np.append(c.val, 0.5)
+    assert np.array_equal(O.diag_shift(c.n, row, col, val, 0.0), val)
+    s = 1e-3
+    out = O.diag_shift(c.n, row, col, val, s)
+    changed = np.flatnonzero(out != val)
+    # exactly one entry per row that has a diagonal, and it is the first one
+    assert len(changed) == int(has.sum())
+    assert np.all(row[changed] == col[changed])
+    first = {}
+    for e in range(len(row)):
+        if row[e] == col[e] and row[e] not in first:
+            first[int(row[e])] = e
+    assert sorted(first.values()) == changed.tolist()
+    assert out[-1] == val[-1]                   # the later duplicate
+    np.testing.assert_allclose(np.abs(out[changed]), (1 + s) * np.abs(val[changed]), rtol=1e-15)
+    assert np.all(np.sign(out[changed]) == np.sign(val[changed]))
+    # preconditioning only: fp32 factors of the shifted copy (same q) drive the
+    # restated GMRES-IR to the unshifted fp64 solution
+    q = L.colamd(c)
+    vs = O.diag_shift(c.n, c.row, c.col, c.val, 1e-7)
+    F = O.lu_factor(c.n, c.row, c.col, vs, q)
+    F32 = dict(F, Lx=F["Lx"].astype(np.float32), Ux=F["Ux"].astype(np.float32))
+    x, info = O.gmres_ir(c.n, c.row, c.col, c.val, c.rhs, F32, q)
+    rc, xo, _, _ = O.cpu_lu_ref(c.n, c.row, c.col, c.val, q, c.rhs)
+    assert rc == -1 and info["rc"] == 0
+    assert O.rel_l2(x, xo) <= 1e-6
+    # and a shift large enough to matter makes the direct solve miss
+    Fb = O.lu_factor(c.n, c.row, c.col, O.diag_shift(c.n, c.row, c.col, c.val, 1e-2), q)
+    xd = O.trisolve_f32(c.n, dict(Fb, Lx=Fb["Lx"].astype(np.float32), Ux=Fb["Ux"].astype(np.float32)),
+                        q, c.rhs)
+    assert O.rel_l2(xd, xo) > 1e-6
+
+
 def test_oracle_sell_order_rule():
     rng = np.random.default_rng(2)
     lens = rng.integers(0, 9, 200)

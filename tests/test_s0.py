@@ -1,0 +1,162 @@
+"""CPU tests of the s0 file path: the product's C readers (spmv.h
+read_s0_to_csr / read_rhs, the CLI reader lsp_s0_read and the .flt pin),
+the Python writer and Matrix Market converter, and the stage-benchmark suite
+runner's JSON aggregation (driven with stand-in CLIs that only print).
+
+The reference holds no s0 fixture files; the checks are the format's own
+rules (src/spmv.cu:42-145, src/mp/mp_lu_bench.cu:46-111,
+scripts/run_gpu_bench.sh:19-83): 17-digit values read back bit for bit,
+rows in file order, RHS zero or missing -> 1.0, non-"A" lines skipped,
+duplicates summed and explicit zeros dropped by the converter, b = A * ones.
+"""
+import ctypes as C
+import json
+import os
+import stat
+
+import numpy as np
+import pytest
+
+import luspice_amd as L
+from luspice_amd import _native as N
+from luspice_amd import s0tools as S
+
+
+class CSRMatrix(C.Structure):
+    _fields_ = [("n", C.c_int), ("nnz", C.c_longlong), ("rowptr", C.POINTER(C.c_int)),
+                ("colidx", C.POINTER(C.c_int)), ("values", C.POINTER(C.c_double))]
+
+
+class S0(C.Structure):
+    _fields_ = [("n", C.c_int), ("nnz", C.c_long), ("row", C.POINTER(C.c_int)),
+                ("col", C.POINTER(C.c_int)), ("val", C.POINTER(C.c_double)),
+                ("b", C.POINTER(C.c_double))]
+
+
+def _read_csr(path):
+    A = CSRMatrix()
+    assert N.lib.read_s0_to_csr(path.encode(), C.byref(A)) == 0
+    n, nnz = A.n, A.nnz
+    out = (n, np.ctypeslib.as_array(A.rowptr, (n + 1,)).copy(),
+           np.ctypeslib.as_array(A.colidx, (nnz,)).copy(), np.ctypeslib.as_array(A.values, (nnz,)).copy())
+    N.lib.csr_free(C.byref(A))
+    return out
+
+
+def _read_cli(mtx, rhs, flt=None, pin=0.0):
+    M = S0()
+    assert N.lib.lsp_s0_read(mtx.encode(), rhs.encode(), C.byref(M)) == 0
+    added = N.lib.lsp_flt_pin(C.byref(M), flt.encode() if flt else None, C.c_double(pin))
+    nnz = M.nnz
+    out = dict(n=M.n, row=np.ctypeslib.as_array(M.row, (nnz,)).copy() - 1,
+               col=np.ctypeslib.as_array(M.col, (nnz,)).copy() - 1,
+               val=np.ctypeslib.as_array(M.val, (nnz,)).copy(),
+               b=np.ctypeslib.as_array(M.b, (M.n + 1,)).copy()[1:], added=added)
+    N.lib.lsp_s0_free(C.byref(M))
+    return out
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _sigs():
+    N.lib.read_s0_to_csr.argtypes = [C.c_char_p, C.POINTER(CSRMatrix)]
+    N.lib.read_s0_to_csr.restype = C.c_int
+    N.lib.csr_free.argtypes = [C.POINTER(CSRMatrix)]
+    N.lib.lsp_s0_read.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(S0)]
+    N.lib.lsp_s0_read.restype = C.c_int
+    N.lib.lsp_s0_free.argtypes = [C.POINTER(S0)]
+    N.lib.lsp_flt_pin.argtypes = [C.POINTER(S0), C.c_char_p, C.c_double]
+    N.lib.lsp_flt_pin.restype = C.c_int
+
+
+def test_write_then_read_bitwise(tmp_path):
+    c = L.mna("c0", seed=0)
+    mtx, rhs = S.write_s0(str(tmp_path / "c0"), c.n, c.row, c.col, c.val, c.rhs)
+    n, rp, ci, v = _read_csr(mtx)
+    assert n == c.n
+    order = np.argsort(c.row, kind="stable")            # rows in file order inside each row
+    assert np.array_equal(np.diff(rp), np.bincount(c.row, minlength=c.n))
+    assert np.array_equal(ci, c.col[order]) and np.array_equal(v, c.val[order])
+    M = _read_cli(mtx, rhs)
+    assert np.array_equal(M["row"], c.row) and np.array_equal(M["col"], c.col)
+    assert np.array_equal(M["val"], c.val)
+    b = c.rhs.copy()
+    b[b == 0.0] = 1.0
+    assert np.array_equal(M["b"], b)
+    n2, r2, c2, v2, b2 = S.read_s0(mtx, rhs)
+    assert n2 == c.n and np.array_equal(v2, c.val) and np.array_equal(b2, b)
+
+
+def test_reader_quirks_and_flt_pin(tmp_path):
+    mtx = tmp_path / "q.mtx"
+    mtx.write_text("n 3\nA 1 1 2.0\n% comment\nA 2 2 0.5\nA 3 1 -1.0\nX 1 1 9\nA 3 3 4.0\nA 1 2 1.5\n")
+    rhs = tmp_path / "q.rhs"
+    rhs.write_text("n 3\nRHS 1 0.0\nRHS 3 -2.5\n")          # row 1 zero, row 2 missing
+    flt = tmp_path / "q.flt"
+    flt.write_text("F 2\nF 3\nF 1\n")
+    n, rp, ci, v = _read_csr(str(mtx))
+    assert n == 3 and rp.tolist() == [0, 2, 3, 5]
+    assert ci.tolist() == [0, 1, 1, 0, 2] and v.tolist() == [2.0, 1.5, 0.5, -1.0, 4.0]
+    M = _read_cli(str(mtx), str(rhs))
+    assert M["b"].tolist() == [1.0, 1.0, -2.5]
+    assert M["row"].tolist() == [0, 1, 2, 2, 0]
+    # pin: added to the existing diagonal entries, appended where missing
+    mtx2 = tmp_path / "p.mtx"
+    mtx2.write_text("n 3\nA 1 1 2.0\nA 3 1 -1.0\nA 3 3 4.0\nA 3 3 7.0\n")
+    P = _read_cli(str(mtx2), str(rhs), str(flt), 1e-3)
+    assert P["added"] == 1                                 # row 2 had no diagonal
+    assert P["val"].tolist() == [2.0 + 1e-3, -1.0, 4.0 + 1e-3, 7.0, 1e-3]
+    assert P["row"].tolist() == [0, 2, 2, 2, 1] and P["col"].tolist() == [0, 0, 2, 2, 1]
+    # pin <= 0 or a missing file changes nothing
+    Z = _read_cli(str(mtx2), str(rhs), str(tmp_path / "none.flt"), 1.0)
+    assert Z["added"] == 0 and Z["val"].tolist() == [2.0, -1.0, 4.0, 7.0]
+
+
+def test_mtx_to_s0_converter(tmp_path):
+    mm = tmp_path / "m.mtx"
+    # general coordinate file with a duplicate (2,1) and an explicit zero (3,3)
+    mm.write_text("%%MatrixMarket matrix coordinate real general\n3 3 6\n"
+                  "2 1 1.25\n1 1 3.0\n3 3 0.0\n2 1 0.5\n3 2 -2.0\n1 3 0.1\n")
+    out = tmp_path / "s0"
+    mtx, rhs = S.mtx_to_s0(str(mm), str(out))
+    n, row, col, val, b = S.read_s0(mtx, rhs)
+    assert n == 3
+    assert row.tolist() == [0, 0, 1, 2] and col.tolist() == [0, 2, 0, 1]
+    assert val.tolist() == [3.0, 0.1, 1.75, -2.0]
+    assert b.tolist() == [3.1, 1.75, -2.0]
+    # symmetric storage is expanded to both triangles
+    sym = tmp_path / "s.mtx"
+    sym.write_text("%%MatrixMarket matrix coordinate real symmetric\n2 2 2\n1 1 4.0\n2 1 -1.0\n")
+    mtx2, _ = S.mtx_to_s0(str(sym), str(out))
+    n, row, col, val, _ = S.read_s0(mtx2)
+    assert sorted(zip(row.tolist(), col.tolist(), val.tolist())) == [(0, 0, 4.0), (0, 1, -1.0), (1, 0, -1.0)]
+
+
+def _stub(path, body):
+    path.write_text("#!/bin/sh\n" + body)
+    path.chmod(path.stat().st_mode | stat.S_IEXEC)
+
+
+def test_suite_runner_aggregates(tmp_path):
+    b = tmp_path / "bin"
+    b.mkdir()
+    _stub(b / "mp_lu_bench", 'case "$1" in\n'
+          '  *good*) echo "== x =="; echo \'JSON:{"matrix":"good","n":3}\' ;;\n'
+          '  *bad*) echo \'JSON:{"matrix":"bad",\' ;;\n'
+          '  *slow*) sleep 5 ;;\n'
+          '  *) exit 1 ;;\nesac\n')
+    _stub(b / "mp_spmv_bench", 'echo \'JSON:{"matrix":"s","t_csr_fp64_ms":0.01}\'\n')
+    out = tmp_path / "r.json"
+    recs = S.run_suite(str(tmp_path), ["good", "bad", "slow", "gone"], str(out), bin_dir=str(b),
+                       lu_timeout=1, spmv_timeout=5)
+    assert json.loads(out.read_text()) == recs
+    st = {r["matrix"]: r["lu_status"] for r in recs}
+    assert st["good"] == "ok" and st["bad"].startswith("parse-fail")
+    assert st["slow"] == "timeout-or-no-output" and st["gone"] == "exit 1"
+    assert recs[0]["lu"] == {"matrix": "good", "n": 3}
+    assert all(r["spmv_status"] == "ok" and r["spmv"]["t_csr_fp64_ms"] == 0.01 for r in recs)
+
+
+def test_clis_built():
+    for name in ("mp_lu_bench", "mp_spmv_bench"):
+        p = os.path.join(S.BIN, name)
+        assert os.path.isfile(p) and os.access(p, os.X_OK), p
