@@ -78,10 +78,11 @@ __device__ __forceinline__ double max16(double v)
     return v;
 }
 
-__global__ void __launch_bounds__(256) dense_diag_kernel(const DenseJob *jobs, const int *active, int p)
+// The block body, run by all 256 threads of a workgroup: panel p of job J,
+// its input read from src (column-major, leading dimension lds: the block in
+// S itself, or a copy in LDS that the caller has made visible with a barrier).
+__device__ __forceinline__ void diag_block(const DenseJob &J, int p, const double *src, size_t lds)
 {
-    const DenseJob J = jobs[blockIdx.x];
-    if (p >= J.T || (active && !active[J.mat])) return;
     const int tid = threadIdx.x;
     const int r = tid & 15, s = tid >> 4;
     const int b = min(kNb, J.T - p);
@@ -97,7 +98,7 @@ __global__ void __launch_bounds__(256) dense_diag_kernel(const DenseJob *jobs, c
 #pragma unroll
         for (int bb = 0; bb < 4; bb++) {
             const int j = s + 16 * bb;
-            const double v = Sp[(size_t)min(j, b - 1) * ld + min(i, b - 1)];
+            const double v = src[(size_t)min(j, b - 1) * lds + min(i, b - 1)];
             A[a][bb] = (i < b && j < b) ? v : (i == j ? 1.0 : 0.0);
             X[a][bb] = 0.0;
         }
@@ -233,6 +234,13 @@ __global__ void __launch_bounds__(256) dense_diag_kernel(const DenseJob *jobs, c
         am = max16(am);
         if (r == 0 && j < b) atomicMax(&J.colmax[p + j], dbits(am));
     }
+}
+
+__global__ void __launch_bounds__(256) dense_diag_kernel(const DenseJob *jobs, const int *active, int p)
+{
+    const DenseJob J = jobs[blockIdx.x];
+    if (p >= J.T || (active && !active[J.mat])) return;
+    diag_block(J, p, J.S + (size_t)p * J.T + p, (size_t)J.T);
 }
 
 // ---------------------------------------------------------------------------
@@ -441,7 +449,17 @@ __global__ void __launch_bounds__(256) dense_gemm_kernel(const DenseJob *jobs, c
         const int e = tid + 256 * u;
         const int j = e >> 6, i = e & 63;
         const int rr = r0 + i, c = c0 + j;
-        if (rr < J.T && c < J.T) J.S[(size_t)c * ld + rr] = cv[u] - Cs[j * kBsLd + i];
+        const double v = cv[u] - Cs[j * kBsLd + i];
+        if (rr < J.T && c < J.T) J.S[(size_t)c * ld + rr] = v;
+        if (tile == 0) Cs[j * kBsLd + i] = v;           // element read by this thread alone above
+    }
+    // Look-ahead: tile 0 is the next panel's diagonal block, now final (every
+    // earlier panel's update has reached it), so this workgroup factors it
+    // while the other tiles of this update are still running; the next
+    // panel's TRSM launch follows this kernel.
+    if (tile == 0) {
+        __syncthreads();
+        diag_block(J, q0, Cs, kBsLd);
     }
 }
 
@@ -640,11 +658,16 @@ hipError_t dense_set_lds()
                                (int)kGemmLds);
 }
 
+// Panel 0's diagonal block has a launch of its own; every later diagonal
+// block is factored by tile 0 of the preceding trailing update.  For a job
+// whose T is below maxT the update kernel returns before its tiles once
+// p + 64 >= T, so each of its panels is factored exactly once.
 hipError_t launch_dense_lu(const DenseJob *jobs, int njobs, int maxT, const int *active, hipStream_t st)
 {
+    if (maxT <= 0) return hipSuccess;
+    hipLaunchKernelGGL(dense_diag_kernel, dim3(njobs), dim3(256), 0, st, jobs, active, 0);
     for (int p = 0; p < maxT; p += kNb) {
         const int rem = maxT - p - min(kNb, maxT - p);
-        hipLaunchKernelGGL(dense_diag_kernel, dim3(njobs), dim3(256), 0, st, jobs, active, p);
         if (rem > 0) {
             const int nt = (rem + kNb - 1) / kNb;
             hipLaunchKernelGGL(dense_trsm_kernel, dim3(2 * nt, njobs), dim3(256), kGemmLds, st, jobs, active, p);
