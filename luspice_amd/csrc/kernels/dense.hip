@@ -64,8 +64,12 @@ __device__ __forceinline__ double bcast(double v, int src)
 //     same row operations applied to I build L^-1 (Gauss-Jordan): row c of
 //     L^-1 has no entry right of c and the trailing part of A none left of
 //     c, so one published row carries both.
-//   * U^-1 by rows from the bottom: row k is finalised from its accumulated
-//     sums, then every row above adds U(i,k) times it.
+//   * U^-1 in the same steps, by columns: V = U^-1 solves V U = I, so
+//     V(i,c) = (delta_ic - Y(i,c)) / U(c,c) with Y(i,j) = sum_{k<c} V(i,k)
+//     U(k,j).  Row c of U is final (and published) at step c, so the owners
+//     of column c publish delta - Y(:,c) beside column c of A, every thread
+//     scales it by the pivot reciprocal and adds V(i,c) U(c,j) into Y(i,j),
+//     j > c.  No second pass of 64 barrier steps is needed.
 // Outputs: L\U over the block in S, L^-1 (unit lower) and U^-1 (upper) into
 // the job's workspace W (column-major, leading dimension 64), and the column
 // maxima of |L| for the pivot test.
@@ -88,10 +92,9 @@ __device__ __forceinline__ void diag_block(const DenseJob &J, int p, const doubl
     const int b = min(kNb, J.T - p);
     const size_t ld = (size_t)J.T;
     double *Sp = J.S + (size_t)p * ld + p;
-    __shared__ double bc[2][3][kNb];                   // [parity][row c of A | row c of L^-1 | column c]
+    __shared__ double bc[2][4][kNb];                   // [parity][row c of A | row c of L^-1 | column c | delta - Y(:,c)]
     __shared__ double rpiv[2];
-    __shared__ double dg[kNb];
-    double A[4][4], X[4][4];
+    double A[4][4], X[4][4], Y[4][4];
 #pragma unroll
     for (int a = 0; a < 4; a++) {
         const int i = r + 16 * a;
@@ -101,6 +104,7 @@ __device__ __forceinline__ void diag_block(const DenseJob &J, int p, const doubl
             const double v = src[(size_t)min(j, b - 1) * lds + min(i, b - 1)];
             A[a][bb] = (i < b && j < b) ? v : (i == j ? 1.0 : 0.0);
             X[a][bb] = 0.0;
+            Y[a][bb] = 0.0;
         }
     }
     // The 16-step blocks are unrolled (register indices c/16 are compile-time
@@ -127,11 +131,15 @@ __device__ __forceinline__ void diag_block(const DenseJob &J, int p, const doubl
         }
         if (s == (c & 15)) {
 #pragma unroll
-            for (int a = 0; a < 4; a++) bc[par][2][r + 16 * a] = A[a][ca];
+            for (int a = 0; a < 4; a++) {
+                const int i = r + 16 * a;
+                bc[par][2][i] = A[a][ca];
+                bc[par][3][i] = i < c ? -Y[a][ca] : (i == c ? 1.0 : 0.0);
+            }
         }
         __syncthreads();
         const double rp = rpiv[par];
-        double ur[4], xr[4], l[4];
+        double ur[4], xr[4], l[4], vc[4];
 #pragma unroll
         for (int bb = 0; bb < 4; bb++) {
             const int j = s + 16 * bb;
@@ -142,6 +150,7 @@ __device__ __forceinline__ void diag_block(const DenseJob &J, int p, const doubl
         for (int a = 0; a < 4; a++) {
             const int i = r + 16 * a;
             l[a] = i > c ? bc[par][2][i] * rp : 0.0;
+            vc[a] = bc[par][3][i] * rp;                  // V(i, c); zero below the diagonal
         }
         // next pivot A(c+1, c+1): register (cn, cn) of thread ((c+1) % 16, (c+1) % 16)
         if (c + 1 < kNb && r == ((c + 1) & 15) && s == ((c + 1) & 15)) {
@@ -155,6 +164,7 @@ __device__ __forceinline__ void diag_block(const DenseJob &J, int p, const doubl
             for (int bb = 0; bb < 4; bb++) {
                 A[a][bb] = fma(-l[a], ur[bb], A[a][bb]);
                 X[a][bb] = fma(-l[a], xr[bb], X[a][bb]);
+                Y[a][bb] = fma(vc[a], ur[bb], Y[a][bb]);
             }
         if (s == (c & 15)) {
 #pragma unroll
@@ -163,50 +173,9 @@ __device__ __forceinline__ void diag_block(const DenseJob &J, int p, const doubl
                     A[a][ca] = l[a];
                     X[a][ca] = -l[a];
                 }
+                Y[a][ca] = vc[a];
             }
         }
-    }
-    // U^-1: Y holds the running sums of a row until the row is final
-    if (r == s) {
-#pragma unroll
-        for (int a = 0; a < 4; a++) dg[r + 16 * a] = 1.0 / A[a][a];
-    }
-    double Y[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int bb = 0; bb < 4; bb++) Y[a][bb] = 0.0;
-    __syncthreads();
-#pragma unroll
-    for (int ka = 3; ka >= 0; ka--)
-#pragma unroll 1
-    for (int kl = 15; kl >= 0; kl--) {
-        const int k = 16 * ka + kl;
-        const int par = k & 1;
-        if (r == (k & 15)) {
-            const double rd = dg[k];
-#pragma unroll
-            for (int bb = 0; bb < 4; bb++) {
-                const int j = s + 16 * bb;
-                const double y = j >= k ? ((j == k ? 1.0 : 0.0) - Y[ka][bb]) * rd : 0.0;
-                Y[ka][bb] = y;
-                bc[par][0][j] = y;
-            }
-        }
-        if (s == (k & 15)) {
-#pragma unroll
-            for (int a = 0; a < 4; a++) bc[par][2][r + 16 * a] = A[a][ka];
-        }
-        __syncthreads();
-        double yr[4], u[4];
-#pragma unroll
-        for (int bb = 0; bb < 4; bb++) yr[bb] = bc[par][0][s + 16 * bb];
-#pragma unroll
-        for (int a = 0; a < 4; a++) u[a] = r + 16 * a < k ? bc[par][2][r + 16 * a] : 0.0;
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int bb = 0; bb < 4; bb++) Y[a][bb] = fma(u[a], yr[bb], Y[a][bb]);
     }
     double *W = J.W;
 #pragma unroll
