@@ -93,7 +93,6 @@ __device__ __forceinline__ void diag_block(const DenseJob &J, int p, const doubl
     const size_t ld = (size_t)J.T;
     double *Sp = J.S + (size_t)p * ld + p;
     __shared__ double bc[2][4][kNb];                   // [parity][row c of A | row c of L^-1 | column c | delta - Y(:,c)]
-    __shared__ double rpiv[2];
     double A[4][4], X[4][4], Y[4][4];
 #pragma unroll
     for (int a = 0; a < 4; a++) {
@@ -111,17 +110,25 @@ __device__ __forceinline__ void diag_block(const DenseJob &J, int p, const doubl
     // constants); the steps inside a block stay a loop.  Masks are applied to
     // the broadcast operands (row multipliers zero for rows <= c, row values
     // zero outside their columns), so each element costs one fma; column c is
-    // then overwritten with l and -l.  The reciprocal of the next pivot is
-    // taken one step ahead: its owner updates that element first and divides
-    // while the rest of its update runs, so no division sits between a
-    // step's barrier and the next.
-    if (tid == 0) rpiv[0] = 1.0 / A[0][0];
+    // then overwritten with l and -l.  Every LDS operand of a step is read
+    // before any is used (one LDS round trip per step).  The reciprocal of the
+    // next pivot is taken one step ahead by every thread: the owner of
+    // A(c+1, c+1) publishes it with step c's row and column, so after the
+    // barrier each thread forms fma(-A(c+1,c) / A(c,c), U(c,c+1), A(c+1,c+1))
+    // -- the owner's own update of that element, bit for bit -- and divides
+    // while its updates run; no division or extra LDS read sits between two
+    // barriers on one wave alone.
+    __shared__ double nxt[2];
+    if (tid == 0) nxt[1] = A[0][0];
+    __syncthreads();
+    double rp = 1.0 / nxt[1];
 #pragma unroll
     for (int ca = 0; ca < 4; ca++)
 #pragma unroll 1
     for (int cl = 0; cl < 16; cl++) {
         const int c = 16 * ca + cl;
         const int par = c & 1;
+        const int cn1 = ca < 3 ? ca + 1 : 3;
         if (r == (c & 15)) {
 #pragma unroll
             for (int bb = 0; bb < 4; bb++) {
@@ -137,34 +144,51 @@ __device__ __forceinline__ void diag_block(const DenseJob &J, int p, const doubl
                 bc[par][3][i] = i < c ? -Y[a][ca] : (i == c ? 1.0 : 0.0);
             }
         }
+        // A(c+1, c+1) before this step's update: register (cn, cn) of thread
+        // ((c+1) % 16, (c+1) % 16), cn = (c+1) / 16
+        if (c + 1 < kNb && r == ((c + 1) & 15) && s == ((c + 1) & 15))
+            nxt[par] = cl < 15 ? A[ca][ca] : A[cn1][cn1];
         __syncthreads();
-        const double rp = rpiv[par];
+        double rowv[4], xrow[4], colv[4], wv[4];
+#pragma unroll
+        for (int bb = 0; bb < 4; bb++) {
+            rowv[bb] = bc[par][0][s + 16 * bb];
+            xrow[bb] = bc[par][1][s + 16 * bb];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            colv[a] = bc[par][2][r + 16 * a];
+            wv[a] = bc[par][3][r + 16 * a];
+        }
+        const int cn = c + 1 < kNb ? c + 1 : c;
+        const double a_nn = nxt[par], l_n = bc[par][2][cn], u_n = bc[par][0][cn];
+        const double rpc = rp;
         double ur[4], xr[4], l[4], vc[4];
 #pragma unroll
         for (int bb = 0; bb < 4; bb++) {
             const int j = s + 16 * bb;
-            ur[bb] = j > c ? bc[par][0][j] : 0.0;
-            xr[bb] = j < c ? bc[par][1][j] : 0.0;
+            ur[bb] = j > c ? rowv[bb] : 0.0;
+            xr[bb] = j < c ? xrow[bb] : 0.0;
         }
 #pragma unroll
         for (int a = 0; a < 4; a++) {
             const int i = r + 16 * a;
-            l[a] = i > c ? bc[par][2][i] * rp : 0.0;
-            vc[a] = bc[par][3][i] * rp;                  // V(i, c); zero below the diagonal
+            l[a] = i > c ? colv[a] * rpc : 0.0;
+            vc[a] = wv[a] * rpc;                         // V(i, c); zero below the diagonal
         }
-        // next pivot A(c+1, c+1): register (cn, cn) of thread ((c+1) % 16, (c+1) % 16)
-        if (c + 1 < kNb && r == ((c + 1) & 15) && s == ((c + 1) & 15)) {
-            const int cn1 = ca < 3 ? ca + 1 : 3;
-            const double dn = cl < 15 ? fma(-l[ca], ur[ca], A[ca][ca]) : fma(-l[cn1], ur[cn1], A[cn1][cn1]);
-            rpiv[par ^ 1] = 1.0 / dn;
-        }
+        rp = 1.0 / fma(-(l_n * rpc), u_n, a_nn);          // unused after the last step
+        // Register blocks whose operand is zero for every thread at this step
+        // are skipped (ca is a compile-time index here): the multipliers l
+        // vanish in row blocks a < ca, the U^-1 column in a > ca, the U row in
+        // column blocks bb < ca, the L^-1 row in bb > ca.  fma(0, x, y) = y, so
+        // the result is unchanged.
 #pragma unroll
         for (int a = 0; a < 4; a++)
 #pragma unroll
             for (int bb = 0; bb < 4; bb++) {
-                A[a][bb] = fma(-l[a], ur[bb], A[a][bb]);
-                X[a][bb] = fma(-l[a], xr[bb], X[a][bb]);
-                Y[a][bb] = fma(vc[a], ur[bb], Y[a][bb]);
+                if (a >= ca && bb >= ca) A[a][bb] = fma(-l[a], ur[bb], A[a][bb]);
+                if (a >= ca && bb <= ca) X[a][bb] = fma(-l[a], xr[bb], X[a][bb]);
+                if (a <= ca && bb >= ca) Y[a][bb] = fma(vc[a], ur[bb], Y[a][bb]);
             }
         if (s == (c & 15)) {
 #pragma unroll
