@@ -164,16 +164,17 @@ __device__ __forceinline__ void diag_block(const DenseJob &J, int p, const doubl
         const double a_nn = nxt[par], l_n = bc[par][2][cn], u_n = bc[par][0][cn];
         const double rpc = rp;
         double ur[4], xr[4], l[4], vc[4];
+        // masks: only the block that holds index c needs a run-time test
 #pragma unroll
         for (int bb = 0; bb < 4; bb++) {
             const int j = s + 16 * bb;
-            ur[bb] = j > c ? rowv[bb] : 0.0;
-            xr[bb] = j < c ? xrow[bb] : 0.0;
+            ur[bb] = bb > ca ? rowv[bb] : bb < ca ? 0.0 : (j > c ? rowv[bb] : 0.0);
+            xr[bb] = bb < ca ? xrow[bb] : bb > ca ? 0.0 : (j < c ? xrow[bb] : 0.0);
         }
 #pragma unroll
         for (int a = 0; a < 4; a++) {
             const int i = r + 16 * a;
-            l[a] = i > c ? colv[a] * rpc : 0.0;
+            l[a] = a > ca ? colv[a] * rpc : a < ca ? 0.0 : (i > c ? colv[a] * rpc : 0.0);
             vc[a] = wv[a] * rpc;                         // V(i, c); zero below the diagonal
         }
         rp = 1.0 / fma(-(l_n * rpc), u_n, a_nn);          // unused after the last step
