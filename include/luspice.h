@@ -68,6 +68,9 @@ typedef struct {
     double factor_shift;   /* factor-only diagonal shift: the first diagonal entry of each row
                               is factored as a + shift*|a| (0 = none; the residual and SpMV keep
                               the caller's values; test_lu_gpu_f32.cu:354-364) */
+    int krylov_spmv;       /* value precision of the Arnoldi products w = A z inside GMRES:
+                              0 fp64 (default), 1 fp32, 2 FP16 (values must fit FP16); the
+                              residual b - A x is always taken with the fp64 values */
 } lsp_options;
 
 typedef struct {

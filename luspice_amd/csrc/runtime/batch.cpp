@@ -353,6 +353,7 @@ size_t Batch::estimate_device_bytes() const
     b += 8 * (n + nc) + 4 * nlu + 24 * n + 2 * rel;                       // lrp urp lci uci rowat q snend ru fcut bcut rel
     b += 8 * (slices + nc) + 8 * n + 8 * sell;                            // SELL structure
     b += 12 * n + 8 * nnz + 12 * sell;                                    // flags, values
+    if (opt_.krylov_spmv == 2) b += 2 * sell;                              // FP16 SELL values
     b += 16 * nnz + 24 * n + 16 * (nc + 1);                               // Ruiz: row/col, scaled values, scales
     if (opt_.factor_shift > 0.0) b += 8 * n;                               // shifted diagonal entries
     b += 8 * T2 + 16 * T + 65536.0 * nc;                                   // dense block work
@@ -564,6 +565,9 @@ int Batch::upload_all()
     d_sval64_ = (double *)dalloc(nsell_tot_ * sizeof(double));
     d_sval32_ = (float *)dalloc(nsell_tot_ * sizeof(float));
     if (!d_ain_ || !d_sval64_ || !d_sval32_) return LSP_E_ALLOC;
+    if (opt_.krylov_spmv < 0 || opt_.krylov_spmv > 2) return LSP_E_ARG;
+    if (opt_.krylov_spmv == 2 && !(d_sval16_ = (__half *)dalloc(nsell_tot_ * sizeof(__half))))
+        return LSP_E_ALLOC;
     {
         std::vector<double> v0(nnz_tot_);
         for (auto &C : circ_) std::copy(C.val0.begin(), C.val0.end(), v0.begin() + C.aoff);
@@ -592,6 +596,15 @@ int Batch::upload_all()
         }
         d_sdesc64_ = upload(s64);
         d_sdesc32_ = upload(s32);
+        if (d_sval16_) {
+            std::vector<SellDesc<__half>> s16(nc);
+            for (int m = 0; m < nc; m++) {
+                s16[m].off = s64[m].off; s16[m].rlen = s64[m].rlen; s16[m].col = s64[m].col;
+                s16[m].val = d_sval16_ + circ_[m].o_ss; s16[m].perm = s64[m].perm;
+                s16[m].nslices = s64[m].nslices; s16[m].n = s64[m].n;
+            }
+            if (!(d_sdesc16_ = upload(s16))) return LSP_E_ALLOC;
+        }
         d_stask_ = upload(st);
         nstask_ = (int)st.size();
         if (!d_sdesc64_ || !d_sdesc32_ || !d_stask_) return LSP_E_ALLOC;
@@ -713,6 +726,8 @@ int Batch::upload_all()
     // initial SELL values
     rc = check(launch_gather_cast<double>(nsell_tot_, d_ssrc_, d_ain_, d_sval64_, stream_), "gather");
     if (!rc) rc = check(launch_gather_cast<float>(nsell_tot_, d_ssrc_, d_ain_, d_sval32_, stream_), "gather");
+    if (!rc && d_sval16_)
+        rc = check(launch_gather_cast<__half>(nsell_tot_, d_ssrc_, d_ain_, d_sval16_, stream_), "gather");
     if (!rc) rc = check(hipStreamSynchronize(stream_), "sync");
     return rc;
 }
@@ -738,6 +753,8 @@ int Batch::set_values(const double *vals, bool on_device)
     if (rc) return rc;
     rc = check(launch_gather_cast<double>(nsell_tot_, d_ssrc_, d_ain_, d_sval64_, stream_), "gather");
     if (!rc) rc = check(launch_gather_cast<float>(nsell_tot_, d_ssrc_, d_ain_, d_sval32_, stream_), "gather");
+    if (!rc && d_sval16_)
+        rc = check(launch_gather_cast<__half>(nsell_tot_, d_ssrc_, d_ain_, d_sval16_, stream_), "gather");
     return rc;
 }
 
@@ -964,6 +981,20 @@ int Batch::launch_trisolve_prec(int prec, const double *d_b, double *d_x, const 
     return LSP_E_ARG;
 }
 
+// Arnoldi product w = A z with the value precision chosen by krylov_spmv
+// (fp64 accumulation in every case; the residual always uses launch_spmv64)
+int Batch::launch_spmv_krylov(const double *d_xv, double *d_y, const int *active)
+{
+    const int64_t *vo = (const int64_t *)d_voff_;
+    if (opt_.krylov_spmv == 1)
+        return check(launch_sell_spmv<float>(d_sdesc32_, d_stask_, nstask_, vo, d_xv, d_y, nullptr, active,
+                                             stream_), "spmv32");
+    if (opt_.krylov_spmv == 2)
+        return check(launch_sell_spmv<__half>(d_sdesc16_, d_stask_, nstask_, vo, d_xv, d_y, nullptr, active,
+                                              stream_), "spmv16");
+    return launch_spmv64(d_xv, d_y, nullptr, active);
+}
+
 int Batch::launch_spmv64(const double *d_xv, double *d_y, const double *d_bv, const int *active)
 {
     return check(launch_sell_spmv<double>(d_sdesc64_, d_stask_, nstask_, (const int64_t *)d_voff_,
@@ -1087,7 +1118,7 @@ int Batch::gmres_ir(const double *d_bv, double *d_xv, int prec, const std::vecto
             double *Vj = d_V_ + (long)j * ntot_;
             double *Zj = d_Z_ + (long)j * ntot_;
             LSP_TRY(launch_trisolve_prec(prec, Vj, Zj, d_act_));
-            LSP_TRY(launch_spmv64(Zj, d_w_, nullptr, d_act_));
+            LSP_TRY(launch_spmv_krylov(Zj, d_w_, d_act_));
             for (int pass = 0; pass < 2; pass++) {
                 LSP_HIP(k_chunk_dots(K, d_V_, d_w_, j + 1, s), "dots");
                 LSP_HIP(k_reduce(K, j + 1, 1, j, nullptr, s), "reduce");
@@ -1319,7 +1350,7 @@ int Batch::record_step(int prec)
         double *Vj = d_V_ + (long)j * ntot_;
         double *Zj = d_Z_ + (long)j * ntot_;
         LSP_TRY(launch_trisolve_prec(prec, Vj, Zj, d_act_, d_nact_));
-        LSP_TRY(launch_spmv64(Zj, d_w_, nullptr, d_act_));
+        LSP_TRY(launch_spmv_krylov(Zj, d_w_, d_act_));
         for (int pass = 0; pass < 2; pass++) {
             LSP_HIP(k_chunk_dots(K, d_V_, d_w_, j + 1, s), "dots");
             LSP_HIP(k_reduce(K, j + 1, 1, j, nullptr, s), "reduce");

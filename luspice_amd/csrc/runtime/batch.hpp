@@ -87,6 +87,7 @@ public:
                  lsp_solve_report *rep);
     int launch_trisolve_prec(int prec, const double *d_b, double *d_x, const int *active,
                              const int *d_nact = nullptr);
+    int launch_spmv_krylov(const double *d_xv, double *d_y, const int *active);
     int launch_spmv64(const double *d_x, double *d_y, const double *d_b, const int *active);
     hipStream_t stream() const { return stream_; }
     int cur_prec() const { return cur_prec_; }
@@ -140,6 +141,8 @@ private:
     unsigned long long *d_rmx_ = nullptr;
     double *d_ain_ = nullptr, *d_sval64_ = nullptr;
     float *d_sval32_ = nullptr;
+    __half *d_sval16_ = nullptr;         // FP16 SELL values (krylov_spmv == 2 only)
+    SellDesc<__half> *d_sdesc16_ = nullptr;
     SellDesc<double> *d_sdesc64_ = nullptr;
     SellDesc<float> *d_sdesc32_ = nullptr;
     int2 *d_ftask_ = nullptr, *d_fwtask_ = nullptr, *d_bwtask_ = nullptr, *d_stask_ = nullptr;

@@ -146,3 +146,30 @@ def test_batch_uneven_same_alone_and_mixed():
         xs, r2 = solo.solve(c.rhs)
         assert r2["rc"] == 0
         assert O.rel_l2(xi, xs) <= 1e-6
+
+
+def test_zero_pivot_reported_then_recovered():
+    """A Newton step whose values make a fixed pivot exactly zero: the refactor
+    reports the circuit as singular (LSP_E_SINGULAR, first bad circuit 0) and
+    every kernel returns; the next step's values factor and solve normally.
+    The 3 x 3 circuit has no dense trailing block, so the sparse refactor
+    kernel takes the zero pivot."""
+    print("case: zero pivot then recovery", flush=True)
+    tail_max = -1
+    row = np.array([0, 0, 1, 1, 1, 2, 2], np.int32)
+    col = np.array([0, 1, 0, 1, 2, 1, 2], np.int32)
+    v0 = np.array([4.0, 1.0, 1.0, 4.0, 1.0, 1.0, 4.0])
+    b = np.array([1.0, 2.0, 3.0])
+    B = L.Batch(L.Circuit(3, row, col, v0, b, q=np.arange(3, dtype=np.int32)), tail_max=tail_max)
+    # U(1,1) = 0.5 - 1 * 1 / 2 = 0 exactly with the fixed diagonal pivots
+    B.set_values(np.array([2.0, 1.0, 1.0, 0.5, 1.0, 1.0, 4.0]))
+    rep = B.refactor("fp64")
+    print(rep, flush=True)
+    assert rep["rc"] == -4 and rep["singular_circuits"] == 1 and rep["first_bad_circuit"] == 0
+    B.set_values(v0)
+    rep = B.refactor("fp64")
+    assert rep["rc"] == 0 and rep["singular_circuits"] == 0, rep
+    x, sr = B.solve(b)
+    A = np.zeros((3, 3))
+    A[row, col] = v0
+    assert sr["rc"] == 0 and O.rel_l2(x, np.linalg.solve(A, b)) <= 1e-12

@@ -116,3 +116,53 @@ def test_fp32_and_fp64_on_c2():
         x, sr = B.solve(c.rhs)
         assert _rel(x, xr) <= 1e-6, (prec, sr)
     B.close()
+
+
+@pytest.mark.gpu
+def test_krylov_spmv_value_precision():
+    """Arnoldi products with fp32 or FP16 SELL values (the residual stays on
+    the fp64 values): config 0 with fp32 values, and an FP16 candidate (a
+    diagonally dominant grid with values in [1, 40], inside FP16's range) with
+    FP16 values; both reach the fp64 solution to 1e-6, by restarts if the
+    perturbed Arnoldi operator needs them."""
+    print("case: c0 fp32-value Arnoldi products", flush=True)
+    c = L.mna("c0", seed=0)
+    q = L.colamd(c)
+    rc, xr, _, _ = O.cpu_lu_ref(c.n, c.row, c.col, c.val, q, c.rhs)
+    B = L.Batch(L.Circuit(c.n, c.row, c.col, c.val, c.rhs, q=q), krylov_spmv=1)
+    assert B.refactor("fp32")["rc"] == 0
+    x, sr = B.solve(c.rhs)
+    print(sr, flush=True)
+    assert sr["rc"] == 0 and _rel(x, xr) <= 1e-6
+    B.close()
+    print("case: FP16-range grid, FP16-value Arnoldi products", flush=True)
+    rng = np.random.default_rng(11)
+    off = c.row != c.col
+    v = np.zeros(c.nnz)
+    v[off] = -rng.uniform(1.0, 8.0, int(off.sum()))
+    d = np.zeros(c.n)
+    np.add.at(d, c.row[off], -v[off])
+    diag_e = np.flatnonzero(~off)
+    v[diag_e] = 0.0
+    first = {}
+    for e in diag_e:
+        first.setdefault(int(c.row[e]), int(e))
+    for i, e in first.items():
+        v[e] = d[i] + 1.0
+    has = np.zeros(c.n, bool)
+    has[list(first)] = True
+    # rows without a diagonal entry (source rows) get one appended
+    add = np.flatnonzero(~has)
+    row = np.concatenate([c.row, add]).astype(np.int32)
+    col = np.concatenate([c.col, add]).astype(np.int32)
+    val = np.concatenate([v, d[add] + 1.0])
+    assert np.abs(val).max() <= 6.0e4 and np.abs(val[val != 0]).min() >= 1.0
+    q2 = L.colamd(L.Circuit(c.n, row, col, val, c.rhs))
+    rc, xr2, _, _ = O.cpu_lu_ref(c.n, row, col, val, q2, c.rhs)
+    assert rc == -1
+    B = L.Batch(L.Circuit(c.n, row, col, val, c.rhs, q=q2), krylov_spmv=2, ir_outer=10)
+    assert B.refactor("fp32")["rc"] == 0
+    x, sr = B.solve(c.rhs)
+    print(sr, _rel(x, xr2), flush=True)
+    assert sr["rc"] == 0 and _rel(x, xr2) <= 1e-6
+    B.close()
