@@ -544,7 +544,7 @@ int Batch::upload_all()
         d_rmx_ = (unsigned long long *)dalloc(ntot_ * sizeof(unsigned long long));
         if (!d_arow_ || !d_acol_ || !d_raoff_ || !d_rvoff_ || !d_ains_ || !d_rscale_ || !d_cscale_ || !d_rmx_)
             return LSP_E_ALLOC;
-        if (opt_.factor_shift < 0.0) return LSP_E_ARG;
+        if (!(opt_.factor_shift >= 0.0)) return LSP_E_ARG;   // negative or NaN
         if (opt_.factor_shift > 0.0) {
             // the first value entry of each diagonal, in caller order
             // (test_lu_gpu_f32.cu:356-361 stops at the first match per row)
