@@ -38,6 +38,7 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--check", type=int, default=-1, help="circuits checked vs the CPU LU (-1 = all)")
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of the HIP graph")
+    ap.add_argument("--prec", default="fp32", choices=["fp32", "fp64"], help="factor storage of the batched path")
     args = ap.parse_args(argv)
 
     from . import device_sync
@@ -56,14 +57,14 @@ def main(argv=None):
     tails = [NB.B.info(i)["tail_T"] for i in range(len(circ))]
     log(f"[newton] batch built {time.time() - t0:.1f}s, dense tails {sum(t > 0 for t in tails)}, max T {max(tails)}")
     for s in range(args.warmup):
-        NB.run_step(s, graph=not args.no_graph)
+        NB.run_step(s, prec=args.prec, graph=not args.no_graph)
     device_sync()
     lat = []
     reps = []
     t1 = time.perf_counter()
     for s in range(args.warmup, args.warmup + args.steps):
         ts = time.perf_counter()
-        rr, sr = NB.run_step(s, graph=not args.no_graph)
+        rr, sr = NB.run_step(s, prec=args.prec, graph=not args.no_graph)
         lat.append((time.perf_counter() - ts) * 1e3)
         reps.append((rr, sr))
     device_sync()
@@ -107,7 +108,8 @@ def main(argv=None):
     inner = [r[1]["inner_total"] for r in reps]
     fallback = sum(r[1]["fallback"] for r in reps)
     line = {
-        "metric": "batched Newton refactor+solve units/s at rel err <=1e-6 (mixed batched vs plain fp64)",
+        "metric": ("batched Newton refactor+solve units/s at rel err <=1e-6 ("
+                   + ("mixed" if args.prec == "fp32" else "fp64") + " batched vs plain fp64)"),
         "value": round(units_b, 2),
         "unit": "circuit refactor+solve/s",
         "plain_fp64": round(units_p, 2),
