@@ -6,17 +6,23 @@
  *   dsh_symbolic   once per sparsity pattern: COO pattern (1-based row/col,
  *                  as the reference's coo_to_csc reads them) + column order
  *                  (NULL = natural order)
- *   dsh_numeric    every Newton step: new values; the first call fixes the
- *                  pivot sequence on the host (threshold partial pivoting)
- *                  and uploads the static structure, later calls refactor on
- *                  the GPU with fp32 factor storage (fp64 pivot test); a
- *                  pivot that fails the refactor test re-derives the pivot
- *                  sequence for the new values.  pin > 0 is added to every
- *                  diagonal entry present in the pattern.
- *   dsh_trisolve   x = Q U^-1 L^-1 P b with the current fp32 factors, fp64
- *                  accumulation (one preconditioner application)
- *   dsh_gmres_ir   fp64 GMRES-IR to rtol with the fp32 factors as right
- *                  preconditioner; residual_seq (optional, max_outer+1
+ *   dsh_numeric    every Newton step: new values (1-based COO; a later
+ *                  duplicate overwrites an earlier one); pin > 0 is added to
+ *                  every diagonal entry present in the pattern.  Factors in
+ *                  fp64 on the GPU with the reference's partial pivoting
+ *                  (tol = 1.0): the first call picks the pivot sequence on the
+ *                  host, later calls refactor over the kept structure and
+ *                  re-derive the pivots only when splu would pick another
+ *                  pivot on the new values, so P equals splu's every call.
+ *                  Fills lnnz, unnz, cap, P, min_diag_U, pivot_growth, and
+ *                  factor_cache, which points at the fp64 factors in splu's
+ *                  lu_factors layout (splu.h:29-41: pinv, L.Lp/Li/Lx with
+ *                  the unit diagonal first, U.Up/Ui/Ux with the diagonal
+ *                  last, rows as pivot positions, rows ascending per column)
+ *   dsh_trisolve   x = Q U^-1 L^-1 P b with the fp64 factors (the
+ *                  reference's lu_solve_perm)
+ *   dsh_gmres_ir   fp64 GMRES-IR to rtol with fp32 factors (same pivots) as
+ *                  right preconditioner; residual_seq (optional, max_outer+1
  *                  entries) receives ||b - A x||/||b|| after each restart
  *
  * Return codes: 0 OK, 1 SINGULAR, 2 ALLOC, 3 ARG / not converged.
@@ -44,7 +50,7 @@ typedef struct {
     double  min_diag_U;    /* min |U(k,k)| of the last factorization */
     double  pivot_growth;  /* max |U| / max |A| of the last factorization */
     int     dev_allocated;
-    void   *factor_cache;  /* device-resident state */
+    void   *factor_cache;  /* fp64 factors in splu's lu_factors layout (host) */
     double  pin;           /* diagonal perturbation added by dsh_numeric */
 } dsh_lu_static;
 

@@ -71,6 +71,17 @@ typedef struct {
     int krylov_spmv;       /* value precision of the Arnoldi products w = A z inside GMRES:
                               0 fp64 (default), 1 fp32, 2 FP16 (values must fit FP16); the
                               residual b - A x is always taken with the fp64 values */
+    int repivot;           /* new pivot sequences on the host when a refactor's fixed pivots
+                              no longer fit the step's values (the reference's splu picks its
+                              pivots anew at every factorization, splu.h:174-185):
+                              0  never: failures are only reported;
+                              1  (default) a circuit with a zero pivot or a pivot failing
+                                 refactor_ptol gets the pivot sequence the reference's partial
+                                 pivoting (tol = pivot_tol) picks on the step's values, and the
+                                 batch is refactored again within the same call;
+                              2  also whenever that partial pivoting would pick a different
+                                 pivot for any column (P equals the reference's at every step;
+                                 meant for fp64 factors) */
 } lsp_options;
 
 typedef struct {
@@ -79,6 +90,10 @@ typedef struct {
     int singular_circuits; /* circuits with a zero pivot */
     int first_bad_circuit; /* -1 if none */
     double ms;             /* device time of the refactor launch(es) */
+    int repivoted;         /* circuits given a new pivot sequence by this call */
+    int pivot_mismatch;    /* columns whose fixed pivot the reference's partial pivoting would
+                              not pick on these values (counted when repivot = 2, before any
+                              new pivot sequence) */
 } lsp_refactor_report;
 
 typedef struct {
@@ -87,7 +102,8 @@ typedef struct {
     int inner_total;       /* inner iterations summed over restarts (max) */
     int converged;         /* circuits that met ir_outer_tol */
     int fallback;          /* circuits that fell back to fp64 factors */
-    double max_rel_res;    /* max ||b - A x|| / ||b|| over circuits */
+    double max_rel_res;    /* max ||b - A x|| / ||b|| over circuits (INF if one diverged) */
+    int singular_fallback; /* fallback circuits whose fp64 refactor met a zero pivot */
 } lsp_solve_report;
 
 typedef struct {

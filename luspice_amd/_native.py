@@ -41,18 +41,19 @@ class Options(C.Structure):
                 ("refactor_blocks", C.c_int), ("trisolve_blocks", C.c_int),
                 ("tail_max", C.c_int), ("tail_density", C.c_double), ("ruiz_sweeps", C.c_int),
                 ("pivot_scaled", C.c_int), ("poll_backoff", C.c_int),
-                ("factor_shift", C.c_double), ("krylov_spmv", C.c_int)]
+                ("factor_shift", C.c_double), ("krylov_spmv", C.c_int), ("repivot", C.c_int)]
 
 
 class RefactorReport(C.Structure):
     _fields_ = [("status", C.c_int), ("bad_pivots", C.c_int),
                 ("singular_circuits", C.c_int), ("first_bad_circuit", C.c_int),
-                ("ms", C.c_double)]
+                ("ms", C.c_double), ("repivoted", C.c_int), ("pivot_mismatch", C.c_int)]
 
 
 class SolveReport(C.Structure):
     _fields_ = [("status", C.c_int), ("outer", C.c_int), ("inner_total", C.c_int),
-                ("converged", C.c_int), ("fallback", C.c_int), ("max_rel_res", C.c_double)]
+                ("converged", C.c_int), ("fallback", C.c_int), ("max_rel_res", C.c_double),
+                ("singular_fallback", C.c_int)]
 
 
 class CircuitInfo(C.Structure):

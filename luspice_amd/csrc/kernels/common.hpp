@@ -328,6 +328,35 @@ __device__ __forceinline__ double wave_max(double v)
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
     return v;
 }
+// Would the reference's partial pivoting (splu.h:174-185) pick the fixed
+// pivot of this column on the current values?  splu takes the first strict
+// maximum of |x| over the rows not yet pivotal (depth-first reach order) and
+// then keeps the diagonal row (original row q[k]) instead when its |x| is at
+// least tol times that maximum.  Arguments, all magnitudes: pv of the fixed
+// pivot, lmax the largest over the other candidate rows, dv of the diagonal
+// row when it is one of those rows (negative when it is not a candidate);
+// diag_pivot when the fixed pivot is the diagonal row.  Decisions closer than
+// a relative 1e-10 to flipping are "uncertain": summation order separates
+// these values from splu's, so the host decides those columns.
+// Returns 0 agree, 1 disagree, 2 uncertain.
+__device__ __forceinline__ int splu_pivot_verdict(bool diag_pivot, double pv, double lmax, double dv,
+                                                  double tol)
+{
+    constexpr double eps = 1e-10;
+    if (pv == 0.0) return lmax > 0.0 ? 1 : 0;          // splu: another row, or singular for both
+    if (diag_pivot) {                                   // kept iff pv >= tol * max(pv, lmax)
+        if (pv >= tol * lmax * (1.0 + eps)) return 0;
+        return pv < tol * lmax * (1.0 - eps) ? 1 : 2;
+    }
+    const double a = fmax(pv, lmax);
+    if (dv >= 0.0) {                                    // the diagonal row would win
+        if (dv >= tol * a * (1.0 + eps)) return 1;
+        if (dv >= tol * a * (1.0 - eps)) return 2;
+    }
+    if (pv > lmax * (1.0 + eps)) return 0;              // the unique maximum
+    return pv < lmax * (1.0 - eps) ? 1 : 2;             // smaller, or a tie broken by reach order
+}
+
 __device__ __forceinline__ int wave_min_int(int v)
 {
 #pragma unroll

@@ -462,7 +462,8 @@ __global__ void __launch_bounds__(256) dense_gemm_kernel(const DenseJob *jobs, c
 // the sparse factors, dense copy in storage precision.
 template <typename T>
 __global__ void __launch_bounds__(64) dense_finish_kernel(const DenseJob *jobs, const void *mats,
-                                                          const int *active, double ptol)
+                                                          const int *active, double ptol,
+                                                          double stol)
 {
     const DenseJob J = jobs[blockIdx.y];
     const int c = blockIdx.x;
@@ -495,6 +496,18 @@ __global__ void __launch_bounds__(64) dense_finish_kernel(const DenseJob *jobs, 
         }
         const double vmax = sizeof(T) == 2 ? 65504.0 : sizeof(T) == 4 ? 3.4028234663852886e38 : 1.0e308;
         if (!(fabs(piv) <= vmax) || !(lmax <= vmax)) atomicAdd(&M.mstat[3], 1);
+        if (stol > 0.0) {
+            // splu's choice on these values, in units of |pivot|: the other
+            // candidates are the multipliers L(r, c), r > c, of the block
+            const int dpos = M.dpos[k];
+            const double md = dpos > k ? fabs(Sc[dpos - M.tail_k0]) : -1.0;
+            const int v = piv == 0.0 ? (lmax > 0.0 ? 1 : 0)
+                                     : splu_pivot_verdict(dpos == k, 1.0, lmax, md, stol);
+            if (v) {
+                atomicAdd(&M.mstat[v == 1 ? 4 : 5], 1);
+                atomicMin(&M.mstat[6], k);
+            }
+        }
     }
 }
 
@@ -674,14 +687,15 @@ hipError_t launch_dense_lu(const DenseJob *jobs, int njobs, int maxT, const int 
 
 template <typename T>
 hipError_t launch_dense_finish(const DenseJob *jobs, int njobs, int maxT, const void *mats,
-                               const int *active, double ptol, hipStream_t st)
+                               const int *active, double ptol, double stol, hipStream_t st)
 {
-    hipLaunchKernelGGL(dense_finish_kernel<T>, dim3(maxT, njobs), dim3(64), 0, st, jobs, mats, active, ptol);
+    hipLaunchKernelGGL(dense_finish_kernel<T>, dim3(maxT, njobs), dim3(64), 0, st, jobs, mats, active, ptol,
+                       stol);
     return hipGetLastError();
 }
-template hipError_t launch_dense_finish<float>(const DenseJob *, int, int, const void *, const int *, double, hipStream_t);
-template hipError_t launch_dense_finish<double>(const DenseJob *, int, int, const void *, const int *, double, hipStream_t);
-template hipError_t launch_dense_finish<__half>(const DenseJob *, int, int, const void *, const int *, double, hipStream_t);
+template hipError_t launch_dense_finish<float>(const DenseJob *, int, int, const void *, const int *, double, double, hipStream_t);
+template hipError_t launch_dense_finish<double>(const DenseJob *, int, int, const void *, const int *, double, double, hipStream_t);
+template hipError_t launch_dense_finish<__half>(const DenseJob *, int, int, const void *, const int *, double, double, hipStream_t);
 
 template <typename T>
 hipError_t launch_tail_solve(const void *mats, const int *tails, int ntails, int maxT, const long *voff,

@@ -16,6 +16,9 @@ enum : int {
     kErrCapacity = 2,    // a column/row exceeded its scratch capacity
 };
 
+// Status words per matrix written by the refactor (FactorDesc::mstat).
+constexpr int kMstat = 8;
+
 // Trisolve task codes at or above this value name a packet of short rows;
 // each lane of a packet row gathers up to kTriPacketLane of its entries.
 constexpr int kTriPacket = 1 << 30;
@@ -35,11 +38,14 @@ struct FactorDesc {
     const uint16_t *rel;    // segment maps: local slots of each run's last L rows
     const int *snend;       // n: one past the last column of j's supernode
     const int *dst;         // slot -> index in lrow / urow / diag
+    const int *dpos;        // n: position of the diagonal row q[k] (pinv[q[k]])
     T *val;                 // column storage of L and U
     T *lrow, *urow, *diag;  // row-wise copies for the trisolve
     int *done;              // per-column epoch flags
-    int *mstat;             // 4 ints: bad pivots, first bad column,
-                            // first zero-pivot column, non-finite count
+    int *mstat;             // kMstat ints: bad pivots, first bad column,
+                            // first zero-pivot column, non-finite count,
+                            // columns whose pivot splu would not pick, columns
+                            // too close to call, first of either, pad
     // dense trailing block (tail_T = 0: none, tail_k0 = n)
     const int *tail_ds;     // tail_T: sparse dependencies of tail column c
     double *tail_S;         // tail_T^2 fp64 work block, column-major

@@ -153,7 +153,8 @@ __global__ void __launch_bounds__(256) fill_unset_kernel(const FactorDesc<T> *ma
 // a time; each lane fetches the metadata and waits on the flag of one slot.
 template <typename T, typename XP>
 __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
-                              int epoch, int *abort_word, double ptol, int *dbg, long long *tw, int backoff)
+                              int epoch, int *abort_word, double ptol, double stol, int *dbg, long long *tw,
+                              int backoff)
 {
     const int lane = lane_id();
     const LSP_GLOBAL int64_t *cp = gp(M.cp);
@@ -318,6 +319,23 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
         }
         if (nonfinite) atomicAdd(&M.mstat[3], 1);
     }
+    if (stol > 0.0) {
+        // would splu (tol = stol) pick this pivot on these values?
+        const int dpos = gp(M.dpos)[k];
+        double dv = -1.0;
+        if (dpos > k) {
+            for (int i = dl + 1 + lane; i < m; i += kWave)
+                if (pos[c0 + i] == dpos) dv = fabs(x[i]);
+            dv = wave_max(dv);
+        }
+        if (lane == 0) {
+            const int v = splu_pivot_verdict(dpos == k, fabs(piv), amax, dv, stol);
+            if (v) {
+                atomicAdd(&M.mstat[v == 1 ? 4 : 5], 1);
+                atomicMin(&M.mstat[6], k);
+            }
+        }
+    }
     const LSP_GLOBAL int *dst = gp(M.dst);
     for (int i = lane; i < dl; i += kWave) {
         const T v = from_f64<T>(x[i]);
@@ -356,9 +374,9 @@ __global__ void __launch_bounds__(256) refactor_kernel(FactorLaunch L)
         long long *wp = L.timeline ? tw : nullptr;
         const long long t0 = L.timeline ? (long long)wall_clock64() : 0;
         if (m <= L.cap) {
-            factor_column<T>(M, k, xl, epoch, L.abort_word, L.ptol, L.dbg, wp, L.poll_backoff);
+            factor_column<T>(M, k, xl, epoch, L.abort_word, L.ptol, L.stol, L.dbg, wp, L.poll_backoff);
         } else if (m <= L.scratch_stride) {
-            factor_column<T>(M, k, xg, epoch, L.abort_word, L.ptol, L.dbg, wp, L.poll_backoff);
+            factor_column<T>(M, k, xg, epoch, L.abort_word, L.ptol, L.stol, L.dbg, wp, L.poll_backoff);
         } else {
             if (lane_id() == 0) st_relaxed(L.abort_word, kErrCapacity);
         }
@@ -388,10 +406,15 @@ __global__ void reset_mstat_kernel(int *mstat, int nmat, int *abort_word, int *d
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (dbg && i < 8) dbg[i] = -1;
     if (i < nmat) {
-        mstat[4 * i + 0] = 0;
-        mstat[4 * i + 1] = 0x7fffffff;
-        mstat[4 * i + 2] = 0x7fffffff;
-        mstat[4 * i + 3] = 0;
+        int *w = mstat + kMstat * i;
+        w[0] = 0;
+        w[1] = 0x7fffffff;
+        w[2] = 0x7fffffff;
+        w[3] = 0;
+        w[4] = 0;
+        w[5] = 0;
+        w[6] = 0x7fffffff;
+        w[7] = 0;
     }
     if (i == 0 && abort_word) *abort_word = 0;
 }

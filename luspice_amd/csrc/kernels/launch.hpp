@@ -22,6 +22,7 @@ struct FactorLaunch {
     int cap;                // LDS doubles per wave
     int total_waves;
     double ptol;
+    double stol;            // > 0: also classify each pivot against splu's choice at this tol
     int *dbg;               // 8 ints: first stuck wait (dep, seen, epoch, spins, column)
     long long *timeline;    // optional, 4 per task (100 MHz wall clock): start, flag
                             // published (end for a dense-block column), end of the
@@ -84,7 +85,7 @@ hipError_t dense_set_lds();
 hipError_t launch_dense_lu(const DenseJob *jobs, int njobs, int maxT, const int *active, hipStream_t st);
 template <typename T>
 hipError_t launch_dense_finish(const DenseJob *jobs, int njobs, int maxT, const void *mats,
-                               const int *active, double ptol, hipStream_t st);
+                               const int *active, double ptol, double stol, hipStream_t st);
 template <typename T>
 hipError_t launch_tail_solve(const void *mats, const int *tails, int ntails, int maxT, const long *voff,
                              const int *active, const double *b, double *y, double *z, double *x,

@@ -3,7 +3,7 @@
 // accumulation (gfx950).
 //
 // Reference behaviour: lu_solve_perm (src/third_party/splu.h:210-239) and the
-// fp32-factor trisolves (src/lu_gpu_f32.cu:255-333: factors read as double,
+// fp32-factor trisolves (src/lu_gpu_f32.cu:94-172: factors read as double,
 // fp64 accumulation, pb = P b, x[q[k]] = z[k]).  The reference's
 // level-scheduled form (src/mp/fusion_bench.cu:80-116) runs one block with a
 // barrier per level; here rows are tasks in a global queue sorted by level,

@@ -94,6 +94,14 @@ int Batch::alloc_krylov(int mm)
 int Batch::set_ir_params(int outer, int inner, double inner_tol, double outer_tol)
 {
     if (outer < 1 || inner < 1 || !(inner_tol > 0) || !(outer_tol > 0)) return LSP_E_ARG;
+    // a recorded step graph holds the Krylov buffers and tolerances of its
+    // recording: record again on the next graph_step
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+    graph_prec_ = -1;
     if (inner > mm_alloc_) {
         int rc = alloc_krylov(inner);
         if (rc) return rc;
@@ -251,12 +259,13 @@ int Batch::ensure_factor_set(FactorSet<T> &fs)
         F.rel = d_rel_ + C.o_rel;
         F.snend = d_snend_ + C.o_n;
         F.dst = d_dst_ + C.o_slot;
+        F.dpos = d_dpos_ + C.o_n;
         F.val = fs.val + C.o_slot;
         F.lrow = fs.lrow + C.o_l;
         F.urow = fs.urow + C.o_u;
         F.diag = fs.diag + C.o_n;
         F.done = d_done_ + C.o_n;
-        F.mstat = d_mstat_ + 4 * m;
+        F.mstat = d_mstat_ + kMstat * m;
         F.n = C.n;
         F.tail_ds = d_tail_ds_ + C.o_t;
         F.tail_S = d_tailS_ + C.o_t2;
@@ -310,6 +319,9 @@ int Batch::prepare(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
 {
     if (!cin || nc <= 0 || !circ_.empty()) return LSP_E_ARG;
     if (opt) opt_ = *opt; else lsp_options_default(&opt_);
+    if (opt_.repivot < 0 || opt_.repivot > 2) return LSP_E_ARG;
+    refactor_tl_path_ = getenv("LSP_REFACTOR_TIMELINE");     // diagnostic dumps
+    trisolve_tl_path_ = getenv("LSP_TRISOLVE_TIMELINE");
     circ_.resize(nc);
     std::vector<int> rcs(nc, 0);
     {
@@ -372,9 +384,11 @@ size_t Batch::device_bytes() const
 }
 
 // Device phase: concatenates the circuits' structures and uploads them.
+// Also the second half of a rebuild (release_device first): the stream,
+// events, pinned words and the b / x arenas are kept.
 int Batch::upload_all()
 {
-    if (circ_.empty() || stream_) return LSP_E_ARG;
+    if (circ_.empty() || uploaded_) return LSP_E_ARG;
     const int nc = (int)circ_.size();
     int dev = 0;
     hipDeviceProp_t prop;
@@ -383,9 +397,11 @@ int Batch::upload_all()
     rc = check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
     if (rc) return rc;
     cu_count_ = prop.multiProcessorCount;
-    rc = check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream");
-    if (rc) return rc;
-    own_stream_ = true;
+    if (!stream_) {
+        rc = check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream");
+        if (rc) return rc;
+        own_stream_ = true;
+    }
 
     // ---- concatenate structures ----
     long o_np1 = 0, o_n = 0, o_slot = 0, o_a = 0, o_rel = 0, o_l = 0, o_u = 0, o_sl = 0,
@@ -410,6 +426,7 @@ int Batch::upload_all()
     {
         std::vector<int64_t> cp, relp, soff;
         std::vector<int> dp, pos, aptr, asrc, aslot, dst, lrp, lci, urp, uci, rowat, q, snend, ru, fcut, bcut, fpk, bpk;
+        std::vector<int> dpos;
         std::vector<int> srlen, scol, sperm, ssrc;
         std::vector<uint16_t> rel;
         cp.reserve(o_np1); relp.reserve(o_np1); rel.reserve(o_rel);
@@ -436,6 +453,7 @@ int Batch::upload_all()
             fpk.insert(fpk.end(), P.fpk.begin(), P.fpk.end());
             bpk.insert(bpk.end(), P.bpk.begin(), P.bpk.end());
             rel.insert(rel.end(), P.rel.begin(), P.rel.end());
+            for (int k = 0; k < C.n; k++) dpos.push_back(P.pinv[P.q[k]]);
             const LspSell &S = C.sell;
             soff.insert(soff.end(), S.off.begin(), S.off.end());
             std::vector<int> rl(C.n);
@@ -460,6 +478,8 @@ int Batch::upload_all()
         d_fpk_ = upload(fpk); d_bpk_ = upload(bpk);
         d_srlen_ = upload(srlen); d_scol_ = upload(scol); d_sperm_ = upload(sperm);
         d_ssrc_ = upload(ssrc);
+        d_dpos_ = upload(dpos);
+        if (!d_dpos_) return LSP_E_ALLOC;
         if (!d_cp_ || !d_relp_ || !d_soff_ || !d_dp_ || !d_pos_ || !d_aptr_ || !d_asrc_ ||
             !d_aslot_ || !d_dst_ || !d_lrp_ || !d_lci_ || !d_urp_ || !d_uci_ || !d_rowat_ ||
             !d_q_ || !d_rel_ || !d_snend_ || !d_ru_ || !d_fcut_ || !d_bcut_ || !d_fpk_ || !d_bpk_ || !d_srlen_ || !d_scol_ || !d_sperm_ || !d_ssrc_)
@@ -473,7 +493,7 @@ int Batch::upload_all()
         d_ctr_ = upload(z6); d_abort_ = upload(z6);
         std::vector<int> e1(4, 1);                 // epochs start at 1, flags at 0
         d_epoch_ = upload(e1);
-        std::vector<int> ms(4 * nc, 0);
+        std::vector<int> ms(kMstat * nc, 0);
         d_mstat_ = upload(ms);
         d_dbg_ = upload(z6);
         if (!d_done_ || !d_fdone_ || !d_bdone_ || !d_ctr_ || !d_abort_ || !d_mstat_ || !d_epoch_)
@@ -687,7 +707,8 @@ int Batch::upload_all()
     // vectors + Krylov state
     {
         const int mm = std::max(1, opt_.ir_inner);
-        d_b_ = (double *)dalloc(ntot_ * 8); d_x_ = (double *)dalloc(ntot_ * 8);
+        if (!d_b_) d_b_ = (double *)dalloc(ntot_ * 8);     // kept by a rebuild (with their contents)
+        if (!d_x_) d_x_ = (double *)dalloc(ntot_ * 8);
         d_r_ = (double *)dalloc(ntot_ * 8); d_w_ = (double *)dalloc(ntot_ * 8);
         d_z_ = (double *)dalloc(ntot_ * 8); d_t_ = (double *)dalloc(ntot_ * 8);
         d_ty_ = (double *)dalloc(ntot_ * 8); d_tz_ = (double *)dalloc(ntot_ * 8);
@@ -716,10 +737,11 @@ int Batch::upload_all()
         if (!d_b_ || !d_x_ || !d_r_ || !d_w_ || !d_z_ || !d_t_ || !d_ty_ || !d_tz_ ||
             !d_voff_ || !d_chunks_ || !d_cbeg_ || !d_act_)
             return LSP_E_ALLOC;
-        if (hipHostMalloc((void **)&h_pin_, 4096, hipHostMallocDefault) != hipSuccess)
+        if (!h_pin_ && hipHostMalloc((void **)&h_pin_, 4096, hipHostMallocDefault) != hipSuccess)
             return LSP_E_ALLOC;
     }
     for (auto &e : ev_) {
+        if (e) continue;
         rc = check(hipEventCreate(&e), "event");
         if (rc) return rc;
     }
@@ -729,7 +751,35 @@ int Batch::upload_all()
     if (!rc && d_sval16_)
         rc = check(launch_gather_cast<__half>(nsell_tot_, d_ssrc_, d_ain_, d_sval16_, stream_), "gather");
     if (!rc) rc = check(hipStreamSynchronize(stream_), "sync");
+    uploaded_ = rc == LSP_OK;
     return rc;
+}
+
+// Frees every device array of the batch except the b / x arenas (callers may
+// hold their addresses, and b carries the caller's right-hand sides) and
+// resets the derived sizes; upload_all() then rebuilds from circ_.  The
+// stream, events and pinned host words stay.
+void Batch::release_device()
+{
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+    graph_prec_ = -1;
+    double *keep_b = d_b_, *keep_x = d_x_;
+    std::vector<DevMem> kept;
+    for (auto &m : mem_) {
+        if (m.p == keep_b || m.p == keep_x) kept.push_back(m);
+        else if (m.p) (void)hipFree(m.p);
+    }
+    mem_.swap(kept);
+    static_cast<DeviceArrays &>(*this) = DeviceArrays{};
+    d_b_ = keep_b;
+    d_x_ = keep_x;
+    cur_prec_ = -1;
+    uploaded_ = false;
+    tri_pending_ = false;
 }
 
 int Batch::set_stream(hipStream_t s)
@@ -790,8 +840,7 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
     L.ntask = (int)ntot_;
     L.epoch = d_epoch_;
     L.timeline = nullptr;
-    const char *tl_path = getenv("LSP_REFACTOR_TIMELINE");   // diagnostic dump
-    if (tl_path) {
+    if (refactor_tl_path_) {                                  // diagnostic dump
         if (!d_timeline_) d_timeline_ = (long long *)dalloc((size_t)ntot_ * 4 * sizeof(long long));
         L.timeline = d_timeline_;
     }
@@ -803,6 +852,7 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
     L.total_waves = fblocks_ * 4;
     L.poll_backoff = opt_.poll_backoff > 0 ? opt_.poll_backoff : 7;
     L.ptol = opt_.refactor_ptol;
+    L.stol = opt_.repivot == 2 ? opt_.pivot_tol : 0.0;
     L.dbg = d_dbg_;
     rc = check(launch_reset_mstat(d_mstat_, (int)circ_.size(), d_abort_, d_dbg_, stream_), "reset");
     if (rc) return rc;
@@ -817,7 +867,7 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
         rc = check(launch_dense_lu(d_djobs_, njobs_, max_tail_T_, d_active, stream_), "dense lu");
         if (!rc)
             rc = check(launch_dense_finish<T>(d_djobs_, njobs_, max_tail_T_, fs.fdesc, d_active,
-                                              opt_.refactor_ptol, stream_), "dense finish");
+                                              opt_.refactor_ptol, L.stol, stream_), "dense finish");
     }
     if (!capturing_) (void)hipEventRecord(ev_[5], stream_);      // + dense trailing block
     return rc;
@@ -850,7 +900,9 @@ int Batch::launch_refactor_prec(int prec, const int *d_active)
     return LSP_E_ARG;
 }
 
-int Batch::refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *subset)
+// One refactor launch sequence over the circuits of `subset` (all when null)
+// and a host read of their status words (kMstat per circuit into ms).
+int Batch::refactor_pass(int prec, const std::vector<int> *subset, std::vector<int> &ms)
 {
     const int *d_active = nullptr;
     int rc;
@@ -865,8 +917,7 @@ int Batch::refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *
     rc = launch_refactor_prec(prec, d_active);
     if (rc) return rc;
     cur_prec_ = prec;
-    const int nc = (int)circ_.size();
-    std::vector<int> ms(4 * nc);
+    ms.assign((size_t)kMstat * circ_.size(), 0);
     rc = check(hipMemcpyAsync(ms.data(), d_mstat_, ms.size() * 4, hipMemcpyDeviceToHost, stream_), "mstat");
     if (!rc) rc = check(hipMemcpyAsync(h_pin_, d_abort_, 4, hipMemcpyDeviceToHost, stream_), "abort");
     if (!rc) rc = check(hipStreamSynchronize(stream_), "refactor sync");
@@ -878,30 +929,18 @@ int Batch::refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *
     stats_.refactor_ms += t_kernel;
     stats_.dense_ms += t_all - t_kernel;
     stats_.refactor_count++;
-    const int abort_code = h_pin_[0];
-    lsp_refactor_report r{};
-    r.first_bad_circuit = -1;
-    r.ms = t_refactor_;
-    for (int m = 0; m < nc; m++) {
-        if (subset && !(*subset)[m]) continue;
-        r.bad_pivots += ms[4 * m];
-        if (ms[4 * m + 2] != 0x7fffffff) r.singular_circuits++;
-        if ((ms[4 * m] || ms[4 * m + 2] != 0x7fffffff || ms[4 * m + 3]) && r.first_bad_circuit < 0)
-            r.first_bad_circuit = m;
-    }
-    const char *tl_path = getenv("LSP_REFACTOR_TIMELINE");
-    if (tl_path && d_timeline_) {
+    if (refactor_tl_path_ && d_timeline_) {
         std::vector<long long> tl((size_t)ntot_ * 4);
         (void)hipMemcpy(tl.data(), d_timeline_, tl.size() * sizeof(long long), hipMemcpyDeviceToHost);
         std::vector<int2> tk(ntot_);
         (void)hipMemcpy(tk.data(), d_ftask_, ntot_ * sizeof(int2), hipMemcpyDeviceToHost);
-        if (FILE *f = fopen(tl_path, "wb")) {
+        if (FILE *f = fopen(refactor_tl_path_, "wb")) {
             fwrite(tl.data(), sizeof(long long), tl.size(), f);
             fwrite(tk.data(), sizeof(int2), tk.size(), f);
             fclose(f);
         }
     }
-    if (abort_code == kErrSpin) {
+    if (h_pin_[0] == kErrSpin) {
         int dbg[8] = {0};
         (void)hipMemcpy(dbg, d_dbg_, sizeof(dbg), hipMemcpyDeviceToHost);
         int flag_now = -99;
@@ -911,6 +950,104 @@ int Batch::refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *
                      std::to_string(dbg[0]) + " saw " + std::to_string(dbg[1]) + " epoch " +
                      std::to_string(dbg[2]) + " spins " + std::to_string(dbg[3]) +
                      "; flag in memory after the launch: " + std::to_string(flag_now);
+    }
+    return LSP_OK;
+}
+
+// New pivot sequences for the candidate circuits, chosen on the host by the
+// reference's partial pivoting (lsp_gp_factor, tol = pivot_tol) on each
+// circuit's current values.  A circuit whose new sequence equals its current
+// one keeps its structure; if any changes, the device state is rebuilt with
+// every circuit's current values (release_device + upload_all).
+int Batch::repivot(const std::vector<int> &cands, int *changed)
+{
+    *changed = 0;
+    if (cands.empty()) return LSP_OK;
+    std::vector<double> vals(nnz_tot_);
+    int rc = check(hipMemcpyAsync(vals.data(), d_ain_, nnz_tot_ * sizeof(double), hipMemcpyDeviceToHost,
+                                  stream_), "values");
+    if (!rc) rc = check(hipStreamSynchronize(stream_), "sync");
+    if (rc) return rc;
+    std::vector<CircuitHost> fresh(cands.size());
+    std::vector<int> rcs(cands.size(), LSP_OK);
+    {
+        std::atomic<int> next{0};
+        const int nt = (int)std::min<size_t>(cands.size(),
+                                             std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nt; t++)
+            pool.emplace_back([&]() {
+                for (int i = next++; i < (int)cands.size(); i = next++) {
+                    const CircuitHost &C = circ_[cands[i]];
+                    lsp_circuit_in in{};
+                    in.n = C.n;
+                    in.nnz = C.nnz;
+                    in.row = C.row.data();
+                    in.col = C.col.data();
+                    in.val = vals.data() + C.aoff;
+                    in.q = C.q.data();
+                    rcs[i] = build_circuit(fresh[i], in, opt_);
+                }
+            });
+        for (auto &th : pool) th.join();
+    }
+    for (size_t i = 0; i < cands.size(); i++) {
+        if (rcs[i] == LSP_E_SINGULAR) continue;             // singular for the reference too
+        if (rcs[i] != LSP_OK) return rcs[i];
+        CircuitHost &C = circ_[cands[i]];
+        if (fresh[i].plan.pinv == C.plan.pinv) continue;    // the host confirms the current pivots
+        C.sym = std::move(fresh[i].sym);
+        C.plan = std::move(fresh[i].plan);
+        C.sell = std::move(fresh[i].sell);
+        (*changed)++;
+    }
+    if (*changed == 0) return LSP_OK;
+    // upload_all() takes every circuit's values from val0
+    for (auto &C : circ_) std::copy(vals.begin() + C.aoff, vals.begin() + C.aoff + C.nnz, C.val0.begin());
+    release_device();
+    rc = upload_all();
+    if (rc && last_error.empty()) last_error = "rebuild after new pivot sequences failed";
+    return rc;
+}
+
+int Batch::refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *subset)
+{
+    const int nc = (int)circ_.size();
+    std::vector<int> ms;
+    int rc = refactor_pass(prec, subset, ms);
+    if (rc) return rc;
+    auto in_subset = [&](int m) { return !subset || (*subset)[m]; };
+    lsp_refactor_report r{};
+    for (int m = 0; m < nc; m++)
+        if (in_subset(m)) r.pivot_mismatch += ms[kMstat * m + 4];
+    if (opt_.repivot > 0 && h_pin_[0] == kErrNone) {
+        std::vector<int> cands;
+        for (int m = 0; m < nc; m++) {
+            if (!in_subset(m)) continue;
+            const int *w = &ms[kMstat * m];
+            const bool unstable = w[0] > 0 || w[2] != 0x7fffffff;
+            const bool differs = opt_.repivot == 2 && (w[4] > 0 || w[5] > 0);
+            if (unstable || differs) cands.push_back(m);
+        }
+        int changed = 0;
+        rc = repivot(cands, &changed);
+        if (rc) return rc;
+        if (changed) {
+            // the rebuild dropped every factor set: refactor every circuit
+            r.repivoted = changed;
+            rc = refactor_pass(prec, nullptr, ms);
+            if (rc) return rc;
+        }
+    }
+    const int abort_code = h_pin_[0];
+    r.first_bad_circuit = -1;
+    r.ms = t_refactor_;
+    for (int m = 0; m < nc; m++) {
+        if (!in_subset(m)) continue;
+        const int *w = &ms[kMstat * m];
+        r.bad_pivots += w[0];
+        if (w[2] != 0x7fffffff) r.singular_circuits++;
+        if ((w[0] || w[2] != 0x7fffffff || w[3]) && r.first_bad_circuit < 0) r.first_bad_circuit = m;
     }
     r.status = abort_code == kErrSpin ? LSP_E_DEVICE
              : abort_code == kErrCapacity ? LSP_E_CAPACITY
@@ -943,7 +1080,7 @@ int Batch::trisolve_t(FactorSet<T> &fs, const double *d_b, double *d_x, const in
     Lf.cs = scaled ? d_cscale_ : nullptr;
     Lf.n_active = d_nact;
     Lf.timeline = nullptr;
-    if (getenv("LSP_TRISOLVE_TIMELINE")) {                   // diagnostic dump (Batch::trisolve)
+    if (trisolve_tl_path_) {                                 // diagnostic dump (Batch::trisolve)
         if (!d_tri_tl_) d_tri_tl_ = (long long *)dalloc((size_t)(nfw_tot_ + nbw_tot_) * 2 * sizeof(long long));
         Lf.timeline = d_tri_tl_;
     }
@@ -1025,7 +1162,7 @@ int Batch::trisolve(const double *b, double *x, bool on_device)
     if (!rc) rc = check(hipStreamSynchronize(stream_), "sync");
     if (rc) return rc;
     account_trisolve();
-    const char *tl_path = getenv("LSP_TRISOLVE_TIMELINE");
+    const char *tl_path = trisolve_tl_path_;
     if (tl_path && d_tri_tl_) {
         // [nfw, nbw] (int64), forward then backward task lists (int32 x 2 per
         // task), then per task: ticket granted, task finished (int64 ticks)
@@ -1205,8 +1342,11 @@ int Batch::solve(const double *b, double *x, bool on_device, lsp_solve_report *r
             const int keep = cur_prec_;
             lsp_refactor_report rr{};
             rc = refactor(LSP_PREC_FP64, &rr, &sub);
-            cur_prec_ = keep;
+            // a rebuild (new pivot sequences) refactored every circuit in fp64
+            // and dropped the other factor sets
+            if (!rr.repivoted) cur_prec_ = keep;
             if (rc && rc != LSP_E_SINGULAR) return rc;
+            r.singular_fallback = rr.singular_circuits;
             rc = check(k_zero_int(d_abort_ + 1, 1, stream_), "abort reset");
             if (rc) return rc;
             lsp_solve_report r2{};
@@ -1223,6 +1363,7 @@ int Batch::solve(const double *b, double *x, bool on_device, lsp_solve_report *r
             for (int m = 0; m < nc; m++) {
                 double rel = sub[m] ? (nb[m] > 0 ? b2[m] / nb[m] : b2[m])
                                     : (nb[m] > 0 ? beta[m] / nb[m] : beta[m]);
+                if (!(rel == rel)) rel = INFINITY;             // a diverged circuit stays visible
                 r.max_rel_res = std::max(r.max_rel_res, rel);
             }
         }
@@ -1409,7 +1550,7 @@ int Batch::graph_step(int prec, lsp_refactor_report *rrep, lsp_solve_report *sre
     if (rc) return rc;
     cur_prec_ = prec;
     const int nc = (int)circ_.size();
-    std::vector<int> ms(4 * nc), jf(nc);
+    std::vector<int> ms(kMstat * nc), jf(nc);
     std::vector<double> beta(nc), nb(nc);
     int ab[2] = {0, 0};
     rc = check(hipMemcpyAsync(ms.data(), d_mstat_, ms.size() * 4, hipMemcpyDeviceToHost, stream_), "mstat");
@@ -1425,13 +1566,39 @@ int Batch::graph_step(int prec, lsp_refactor_report *rrep, lsp_solve_report *sre
     lsp_refactor_report r{};
     r.first_bad_circuit = -1;
     for (int m = 0; m < nc; m++) {
-        r.bad_pivots += ms[4 * m];
-        if (ms[4 * m + 2] != 0x7fffffff) r.singular_circuits++;
-        if ((ms[4 * m] || ms[4 * m + 2] != 0x7fffffff || ms[4 * m + 3]) && r.first_bad_circuit < 0)
-            r.first_bad_circuit = m;
+        const int *w = &ms[kMstat * m];
+        r.bad_pivots += w[0];
+        r.pivot_mismatch += w[4];
+        if (w[2] != 0x7fffffff) r.singular_circuits++;
+        if ((w[0] || w[2] != 0x7fffffff || w[3]) && r.first_bad_circuit < 0) r.first_bad_circuit = m;
     }
     r.status = ab[0] == kErrSpin ? LSP_E_DEVICE : ab[0] == kErrCapacity ? LSP_E_CAPACITY
              : r.singular_circuits ? LSP_E_SINGULAR : LSP_OK;
+    if (opt_.repivot > 0 && ab[0] == kErrNone) {
+        std::vector<int> cands;
+        for (int m = 0; m < nc; m++) {
+            const int *w = &ms[kMstat * m];
+            if (w[0] > 0 || w[2] != 0x7fffffff || (opt_.repivot == 2 && (w[4] > 0 || w[5] > 0)))
+                cands.push_back(m);
+        }
+        int changed = 0;
+        rc = repivot(cands, &changed);
+        if (rc) return rc;
+        if (changed) {
+            // new pivot sequences dropped the graph and the factors: this
+            // step runs eagerly (the next graph_step records a new graph)
+            lsp_refactor_report r2{};
+            rc = refactor(prec, &r2);
+            r2.repivoted += changed;
+            r2.pivot_mismatch = r.pivot_mismatch;
+            if (rrep) *rrep = r2;
+            if (rc) return rc;
+            lsp_solve_report s2{};
+            rc = solve(d_b_, d_x_, true, &s2);
+            if (srep) *srep = s2;
+            return rc == LSP_E_NOT_CONVERGED ? LSP_OK : rc;
+        }
+    }
     if (rrep) *rrep = r;
     if (r.status == LSP_E_DEVICE || r.status == LSP_E_CAPACITY) return r.status;
     if (ab[1]) return LSP_E_DEVICE;

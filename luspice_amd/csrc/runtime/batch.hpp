@@ -50,72 +50,13 @@ struct FactorSet {                       // per storage precision
     bool ready = false;                  // arrays allocated
 };
 
-class Batch {
-public:
-    ~Batch();
-    int create(const lsp_circuit_in *c, int nc, const lsp_options *opt);
-    int prepare(const lsp_circuit_in *c, int nc, const lsp_options *opt);   // host symbolic only
-    int upload_all();                                                         // device phase
-    size_t estimate_device_bytes() const;
-    size_t device_bytes() const;
-    int set_stream(hipStream_t s);
-    int set_values(const double *vals, bool on_device);
-    int refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *subset = nullptr);
-    int launch_refactor_prec(int prec, const int *d_active);
-    int record_step(int prec);
-    int trisolve(const double *b, double *x, bool on_device);
-    int solve(const double *b, double *x, bool on_device, lsp_solve_report *rep);
-    int spmv(const double *x, double *y, bool on_device, int prec);
-    int info(int c, lsp_circuit_info *inf) const;
-    int export_factors(int c, int *pinv, int *Lp, int *Li, double *Lx, int *Up, int *Ui,
-                       double *Ux);
-    int device_vectors(double **b, double **x);
-    int graph_step(int prec, lsp_refactor_report *rrep, lsp_solve_report *srep);
-    int copy_vector(const double *p, bool to_rhs);
-    int last_times(double *rf, double *fw, double *bw) const;
-    int kernel_stats(lsp_kernel_stats *st, int reset);
-    int set_ir_params(int outer, int inner, double inner_tol, double outer_tol);
-    int diag_stats(int c, double *min_abs, double *max_abs);
-    const std::vector<double> &residual_history() const { return res_hist0_; }
-    const CircuitHost &circuit(int c) const { return circ_[c]; }
-    long total_n() const { return ntot_; }
-    long total_nnz() const { return nnz_tot_; }
-    int ncircuits() const { return (int)circ_.size(); }
-
-    // device-resident pieces used by the serving loop and the bench
-    int gmres_ir(const double *d_b, double *d_x, int prec, const std::vector<char> *subset,
-                 lsp_solve_report *rep);
-    int launch_trisolve_prec(int prec, const double *d_b, double *d_x, const int *active,
-                             const int *d_nact = nullptr);
-    int launch_spmv_krylov(const double *d_xv, double *d_y, const int *active);
-    int launch_spmv64(const double *d_x, double *d_y, const double *d_b, const int *active);
-    hipStream_t stream() const { return stream_; }
-    int cur_prec() const { return cur_prec_; }
-    std::string last_error;
-
-private:
-    template <typename T> int ensure_factor_set(FactorSet<T> &fs);
-    template <typename T> int refactor_t(FactorSet<T> &fs, const int *d_active);
-    template <typename T> int trisolve_t(FactorSet<T> &fs, const double *d_b, double *d_x,
-                                         const int *d_active, const int *d_nact = nullptr);
-    void *dalloc(size_t bytes);
-    void dfree(void *p);
-    int alloc_krylov(int mm);
-    template <typename X> X *upload(const std::vector<X> &v);
-    int check(hipError_t e, const char *what);
-    int sync_read_int(const int *d, int *h, int count);
-    KrylovArgs kargs() const;
-    void account_trisolve();
-
-    lsp_options opt_{};
-    hipStream_t stream_ = nullptr;
-    bool own_stream_ = false;
-    std::vector<CircuitHost> circ_;
-    std::vector<DevMem> mem_;
+// Everything upload_all() places on the device, with the sizes it derives.
+// Batch inherits it, so a rebuild (new pivot sequences) frees the memory and
+// resets every pointer and count with one assignment.
+struct DeviceArrays {
     long ntot_ = 0, nnz_tot_ = 0;
     int cu_count_ = 0;
-    int cur_prec_ = -1;
-
+    int mm_alloc_ = 0;
     // structure (device)
     int64_t *d_cp_ = nullptr, *d_relp_ = nullptr, *d_soff_ = nullptr;
     int *d_dp_ = nullptr, *d_pos_ = nullptr, *d_aptr_ = nullptr, *d_asrc_ = nullptr,
@@ -125,13 +66,9 @@ private:
         *d_scol_ = nullptr, *d_sperm_ = nullptr, *d_ssrc_ = nullptr;
     uint16_t *d_rel_ = nullptr;
     int *d_snend_ = nullptr, *d_ru_ = nullptr, *d_fcut_ = nullptr, *d_bcut_ = nullptr;
+    int *d_dpos_ = nullptr;              // per column: position of the diagonal row q[k]
     int *d_fpk_ = nullptr, *d_bpk_ = nullptr;
     double *d_tailW_ = nullptr;
-    // HIP graph of one Newton step (graph_step)
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t graph_exec_ = nullptr;
-    int graph_prec_ = -1;
-    bool capturing_ = false;
     // Ruiz equilibration (FP16 factor storage)
     int *d_arow_ = nullptr, *d_acol_ = nullptr;
     long *d_raoff_ = nullptr, *d_rvoff_ = nullptr;
@@ -151,9 +88,8 @@ private:
     int *d_ctr_ = nullptr;               // [0..1] factor, [2..3] fwd, [4..5] bwd
     int *d_epoch_ = nullptr;             // epoch counters: [0] factor, [1] fwd, [2] bwd
     int *d_abort_ = nullptr;             // [0] factor, [1] trisolve
-    int *d_mstat_ = nullptr;             // 4 per circuit
+    int *d_mstat_ = nullptr;             // kMstat per circuit
     int *d_dbg_ = nullptr;               // first stuck wait of the refactor
-    int next_epoch_peek_ = 0;
     long long *d_timeline_ = nullptr;    // optional refactor timeline (LSP_REFACTOR_TIMELINE)
     long long *d_tri_tl_ = nullptr;      // optional trisolve timeline (LSP_TRISOLVE_TIMELINE)
     double *d_scratch_ = nullptr;
@@ -185,11 +121,94 @@ private:
            *d_hn_ = nullptr, *d_hcol_ = nullptr;
     int *d_act_ = nullptr, *d_jfin_ = nullptr, *d_conv_ = nullptr, *d_cycled_ = nullptr,
         *d_nact_ = nullptr, *d_subset_ = nullptr;
+};
+
+class Batch : private DeviceArrays {
+public:
+    ~Batch();
+    int create(const lsp_circuit_in *c, int nc, const lsp_options *opt);
+    int prepare(const lsp_circuit_in *c, int nc, const lsp_options *opt);   // host symbolic only
+    int upload_all();                                                         // device phase
+    size_t estimate_device_bytes() const;
+    size_t device_bytes() const;
+    int set_stream(hipStream_t s);
+    int set_values(const double *vals, bool on_device);
+    int refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *subset = nullptr);
+    int launch_refactor_prec(int prec, const int *d_active);
+    int record_step(int prec);
+    int trisolve(const double *b, double *x, bool on_device);
+    int solve(const double *b, double *x, bool on_device, lsp_solve_report *rep);
+    int spmv(const double *x, double *y, bool on_device, int prec);
+    int info(int c, lsp_circuit_info *inf) const;
+    int export_factors(int c, int *pinv, int *Lp, int *Li, double *Lx, int *Up, int *Ui,
+                       double *Ux);
+    int device_vectors(double **b, double **x);
+    int graph_step(int prec, lsp_refactor_report *rrep, lsp_solve_report *srep);
+    int copy_vector(const double *p, bool to_rhs);
+    int last_times(double *rf, double *fw, double *bw) const;
+    int kernel_stats(lsp_kernel_stats *st, int reset);
+    int set_ir_params(int outer, int inner, double inner_tol, double outer_tol);
+    // repivot policy of later refactors (lsp_options::repivot)
+    int set_repivot(int mode)
+    {
+        if (mode < 0 || mode > 2) return LSP_E_ARG;
+        opt_.repivot = mode;
+        return LSP_OK;
+    }
+    int diag_stats(int c, double *min_abs, double *max_abs);
+    const std::vector<double> &residual_history() const { return res_hist0_; }
+    const CircuitHost &circuit(int c) const { return circ_[c]; }
+    long total_n() const { return ntot_; }
+    long total_nnz() const { return nnz_tot_; }
+    int ncircuits() const { return (int)circ_.size(); }
+
+    // device-resident pieces used by the serving loop and the bench
+    int gmres_ir(const double *d_b, double *d_x, int prec, const std::vector<char> *subset,
+                 lsp_solve_report *rep);
+    int launch_trisolve_prec(int prec, const double *d_b, double *d_x, const int *active,
+                             const int *d_nact = nullptr);
+    int launch_spmv_krylov(const double *d_xv, double *d_y, const int *active);
+    int launch_spmv64(const double *d_x, double *d_y, const double *d_b, const int *active);
+    hipStream_t stream() const { return stream_; }
+    int cur_prec() const { return cur_prec_; }
+    std::string last_error;
+
+private:
+    template <typename T> int ensure_factor_set(FactorSet<T> &fs);
+    int refactor_pass(int prec, const std::vector<int> *subset, std::vector<int> &ms);
+    int repivot(const std::vector<int> &cands, int *changed);
+    void release_device();
+    template <typename T> int refactor_t(FactorSet<T> &fs, const int *d_active);
+    template <typename T> int trisolve_t(FactorSet<T> &fs, const double *d_b, double *d_x,
+                                         const int *d_active, const int *d_nact = nullptr);
+    void *dalloc(size_t bytes);
+    void dfree(void *p);
+    int alloc_krylov(int mm);
+    template <typename X> X *upload(const std::vector<X> &v);
+    int check(hipError_t e, const char *what);
+    int sync_read_int(const int *d, int *h, int count);
+    KrylovArgs kargs() const;
+    void account_trisolve();
+
+    lsp_options opt_{};
+    hipStream_t stream_ = nullptr;
+    bool own_stream_ = false;
+    std::vector<CircuitHost> circ_;
+    std::vector<DevMem> mem_;
+    int cur_prec_ = -1;
+    bool uploaded_ = false;              // upload_all done (cleared by release_device)
+
+    // HIP graph of one Newton step (graph_step)
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    int graph_prec_ = -1;
+    bool capturing_ = false;
     int *h_pin_ = nullptr;               // pinned host words for flag reads
+    const char *refactor_tl_path_ = nullptr;   // LSP_REFACTOR_TIMELINE, read at creation
+    const char *trisolve_tl_path_ = nullptr;   // LSP_TRISOLVE_TIMELINE, read at creation
     hipEvent_t ev_[6] = {};
     float t_refactor_ = 0, t_fwd_ = 0, t_bwd_ = 0;
     bool tri_pending_ = false;
-    int mm_alloc_ = 0;
     std::vector<double> res_hist0_;      // ||r||/||b|| of circuit 0 at each restart
     lsp_kernel_stats stats_{};
 };

@@ -44,6 +44,7 @@ int lsp_options_default(lsp_options *o)
     o->chunk_rows = 0;
     o->sell_sigma = 0;
     o->fallback_fp64 = 1;
+    o->repivot = 1;
     return LSP_OK;
 }
 

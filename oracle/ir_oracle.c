@@ -7,7 +7,7 @@
  *   orc_residual      src/mp/mp_lu_bench.cu:188-203 (||A x - b||_2 / ||b||_2)
  *   orc_rel_l2        src/mp/mp_lu_bench.cu:205-214
  *   orc_csr_spmv      src/spmv.cu:154-171 (one row, entries in CSR order)
- *   orc_trisolve_f32  src/lu_gpu_f32.cu:255-285 (fp32 factors read as double,
+ *   orc_trisolve_f32  src/lu_gpu_f32.cu:94-124 (fp32 factors read as double,
  *                     fp64 accumulation, x[q[k]] = z[k])
  *   orc_gmres_ir      bench/test_lu_gpu_f32.cu:540-715 (right-preconditioned
  *                     restarted GMRES(m), x0 = 0, two-pass modified

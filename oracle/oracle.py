@@ -51,6 +51,7 @@ def lib():
         L.orc_cpu_lu_ref.restype = C.c_int
         L.orc_cpu_lu_ref.argtypes = [C.c_int, ip, ip, dp, ip, dp, C.c_double, dp, lp, lp]
         L.orc_level_compute.argtypes = [C.c_int, ip, ip, ip, ip]
+        L.orc_level_hist.argtypes = [C.c_int, ip, C.c_int, ip]
         L.orc_residual.restype = C.c_double
         L.orc_residual.argtypes = [C.c_int, C.c_long, ip, ip, dp, dp, dp]
         L.orc_rel_l2.restype = C.c_double
@@ -91,6 +92,7 @@ def ref():
         L.cpu_lu_ref.restype = C.c_int
         L.cpu_lu_ref.argtypes = [C.c_int, ip, ip, dp, ip, dp, C.c_double, dp, lp, lp]
         L.dsh_level_compute.argtypes = [C.c_int, ip, ip, ip, ip]
+        L.dsh_level_hist.argtypes = [C.c_int, ip, C.c_int, ip]
         _ref = L
     return _ref
 
@@ -173,6 +175,16 @@ def levels(n, Up, Ui, which="oracle"):
     fn(n, _p(np.ascontiguousarray(Up, np.int32)), _p(np.ascontiguousarray(Ui, np.int32)),
        _p(lv), C.byref(nl))
     return lv, nl.value
+
+
+def level_hist(level, nlevels, which="oracle"):
+    """hist[l] = number of columns at level l (src/level.c:43-48)."""
+    lv = np.ascontiguousarray(level, np.int32)
+    hist = np.empty(max(int(nlevels), 1), np.int32)
+    L = lib() if which == "oracle" else ref()
+    fn = L.orc_level_hist if which == "oracle" else L.dsh_level_hist
+    fn(len(lv), _p(lv), int(nlevels), _p(hist))
+    return hist[:int(nlevels)]
 
 
 def residual(n, row, col, val, b, x):

@@ -15,6 +15,13 @@ Fixtures:
   tiny.npz      hand-made cases: a source/ground MNA with exact +-1 pivot
                 ties, an 8x8 unsymmetric matrix, a structurally singular
                 matrix (empty column) and a numerically singular one.
+  c1.json       the 316x316 power grid + 50 pads (n = 99906, seed 1): the
+                matrix's and q's sha256, and the sha256 of every integer and
+                fp64 array the reference's splu, cpu_lu_ref-chain solve and
+                level.c produce on it (pinv, Lp, Li, Lx, Up, Ui, Ux, x,
+                levels, level histogram), lnnz / unnz / nlevels and a strided
+                sample of x.  Hashes keep the fixture small (the arrays are
+                tens of MB).
   selector.json the five cases of the reference's bench/test_selector.c:36-87
                 with their expected decisions, plus the compiled selector's
                 outputs on 200 random feature vectors.
@@ -22,6 +29,7 @@ Fixtures:
 from __future__ import annotations
 
 import ctypes as C
+import hashlib
 import json
 import os
 import sys
@@ -69,6 +77,30 @@ def c0_fixture():
     out.update(factor_arrays(F32, "f32"))
     np.savez_compressed(os.path.join(HERE, "c0.npz"), **out)
     print(f"c0: n={c.n} lnnz={ln} unnz={un} levels={nl}")
+
+
+def sha(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def c1_fixture():
+    import luspice_amd as L
+    c = L.mna("c1", seed=1)
+    q = L.colamd(c)
+    F = O.lu_factor(c.n, c.row, c.col, c.val, q, which="ref64")
+    assert F["singular"] == -1
+    rc, x, ln, un = O.cpu_lu_ref(c.n, c.row, c.col, c.val, q, c.rhs, which="ref64_factor_solve")
+    assert rc == -1
+    lv, nl = O.levels(c.n, F["Up"], F["Ui"], which="ref")
+    hist = O.level_hist(lv, nl, which="ref")
+    out = dict(n=c.n, nnz=c.nnz, lnnz=int(ln), unnz=int(un), nlevels=int(nl),
+               sha=dict(row=sha(c.row), col=sha(c.col), val=sha(c.val), rhs=sha(c.rhs), q=sha(q),
+                        x=sha(x), level=sha(lv), hist=sha(hist),
+                        **{k: sha(F[k]) for k in ("pinv", "Lp", "Li", "Lx", "Up", "Ui", "Ux")}),
+               x_stride=97, x_sample=[float(v) for v in x[::97]])
+    with open(os.path.join(HERE, "c1.json"), "w") as f:
+        json.dump(out, f)
+    print(f"c1: n={c.n} lnnz={ln} unnz={un} levels={nl}")
 
 
 def tiny_cases():
@@ -162,5 +194,6 @@ if __name__ == "__main__":
     if not O.ref_available():
         raise SystemExit("oracle/_ref/libref.so missing: run `make -C oracle ref` first")
     c0_fixture()
+    c1_fixture()
     tiny_cases()
     selector_fixture()

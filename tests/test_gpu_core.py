@@ -149,8 +149,9 @@ def test_batch_uneven_same_alone_and_mixed():
 
 
 def test_zero_pivot_reported_then_recovered():
-    """A Newton step whose values make a fixed pivot exactly zero: the refactor
-    reports the circuit as singular (LSP_E_SINGULAR, first bad circuit 0) and
+    """With repivot = 0 (pivots kept, failures only reported), a Newton step
+    whose values make a fixed pivot exactly zero: the refactor reports the
+    circuit as singular (LSP_E_SINGULAR, first bad circuit 0) and
     every kernel returns; the next step's values factor and solve normally.
     The 3 x 3 circuit has no dense trailing block, so the sparse refactor
     kernel takes the zero pivot."""
@@ -160,7 +161,8 @@ def test_zero_pivot_reported_then_recovered():
     col = np.array([0, 1, 0, 1, 2, 1, 2], np.int32)
     v0 = np.array([4.0, 1.0, 1.0, 4.0, 1.0, 1.0, 4.0])
     b = np.array([1.0, 2.0, 3.0])
-    B = L.Batch(L.Circuit(3, row, col, v0, b, q=np.arange(3, dtype=np.int32)), tail_max=tail_max)
+    B = L.Batch(L.Circuit(3, row, col, v0, b, q=np.arange(3, dtype=np.int32)), tail_max=tail_max,
+                repivot=0)
     # U(1,1) = 0.5 - 1 * 1 / 2 = 0 exactly with the fixed diagonal pivots
     B.set_values(np.array([2.0, 1.0, 1.0, 0.5, 1.0, 1.0, 4.0]))
     rep = B.refactor("fp64")

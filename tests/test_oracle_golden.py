@@ -336,3 +336,165 @@ def test_perturb_is_seeded_lognormal():
     r = np.log(v1 / c.val)
     assert abs(r.mean()) < 0.02 and abs(r.std() - 0.1) < 0.02
     assert np.all(np.abs(b1) <= 1)
+
+
+# --------------------------------------------------------------------------
+# config 1 (n = 99906): integer structure and fp64 values pinned by the
+# hashes the reference's own splu / level.c wrote (tests/golden/c1.json)
+@pytest.fixture(scope="module")
+def g1():
+    with open(os.path.join(GOLD, "c1.json")) as f:
+        return json.load(f)
+
+
+@pytest.fixture(scope="module")
+def c1_case(g1):
+    c = L.mna("c1", seed=1)
+    q = L.colamd(c)
+    return c, q
+
+
+def _sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def test_c1_inputs_match_fixture(g1, c1_case):
+    """Generator and COLAMD are deterministic: the matrix, RHS and q the
+    fixture was written from are reproduced byte for byte."""
+    c, q = c1_case
+    assert c.n == g1["n"] and c.nnz == g1["nnz"]
+    for k, a in (("row", c.row), ("col", c.col), ("val", c.val), ("rhs", c.rhs), ("q", q)):
+        assert _sha(a) == g1["sha"][k], k
+
+
+def test_product_symbolic_bitwise_c1(g1, c1_case):
+    c, q = c1_case
+    F = L.symbolic_factor(c, q)
+    assert len(F["Li"]) == g1["lnnz"] and len(F["Ui"]) == g1["unnz"]
+    for k in KEYS:
+        assert _sha(F[k]) == g1["sha"][k], k
+    import ctypes as C
+    from luspice_amd import _native as N
+    lv = np.empty(c.n, np.int32)
+    nl = C.c_int()
+    N.lib.dsh_level_compute(c.n, N.ptr(F["Up"]), N.ptr(F["Ui"]), N.ptr(lv), C.byref(nl))
+    assert nl.value == g1["nlevels"] and _sha(lv) == g1["sha"]["level"]
+    hist = np.empty(nl.value, np.int32)
+    N.lib.dsh_level_hist(c.n, N.ptr(lv), nl.value, N.ptr(hist))
+    assert _sha(hist) == g1["sha"]["hist"]
+
+
+def test_product_cpu_lu_ref_bitwise_c1(g1, c1_case):
+    c, q = c1_case
+    rc, x, ln, un = L.cpu_lu_ref(c, q)
+    assert rc == -1 and ln == g1["lnnz"] and un == g1["unnz"]
+    assert _sha(x) == g1["sha"]["x"]
+
+
+def test_oracle_bitwise_c1(g1, c1_case):
+    c, q = c1_case
+    F = O.lu_factor(c.n, c.row, c.col, c.val, q)
+    for k in KEYS:
+        assert _sha(F[k]) == g1["sha"][k], k
+    lv, nl = O.levels(c.n, F["Up"], F["Ui"])
+    assert nl == g1["nlevels"] and _sha(lv) == g1["sha"]["level"]
+    hist = O.level_hist(lv, nl)
+    assert _sha(hist) == g1["sha"]["hist"]
+    assert np.array_equal(hist, np.bincount(lv, minlength=nl))
+    rc, x, _, _ = O.cpu_lu_ref(c.n, c.row, c.col, c.val, q, c.rhs)
+    assert rc == -1 and _sha(x) == g1["sha"]["x"]
+    assert np.array_equal(x[::g1["x_stride"]], np.array(g1["x_sample"]))
+
+
+# --------------------------------------------------------------------------
+# the SpMV, residual and level-histogram checkers, pinned by what the
+# mathematics fixes
+def _rand_coo(rng, n, per_row, lo=-9, hi=10, dup=True):
+    row = np.repeat(np.arange(n), per_row).astype(np.int32)
+    col = rng.integers(0, n, row.size).astype(np.int32)
+    val = rng.integers(lo, hi, row.size).astype(np.float64)
+    if dup:                                     # a repeated (i, j) entry is summed
+        row = np.append(row, row[:5]).astype(np.int32)
+        col = np.append(col, col[:5]).astype(np.int32)
+        val = np.append(val, [3.0, -2.0, 7.0, 1.0, -5.0])
+    return row, col, val
+
+
+def _csr(n, row, col, val):
+    order = np.lexsort((np.arange(len(row)), row))
+    rp = np.zeros(n + 1, np.int32)
+    np.add.at(rp, row + 1, 1)
+    return np.cumsum(rp).astype(np.int32), col[order], val[order]
+
+
+def test_oracle_csr_spmv_exact_on_integers():
+    """Small integers keep every product and partial sum exact, so y must equal
+    the dense product bit for bit: a wrong column index, a transposed operand or
+    a dropped duplicate changes an exact integer."""
+    rng = np.random.default_rng(21)
+    n = 300
+    row, col, val = _rand_coo(rng, n, 6)
+    A = np.zeros((n, n))
+    np.add.at(A, (row, col), val)
+    x = rng.integers(-50, 51, n).astype(np.float64)
+    rp, ci, vv = _csr(n, row, col, val)
+    y = O.csr_spmv(n, rp, ci, vv, x)
+    assert np.array_equal(y, A @ x)
+    assert not np.array_equal(y, A.T @ x)
+    # an empty first row gives 0; the other rows shift down by one
+    y2 = O.csr_spmv(n, np.concatenate([[0, 0], rp[1:n]]).astype(np.int32), ci, vv, x)
+    assert y2[0] == 0.0 and np.array_equal(y2[1:], y[:-1])
+
+
+def test_oracle_csr_spmv_real_values():
+    rng = np.random.default_rng(22)
+    n = 500
+    row, col, _ = _rand_coo(rng, n, 5)
+    val = rng.uniform(-1, 1, row.size)
+    A = np.zeros((n, n))
+    np.add.at(A, (row, col), val)
+    x = rng.uniform(-1, 1, n)
+    rp, ci, vv = _csr(n, row, col, val)
+    y = O.csr_spmv(n, rp, ci, vv, x)
+    np.testing.assert_allclose(y, A @ x, rtol=1e-13, atol=1e-14)
+
+
+def test_oracle_residual_closed_form():
+    """||A x - b|| / ||b|| with b = A x + d e_i (exact in integers) is
+    |d| / ||b||; b = A x gives exactly 0; the COO entries are 1-based."""
+    rng = np.random.default_rng(23)
+    n = 200
+    row, col, val = _rand_coo(rng, n, 4)
+    A = np.zeros((n, n))
+    np.add.at(A, (row, col), val)
+    x = rng.integers(-20, 21, n).astype(np.float64)
+    b = A @ x
+    assert O.residual(n, row, col, val, b, x) == 0.0
+    for i, d in ((0, 3.0), (137, -0.5), (n - 1, 2.0 ** -10)):   # dyadic: b + d e_i is exact
+        bb = b.copy()
+        bb[i] += d
+        want = abs(d) / np.linalg.norm(bb)
+        assert abs(O.residual(n, row, col, val, bb, x) - want) <= 1e-15 * want + 1e-300
+    # the residual of a wrong x scales with the error
+    xe = x.copy()
+    xe[5] += 1.0
+    want = np.linalg.norm(A[:, 5]) / np.linalg.norm(b)
+    assert abs(O.residual(n, row, col, val, b, xe) - want) <= 1e-14 * want
+
+
+def test_oracle_rel_l2_closed_form():
+    a = np.array([3.0, 4.0, 0.0])
+    b = np.array([0.0, 0.0, 5.0])
+    assert O.rel_l2(a, b) == np.sqrt(50.0) / 5.0
+    assert O.rel_l2(b, b) == 0.0
+
+
+def test_oracle_level_hist_matches_compiled_reference_and_bincount():
+    rng = np.random.default_rng(24)
+    for nl in (1, 7, 40):
+        lv = rng.integers(0, nl, 1000).astype(np.int32)
+        h = O.level_hist(lv, nl)
+        assert np.array_equal(h, np.bincount(lv, minlength=nl))
+        if O.ref_available():
+            assert np.array_equal(h, O.level_hist(lv, nl, which="ref"))
