@@ -15,7 +15,19 @@ Every result is checked against the fp64 CPU solution of the reference's own
 splu (oracle/_ref, compiled from the reference sources) or, when that build
 is absent, the oracle's C restatement; the line reports the relative error.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Beside the headline the line carries:
+  cpu_baseline            the CPU fp64 path (splu factor + solve), 1 thread
+  cpu_baseline_all_cores  the same on every CPU of this process's share at
+                          once (independent solves), core count and model
+  cpu_mixed_c0            the reference's CPU mixed path (fp32 factors +
+                          fp64 GMRES-IR) on the 1020-unknown circuit
+  batched_newton          512 circuits (n in [100, 5000], seed 3) x 20 Newton
+                          steps: the resident batched path with fp32 and
+                          with fp64 factors against the plain
+                          one-matrix-per-launch fp64 path, rel err of every
+                          circuit vs the fp64 CPU LU, step p50/p99
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--newton-circuits 512] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -87,6 +99,62 @@ def algorithmic_bytes(inf, nnz, n, vF):
     return fac, tri, spmv
 
 
+def host_cpus():
+    """CPUs this process may run on (its affinity mask), at most 16: the GPU
+    box gives one GPU's process a 16-CPU share whatever the machine holds."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_all_cores(O, c, q, which, workers, runs_per_worker=2):
+    """The CPU fp64 path on every worker thread at once: each thread factors
+    and solves its own copy of the matrix (independent solves, as a CPU
+    Newton farm would); the oracle's C releases the GIL, so the threads run
+    in parallel.  Returns (solves/s over the wall time, wall seconds)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def work(_):
+        for _r in range(runs_per_worker):
+            rc, _x, _ln, _un = O.cpu_lu_ref(c.n, c.row, c.col, c.val, q, c.rhs, which=which)
+            assert rc == -1, rc
+        return runs_per_worker
+
+    t1 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        done = sum(ex.map(work, range(workers)))
+    wall = time.perf_counter() - t1
+    return done / wall, wall
+
+
+def cpu_mixed_c0(L, O, repeats=20):
+    """The reference's CPU mixed path on the 1020-unknown circuit
+    (bench/test_lu_gpu_f32.cu:540-715): fp32-storage splu factors (the
+    reference's own splu compiled with fp32 values when oracle/_ref is built,
+    else the oracle's fp64 factors rounded to fp32) preconditioning fp64
+    GMRES-IR to the 1e-10 true residual; timed factor + solve, 1 thread."""
+    c = L.mna("c0", seed=0)
+    q = L.colamd(c)
+    which = "ref32" if O.ref_available() else "oracle"
+    rc, xr, _, _ = O.cpu_lu_ref(c.n, c.row, c.col, c.val, q, c.rhs,
+                                which="ref64" if O.ref_available() else "oracle")
+    times, err, info = [], None, None
+    for _ in range(repeats):
+        t1 = time.perf_counter()
+        F = O.lu_factor(c.n, c.row, c.col, c.val, q, which=which)
+        F32 = dict(F, Lx=F["Lx"].astype(np.float32), Ux=F["Ux"].astype(np.float32))
+        x, info = O.gmres_ir(c.n, c.row, c.col, c.val, c.rhs, F32, q)
+        times.append(time.perf_counter() - t1)
+        err = float(np.linalg.norm(x - xr) / np.linalg.norm(xr))
+    return {"value": round(1.0 / float(np.median(times)), 2), "unit": "refactor+solve/s", "cores": 1,
+            "kind": "reference" if which == "ref32" else "port",
+            "sample": f"{repeats} runs on c0 (n={c.n}, seed 0): {which} fp32-storage factor + the "
+                      f"oracle's GMRES-IR restatement (fp64 residual), median",
+            "rel_err": err, "ir_inner": int(info["inner"]), "ir_res": float(info["res"])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +164,11 @@ def main():
     ap.add_argument("--calib-mb", type=int, default=64,
                     help="size of the device copy issued before warm-up (counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-workers", type=int, default=0,
+                    help="threads of the all-cores CPU leg (0 = the CPUs this process may use, at most 16)")
+    ap.add_argument("--newton-circuits", type=int, default=512,
+                    help="circuits of the batched Newton comparison (0 = skip it)")
+    ap.add_argument("--newton-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the path runs on one GPU's share); "
@@ -132,7 +205,7 @@ def main():
         log(f"[bench] calibration copy {nb} B")
 
     # ---- reference fp64 solution (CPU baseline leg doubles as the checker) ----
-    cpu = None
+    cpu = cpu_all = cpu_mixed = None
     xref = None
     if not args.no_cpu:
         kind = "reference" if O.ref_available() else "port"
@@ -149,6 +222,15 @@ def main():
                "sample": f"{len(times)} full fp64 factor+solve runs of the n=99906 matrix "
                          f"(splu lu_factor + lu_solve_perm, same q), median, 1 thread, "
                          f"{cpu_model()}"}
+        P = args.cpu_workers or host_cpus()
+        rate, wall = cpu_all_cores(O, c, q, which, P)
+        log(f"[bench] cpu {kind} on {P} threads: {rate:.2f} solves/s ({wall:.1f}s)")
+        cpu_all = {"value": round(rate, 3), "unit": "refactor+solve/s", "cores": P, "kind": kind,
+                   "sample": f"{P} threads x 2 independent fp64 factor+solve runs of the n=99906 "
+                             f"matrix at once, solves over wall time; {cpu_model()}, "
+                             f"{os.cpu_count()} CPUs online, {P} in this process's share"}
+        cpu_mixed = cpu_mixed_c0(L, O)
+        log(f"[bench] cpu mixed path c0: {cpu_mixed['value']} /s err={cpu_mixed['rel_err']:.2e}")
 
     def run_steps(prec, k, BB=None):
         BB = BB or B
@@ -198,6 +280,15 @@ def main():
                "pass_1e-6": bool(all(r[1]["rc"] == 0 for r in m9["reps"]) and
                                  (m9["err"] is None or m9["err"] <= 1e-6))}
     log(f"[bench] selector tolerance: {sel_tol['value']:.2f} steps/s err={m9['err']}")
+
+    # the north-star comparison: the batched resident mixed path against the
+    # plain one-matrix-per-launch fp64 path on the Newton mix (512 circuits,
+    # n in [100, 5000], seed 3, 20 steps)
+    newton = None
+    if args.newton_circuits > 0:
+        from luspice_amd import bench_newton
+        B.close()
+        newton = bench_newton.compare(ncirc=args.newton_circuits, steps=args.newton_steps)
 
     st = m32["stats"]
     n, nnz = c.n, c.nnz
@@ -256,6 +347,9 @@ def main():
                      "algorithmic_bytes": int(ach_bytes), "duration_ms": round(dur, 4)},
         "roofline_other": [tri_roofline(tri_b, tri_ms)],
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
+        "cpu_mixed_c0": cpu_mixed,
+        "batched_newton": newton,
     }
     print(json.dumps(line), flush=True)
 

@@ -73,15 +73,21 @@ typedef struct {
                               residual b - A x is always taken with the fp64 values */
     int repivot;           /* new pivot sequences on the host when a refactor's fixed pivots
                               no longer fit the step's values (the reference's splu picks its
-                              pivots anew at every factorization, splu.h:174-185):
+                              pivots anew at every factorization, splu.h:174-185); a circuit
+                              given new pivots is refactored again within the same call:
                               0  never: failures are only reported;
-                              1  (default) a circuit with a zero pivot or a pivot failing
-                                 refactor_ptol gets the pivot sequence the reference's partial
-                                 pivoting (tol = pivot_tol) picks on the step's values, and the
-                                 batch is refactored again within the same call;
-                              2  also whenever that partial pivoting would pick a different
+                              1  (default) a circuit with a zero pivot gets the pivot sequence
+                                 the reference's partial pivoting (tol = pivot_tol) picks on the
+                                 step's values; small pivots are left to GMRES-IR and its fp64
+                                 fallback (a Newton refactor keeps its pivots, as KLU's does);
+                              2  also a circuit with a pivot failing refactor_ptol;
+                              3  also whenever that partial pivoting would pick a different
                                  pivot for any column (P equals the reference's at every step;
                                  meant for fp64 factors) */
+    int allow_fp16;        /* 1: accept LSP_PREC_FP16 factor storage (Ruiz-scaled).  Default 0:
+                              the FP16 factor path once faulted the GPU with no cause shown, so
+                              refactor / graph_step refuse FP16 with LSP_E_ARG unless the caller
+                              opts in */
 } lsp_options;
 
 typedef struct {
@@ -92,7 +98,7 @@ typedef struct {
     double ms;             /* device time of the refactor launch(es) */
     int repivoted;         /* circuits given a new pivot sequence by this call */
     int pivot_mismatch;    /* columns whose fixed pivot the reference's partial pivoting would
-                              not pick on these values (counted when repivot = 2, before any
+                              not pick on these values (counted when repivot = 3, before any
                               new pivot sequence) */
 } lsp_refactor_report;
 

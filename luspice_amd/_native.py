@@ -41,7 +41,8 @@ class Options(C.Structure):
                 ("refactor_blocks", C.c_int), ("trisolve_blocks", C.c_int),
                 ("tail_max", C.c_int), ("tail_density", C.c_double), ("ruiz_sweeps", C.c_int),
                 ("pivot_scaled", C.c_int), ("poll_backoff", C.c_int),
-                ("factor_shift", C.c_double), ("krylov_spmv", C.c_int), ("repivot", C.c_int)]
+                ("factor_shift", C.c_double), ("krylov_spmv", C.c_int), ("repivot", C.c_int),
+                ("allow_fp16", C.c_int)]
 
 
 class RefactorReport(C.Structure):

@@ -1,7 +1,7 @@
 """Precision sweep over ill-conditioned circuits (fp64 / fp32 / FP16+Ruiz).
 
     python -m luspice_amd.bench_precision [--matrices 64] [--nmin 1000] [--nmax 8000]
-                                          [--precs fp64 fp32 fp16]
+                                          [--precs fp64 fp32 [fp16]]
 
 Workload: ``matrices`` circuits from the precision-sweep generator (mna kind
 'c2', seed 2): n log-uniform in [nmin, nmax], ~6 nonzeros per row,
@@ -54,7 +54,9 @@ def main(argv=None):
     ap.add_argument("--nmin", type=int, default=1000)
     ap.add_argument("--nmax", type=int, default=8000)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--precs", nargs="+", default=["fp64", "fp32", "fp16"],
+    # FP16 storage is opt-in (lsp_options.allow_fp16): its GPU path once
+    # faulted with no cause shown, so a default run leaves it out
+    ap.add_argument("--precs", nargs="+", default=["fp64", "fp32"],
                     choices=["fp64", "fp32", "fp16"], help="factor storage precisions to run")
     args = ap.parse_args(argv)
     import luspice_amd as L
@@ -76,7 +78,7 @@ def main(argv=None):
             # fp64 / fp32 keep the default pivot sequence; FP16 pivots on the
             # Ruiz-scaled matrix it stores
             Bd = L.Batch(circ, fallback_fp64=fb)
-            Bs = L.Batch(circ, fallback_fp64=fb, pivot_scaled=1) if "fp16" in precs else None
+            Bs = L.Batch(circ, fallback_fp64=fb, pivot_scaled=1, allow_fp16=1) if "fp16" in precs else None
             for prec in precs:
                 B = Bs if prec == "fp16" else Bd
                 rr = B.refactor(prec)

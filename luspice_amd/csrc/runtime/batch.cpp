@@ -319,7 +319,7 @@ int Batch::prepare(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
 {
     if (!cin || nc <= 0 || !circ_.empty()) return LSP_E_ARG;
     if (opt) opt_ = *opt; else lsp_options_default(&opt_);
-    if (opt_.repivot < 0 || opt_.repivot > 2) return LSP_E_ARG;
+    if (opt_.repivot < 0 || opt_.repivot > 3) return LSP_E_ARG;
     refactor_tl_path_ = getenv("LSP_REFACTOR_TIMELINE");     // diagnostic dumps
     trisolve_tl_path_ = getenv("LSP_TRISOLVE_TIMELINE");
     circ_.resize(nc);
@@ -852,7 +852,7 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
     L.total_waves = fblocks_ * 4;
     L.poll_backoff = opt_.poll_backoff > 0 ? opt_.poll_backoff : 7;
     L.ptol = opt_.refactor_ptol;
-    L.stol = opt_.repivot == 2 ? opt_.pivot_tol : 0.0;
+    L.stol = opt_.repivot == 3 ? opt_.pivot_tol : 0.0;
     L.dbg = d_dbg_;
     rc = check(launch_reset_mstat(d_mstat_, (int)circ_.size(), d_abort_, d_dbg_, stream_), "reset");
     if (rc) return rc;
@@ -959,6 +959,17 @@ int Batch::refactor_pass(int prec, const std::vector<int> *subset, std::vector<i
 // circuit's current values.  A circuit whose new sequence equals its current
 // one keeps its structure; if any changes, the device state is rebuilt with
 // every circuit's current values (release_device + upload_all).
+// Whether a circuit's status words (kMstat) call for a new pivot sequence
+// under opt_.repivot: 1 a zero pivot, 2 also a pivot failing refactor_ptol,
+// 3 also any column where splu (tol = pivot_tol) would pick another pivot.
+bool Batch::repivot_wanted(const int *w) const
+{
+    const bool zero = w[2] != 0x7fffffff;
+    const bool unstable = w[0] > 0;
+    const bool differs = w[4] > 0 || w[5] > 0;
+    return (opt_.repivot >= 1 && zero) || (opt_.repivot >= 2 && unstable) || (opt_.repivot >= 3 && differs);
+}
+
 int Batch::repivot(const std::vector<int> &cands, int *changed)
 {
     *changed = 0;
@@ -1010,11 +1021,24 @@ int Batch::repivot(const std::vector<int> &cands, int *changed)
     return rc;
 }
 
+// FP16 factor storage is refused unless the options opt in (allow_fp16): the
+// path once faulted the GPU and the fault's cause was not shown.
+int Batch::check_prec(int prec)
+{
+    if (prec == LSP_PREC_FP32 || prec == LSP_PREC_FP64) return LSP_OK;
+    if (prec == LSP_PREC_FP16 && opt_.allow_fp16) return LSP_OK;
+    last_error = prec == LSP_PREC_FP16 ? "FP16 factor storage is off (lsp_options.allow_fp16 = 0)"
+                                       : "unknown factor precision";
+    return LSP_E_ARG;
+}
+
 int Batch::refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *subset)
 {
     const int nc = (int)circ_.size();
     std::vector<int> ms;
-    int rc = refactor_pass(prec, subset, ms);
+    int rc = check_prec(prec);
+    if (rc) return rc;
+    rc = refactor_pass(prec, subset, ms);
     if (rc) return rc;
     auto in_subset = [&](int m) { return !subset || (*subset)[m]; };
     lsp_refactor_report r{};
@@ -1025,9 +1049,7 @@ int Batch::refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *
         for (int m = 0; m < nc; m++) {
             if (!in_subset(m)) continue;
             const int *w = &ms[kMstat * m];
-            const bool unstable = w[0] > 0 || w[2] != 0x7fffffff;
-            const bool differs = opt_.repivot == 2 && (w[4] > 0 || w[5] > 0);
-            if (unstable || differs) cands.push_back(m);
+            if (repivot_wanted(w)) cands.push_back(m);
         }
         int changed = 0;
         rc = repivot(cands, &changed);
@@ -1518,7 +1540,8 @@ int Batch::record_step(int prec)
 int Batch::graph_step(int prec, lsp_refactor_report *rrep, lsp_solve_report *srep)
 {
     if (!stream_ || circ_.empty()) return LSP_E_ARG;
-    int rc;
+    int rc = check_prec(prec);
+    if (rc) return rc;
     if (!graph_exec_ || graph_prec_ != prec) {
         // every allocation happens before the capture
         if (prec == LSP_PREC_FP32) rc = ensure_factor_set(f32_);
@@ -1578,8 +1601,7 @@ int Batch::graph_step(int prec, lsp_refactor_report *rrep, lsp_solve_report *sre
         std::vector<int> cands;
         for (int m = 0; m < nc; m++) {
             const int *w = &ms[kMstat * m];
-            if (w[0] > 0 || w[2] != 0x7fffffff || (opt_.repivot == 2 && (w[4] > 0 || w[5] > 0)))
-                cands.push_back(m);
+            if (repivot_wanted(w)) cands.push_back(m);
         }
         int changed = 0;
         rc = repivot(cands, &changed);

@@ -133,6 +133,8 @@ public:
     size_t device_bytes() const;
     int set_stream(hipStream_t s);
     int set_values(const double *vals, bool on_device);
+    int check_prec(int prec);
+    bool repivot_wanted(const int *mstat_words) const;
     int refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *subset = nullptr);
     int launch_refactor_prec(int prec, const int *d_active);
     int record_step(int prec);

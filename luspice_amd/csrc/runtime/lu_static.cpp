@@ -91,7 +91,7 @@ int csc_slot(const StaticState &S, int r, int c)
 }
 
 // The first numeric call: host symbolic on these values (splu's pivots) and
-// upload.  Later calls refactor; repivot mode 2 re-derives the pivots on the
+// upload.  Later calls refactor; repivot mode 3 re-derives the pivots on the
 // host whenever splu would pick another one (lsp_options::repivot).
 int build_batch(StaticState &S, int n)
 {
@@ -104,7 +104,7 @@ int build_batch(StaticState &S, int n)
     cin.q = S.q.data();
     lsp_options opt;
     lsp_options_default(&opt);
-    opt.repivot = 2;
+    opt.repivot = 3;
     S.batch.reset(new lsp_batch());
     int rc = S.batch->impl.create(&cin, 1, &opt);
     if (rc) S.batch.reset();
@@ -146,7 +146,7 @@ int load_prec(StaticState &S, int prec)
     B.set_repivot(0);
     lsp_refactor_report rep{};
     const int rc = B.refactor(prec, &rep);
-    B.set_repivot(2);
+    B.set_repivot(3);
     return rc;
 }
 

@@ -49,11 +49,12 @@ def step_inputs(circuits, step: int, seed: int = 3, sigma: float = 0.1):
 class NewtonBatched:
     """All circuits in one batch; step inputs resident in HBM."""
 
-    def __init__(self, circuits, steps: int, seed: int = 3, sigma: float = 0.1, **opts):
+    def __init__(self, circuits, steps: int, seed: int = 3, sigma: float = 0.1, host_inputs=None, **opts):
         self.circuits = circuits
         self.B = Batch(circuits, **opts)
         self.steps = steps
-        self.host = [step_inputs(circuits, s, seed, sigma) for s in range(steps)]
+        self.host = host_inputs if host_inputs is not None else [
+            step_inputs(circuits, s, seed, sigma) for s in range(steps)]
         self.dev_vals = [DeviceArray(v) for v, _ in self.host]
         self.dev_rhs = [DeviceArray(b) for _, b in self.host]
 
@@ -71,6 +72,12 @@ class NewtonBatched:
 
     def solution(self) -> np.ndarray:
         return self.B.get_x()
+
+    def close(self):
+        for a in self.dev_vals + self.dev_rhs:
+            a.close()
+        self.dev_vals, self.dev_rhs = [], []
+        self.B.close()
 
 
 class NewtonPlain:
@@ -95,3 +102,8 @@ class NewtonPlain:
             xs.append(x)
             reps.append((rr, sr))
         return np.concatenate(xs), reps
+
+    def close(self):
+        for B in self.Bs:
+            B.close()
+        self.Bs = []

@@ -14,7 +14,7 @@
   factor_cache read as splu's lu_factors, levels of its U, trisolve with a
   residual at fp64 level, then dsh_gmres_ir; then Newton steps whose P must
   equal splu's P for each step's values.
-* Batches with repivot = 2 reproduce splu's P at every Newton step; with the
+* Batches with repivot = 3 reproduce splu's P at every Newton step; with the
   default repivot = 1 a step whose fixed pivot is zero gets new pivots and
   solves in the same call.
 """
@@ -308,15 +308,15 @@ def test_lu_static_newton_steps_follow_splu_pivots():
 
 
 def test_batch_repivot_exact_follows_splu_each_step():
-    """A 3-circuit batch with repivot = 2 and fp64 factors: after every Newton
+    """A 3-circuit batch with repivot = 3 and fp64 factors: after every Newton
     step each circuit's P equals the oracle's splu P for that step's values,
     and the solutions match the oracle."""
     cs = [L.mna("sized", n=n, seed=60 + i) for i, n in enumerate((300, 900, 150))]
     qs = [L.colamd(c) for c in cs]
-    B = L.Batch([L.Circuit(c.n, c.row, c.col, c.val, c.rhs, q=q) for c, q in zip(cs, qs)], repivot=2)
+    B = L.Batch([L.Circuit(c.n, c.row, c.col, c.val, c.rhs, q=q) for c, q in zip(cs, qs)], repivot=3)
     total = 0
     for step in range(1, 4):
-        print("case: batch repivot=2 step", step, flush=True)
+        print("case: batch repivot=3 step", step, flush=True)
         vb = [L.perturb(11, step, c.val, c.n) for c in cs]
         B.set_values(np.concatenate([v for v, _ in vb]))
         rep = B.refactor("fp64")
@@ -355,3 +355,31 @@ def test_zero_pivot_gets_new_pivots_in_the_same_call():
     A = np.zeros((3, 3))
     A[row, col] = v1
     assert sr["rc"] == 0 and O.rel_l2(x, np.linalg.solve(A, b)) <= 1e-12
+
+
+def test_small_pivot_repivots_only_from_mode_2():
+    """A step whose fixed pivot is tiny but nonzero (|U(1,1)| = 1e-5 against
+    |L(2,1)| = 1, failing refactor_ptol = 1e-3): the default repivot = 1 keeps
+    the pivots and reports the bad pivot (GMRES-IR still solves it); repivot =
+    2 takes the pivots splu picks for the step within the same call."""
+    print("case: small pivot, repivot=1 and 2", flush=True)
+    row = np.array([0, 0, 1, 1, 1, 2, 2], np.int32)
+    col = np.array([0, 1, 0, 1, 2, 1, 2], np.int32)
+    v0 = np.array([4.0, 1.0, 1.0, 4.0, 1.0, 1.0, 4.0])
+    b = np.array([1.0, 2.0, 3.0])
+    q = np.arange(3, dtype=np.int32)
+    v1 = np.array([2.0, 1.0, 1.0, 0.5 + 1e-5, 1.0, 1.0, 4.0])
+    A = np.zeros((3, 3))
+    A[row, col] = v1
+    for mode, repiv in ((1, 0), (2, 1)):
+        B = L.Batch(L.Circuit(3, row, col, v0, b, q=q), tail_max=-1, repivot=mode)
+        B.set_values(v1)
+        rep = B.refactor("fp64")
+        print(mode, rep, flush=True)
+        assert rep["rc"] == 0 and rep["repivoted"] == repiv, rep
+        assert rep["bad_pivots"] == (1 if mode == 1 else 0), rep
+        if mode == 2:
+            assert np.array_equal(B.export_factors(0)["pinv"], O.lu_factor(3, row, col, v1, q)["pinv"])
+        x, sr = B.solve(b)
+        assert sr["rc"] == 0 and O.rel_l2(x, np.linalg.solve(A, b)) <= 1e-10, sr
+        B.close()

@@ -175,3 +175,42 @@ def test_zero_pivot_reported_then_recovered():
     A = np.zeros((3, 3))
     A[row, col] = v0
     assert sr["rc"] == 0 and O.rel_l2(x, np.linalg.solve(A, b)) <= 1e-12
+
+
+def test_ir_stall_falls_back_to_fp64(c0, c0_q):
+    """IR with fp32 factors capped at one restart of one inner iteration
+    cannot reach the 1e-10 residual on c0: without the fallback the solve
+    reports not converged; with it the stalled circuit is refactored in fp64,
+    solved again, and lands within 1e-6 of the oracle (one fallback).  A second
+    circuit in the same batch with the IR capped the same way shows the
+    fallback acts per circuit."""
+    print("case: c0 IR stall -> fp64 fallback", flush=True)
+    xo = _oracle_x(c0, c0_q)
+    circ = L.Circuit(c0.n, c0.row, c0.col, c0.val, c0.rhs, q=c0_q)
+    B0 = L.Batch(circ, fallback_fp64=0, ir_outer=1, ir_inner=1)
+    assert B0.refactor("fp32")["rc"] == 0
+    x0, s0 = B0.solve(c0.rhs)
+    print("no fallback", s0, O.rel_l2(x0, xo), flush=True)
+    assert s0["rc"] != 0 and s0["converged"] == 0 and s0["fallback"] == 0
+    B0.close()
+    c1 = L.mna("sized", n=700, seed=41)
+    q1 = L.colamd(c1)
+    B = L.Batch([circ, L.Circuit(c1.n, c1.row, c1.col, c1.val, c1.rhs, q=q1)], fallback_fp64=1,
+                ir_outer=1, ir_inner=1)
+    assert B.refactor("fp32")["rc"] == 0
+    x, sr = B.solve(np.concatenate([c0.rhs, c1.rhs]))
+    print("fallback", sr, flush=True)
+    assert sr["rc"] == 0 and sr["fallback"] == 2 and sr["converged"] == 2, sr
+    assert O.rel_l2(x[:c0.n], xo) <= 1e-6
+    assert O.rel_l2(x[c0.n:], _oracle_x(c1, q1)) <= 1e-6
+    B.close()
+
+
+def test_fp16_refused_without_opt_in(c0, c0_q):
+    """FP16 factor storage is off unless lsp_options.allow_fp16 is set: the
+    refactor returns LSP_E_ARG before any device work."""
+    B = L.Batch(L.Circuit(c0.n, c0.row, c0.col, c0.val, c0.rhs, q=c0_q))
+    rep = B.refactor("fp16")
+    assert rep["rc"] == -1, rep                 # LSP_E_ARG
+    assert B.refactor("fp32")["rc"] == 0
+    B.close()

@@ -65,7 +65,7 @@ def test_fp16_ruiz_scaling_one_solve():
     q = L.colamd(c)
     rc, xr, _, _ = O.cpu_lu_ref(c.n, c.row, c.col, c.val, q, c.rhs)
     assert rc == -1
-    B = L.Batch(L.Circuit(c.n, c.row, c.col, c.val, c.rhs, q=q), fallback_fp64=0, pivot_scaled=1)
+    B = L.Batch(L.Circuit(c.n, c.row, c.col, c.val, c.rhs, q=q), fallback_fp64=0, pivot_scaled=1, allow_fp16=1)
     rr = B.refactor("fp16")
     assert rr["rc"] == 0 and rr["first_bad_circuit"] == -1, rr
     xt = B.trisolve(c.rhs)                       # R and C applied around the FP16 solve
@@ -86,7 +86,7 @@ def test_fp16_ruiz_gmres_ir():
     rc, xr, _, _ = O.cpu_lu_ref(c.n, c.row, c.col, c.val, q, c.rhs)
     assert rc == -1
     circ = L.Circuit(c.n, c.row, c.col, c.val, c.rhs, q=q)
-    B = L.Batch(circ, fallback_fp64=0, pivot_scaled=1)
+    B = L.Batch(circ, fallback_fp64=0, pivot_scaled=1, allow_fp16=1)
     rr = B.refactor("fp16")
     assert rr["rc"] == 0, rr
     xt = B.trisolve(c.rhs)                       # one preconditioner application: finite
@@ -95,7 +95,7 @@ def test_fp16_ruiz_gmres_ir():
     if sr["converged"]:
         assert _rel(x, xr) <= 1e-6, sr
     B.close()
-    B2 = L.Batch(circ, pivot_scaled=1)           # fp64 fallback enabled (default)
+    B2 = L.Batch(circ, pivot_scaled=1, allow_fp16=1)           # fp64 fallback enabled (default)
     B2.refactor("fp16")
     x2, sr2 = B2.solve(c.rhs)
     assert _rel(x2, xr) <= 1e-6, sr2
