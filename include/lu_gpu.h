@@ -82,6 +82,26 @@ int trisolve_f32_launch(int n, const int *Lp, const int *Li, const float *Lx, co
                         const double *b, double *x, int *work_i, double *work_d, int parallel,
                         void *stream);
 
+/* trisolve_f32_launch_noalloc -- no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the same solve as
+ * trisolve_f32_launch (same kernels, same result bit for bit) with work_i /
+ * work_d taken from a static per-GPU device array of 3 * LU_NOALLOC_MAX_N
+ * doubles (6 MiB, held by each GPU the library runs on until the process
+ * ends) instead of arguments.  Allocates nothing.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (two streams, or
+ * two host threads) race and must be serialised by the caller.  The kernels
+ * clear the workspace themselves, so the array's previous contents do not
+ * matter.
+ * Returns 0 when enqueued, 1 on a launch error, 2 when n < 0 or
+ * n > LU_NOALLOC_MAX_N (nothing launched), 3 when the array's device address
+ * could not be obtained (nothing launched). */
+#define LU_NOALLOC_MAX_N 262144
+int trisolve_f32_launch_noalloc(int n, const int *Lp, const int *Li, const float *Lx,
+                                const int *Up, const int *Ui, const float *Ux, const int *pinv,
+                                const int *q, const double *b, double *x, int parallel,
+                                void *stream);
+
 #ifdef __cplusplus
 }
 #endif

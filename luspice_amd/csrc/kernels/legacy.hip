@@ -14,6 +14,8 @@
 // keeps P, nnz(L), nnz(U) equal to cpu_lu_ref's.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "lu_gpu.h"
 #include "lu_gpu_par.h"
 
@@ -397,6 +399,58 @@ int trisolve_f32_launch(int n, const int *Lp, const int *Li, const float *Lx, co
         hipLaunchKernelGGL(trisolve_f32_kernel, dim3(1), dim3(1), 0, (hipStream_t)stream, n, Lp,
                            Li, Lx, Up, Ui, Ux, pinv, q, b, x, work_i, work_d);
     return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Workspace of trisolve_f32_launch_noalloc: 2n doubles (work_d) then n ints
+// (work_i), n <= LU_NOALLOC_MAX_N.  Each GPU holds its own copy for the life
+// of the process.  Host code never passes the array's name to a kernel (on
+// the host it is only a stand-in): it passes the device address that
+// hipGetSymbolAddress returns for the current device, looked up once per
+// device and kept.
+static __device__ __align__(256) double scratch[3 * (size_t)LU_NOALLOC_MAX_N];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's scratch array, or nullptr on failure.
+double *scratch_on(int dev)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        double *p = nullptr;
+        if (hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+int trisolve_f32_launch_noalloc(int n, const int *Lp, const int *Li, const float *Lx, const int *Up,
+                                const int *Ui, const float *Ux, const int *pinv, const int *q,
+                                const double *b, double *x, int parallel, void *stream)
+{
+    if (n < 0 || n > LU_NOALLOC_MAX_N) return 2;
+    // the scratch copy of the device the launch runs on (the stream's device)
+    int cur = 0, dev = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return 3;
+    dev = cur;
+    if (stream && hipStreamGetDevice((hipStream_t)stream, &dev) != hipSuccess) return 3;
+    if (dev < 0 || dev >= kMaxDevices) return 3;
+    if (dev != cur && hipSetDevice(dev) != hipSuccess) return 3;
+    double *ptr = scratch_on(dev);
+    if (dev != cur) (void)hipSetDevice(cur);
+    if (!ptr) return 3;
+    double *work_d = ptr;                          // pb[n], y[n]
+    int *work_i = (int *)(ptr + 2 * (size_t)n);    // rowat[n]
+    return trisolve_f32_launch(n, Lp, Li, Lx, Up, Ui, Ux, pinv, q, b, x, work_i, work_d, parallel,
+                               stream);
 }
 
 int lu_factor_par_launch(int n, const int *Ap, const int *Ai, const double *Ax,
