@@ -1,122 +1,347 @@
-// mp_lu_bench -- mixed-precision sparse LU stage benchmark with the reference
-// CLI contract (src/mp/mp_lu_bench.cu:239-495):
+// mp_lu_bench -- stage benchmark of the sparse LU path, with the reference
+// CLI's contract (src/mp/mp_lu_bench.cu:239-495): its argv, its stdout lines
+// and its JSON keys.
 //
 //   mp_lu_bench <s0.mtx> <s0.rhs> [pin] [flt] [max_n_st] [reps]
-//   env DSH_ST_REPS (default 2) repeats of the single-thread kernels
+//   env DSH_ST_REPS (default 2): timed repeats of the single-thread factor
 //
-// Same pipeline and JSON keys as the reference: s0 read (zero RHS -> 1.0),
-// .flt diagonal pin, COLAMD (all knobs zero) -> q, q-order CSC, cpu_lu_ref
-// (fp64 oracle x and lnnz/unnz), capacities Lcap = Ucap = max(2 nnz + 64,
-// 1.5 lnnz_cpu + 64), then the reference's kernels launched with their own
-// shapes through lu_gpu.h / lu_gpu_par.h (<<<1,1>>> single-thread fp64 and
-// fp32, <<<1,256,12n+256>>> single-workgroup fp64, <<<1,256>>> trisolves),
-// timed as warm-up + min over reps with hipEvents, each checked by rel L2
-// against the CPU x and the residual ||Ax-b||/||b||.
+// Preparation: the s0 files (a zero or missing RHS entry reads as 1.0), the
+// .flt diagonal pin, COLAMD (all knobs zero) -> q, the q-order CSC with rows
+// in file order, and the CPU reference cpu_lu_ref (fp64 x, lnnz / unnz).
+// Factor capacities are max(2 nnz + 64, 1.5 x the CPU count + 64).
 //
-// Added after the reference's stages (extra JSON keys): the device-resident
-// path of luspice.h -- persistent level-ordered refactor with fp32 and fp64
-// factor storage, the level-scheduled trisolve, and the fp64 GMRES-IR solve.
-// The single-workgroup kernel is skipped (n/a, -1) when 12n + 256 bytes of
-// LDS exceed the device limit; the reference fails there.
+// Stages: a table.  Each entry names its stdout tag, the factor set it
+// writes or reads (fp64 or fp32 storage), the shape of its factor kernel
+// (none, <<<1,1>>> or one 256-thread workgroup) and of its solve kernel.  One
+// runner gates a stage (single-thread kernels only up to max_n_st, the
+// workgroup factor only when its 12n + 256 bytes of LDS fit), launches it
+// through the lu_gpu.h / lu_gpu_par.h launchers, times it with HIP events
+// (one warm-up call, then the fastest of the repeats) and checks the solve
+// against the CPU x (relative L2) and the residual ||A x - b|| / ||b||.
+//
+// After the table, the device-resident path of luspice.h (extra JSON keys):
+// persistent refactor with fp64 and fp32 factor storage, the level-ordered
+// trisolve and the fp64 GMRES-IR solve.  Where the reference stops with an
+// error (the workgroup kernel's LDS need above the device limit), this tool
+// reports the stage as n/a (-1) and goes on.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
-#include <cstring>
-#include <utility>
+#include <string>
 #include <vector>
 
 #include "cpu_ref_lu.h"
+#include "csc.h"
 #include "lu_gpu.h"
 #include "lu_gpu_par.h"
 #include "luspice.h"
 #include "s0io.h"
 
-#define CK(x)                                                                                \
-    do {                                                                                     \
-        hipError_t e_ = (x);                                                                 \
-        if (e_ != hipSuccess) {                                                              \
-            fprintf(stderr, "HIP FAIL %s:%d %s: %s\n", __FILE__, __LINE__, #x,               \
-                    hipGetErrorString(e_));                                                  \
-            exit(1);                                                                         \
-        }                                                                                    \
-    } while (0)
+namespace {
 
-static void build_csc_q(const lsp_s0 &M, const int *q, std::vector<int> &Ap, std::vector<int> &Ai,
-                        std::vector<double> &Ax)
+[[noreturn]] void die(const char *what)
 {
-    const int n = M.n;
-    std::vector<int> invq(n);
-    for (int j = 0; j < n; j++) invq[q ? q[j] : j] = j;
-    Ap.assign(n + 1, 0);
-    for (long i = 0; i < M.nnz; i++) Ap[invq[M.col[i] - 1] + 1]++;
-    for (int i = 0; i < n; i++) Ap[i + 1] += Ap[i];
-    Ai.assign(M.nnz, 0);
-    Ax.assign(M.nnz, 0.0);
-    std::vector<int> cur(Ap.begin(), Ap.end() - 1);
-    for (long i = 0; i < M.nnz; i++) {
-        const int p = cur[invq[M.col[i] - 1]]++;
-        Ai[p] = M.row[i] - 1;
-        Ax[p] = M.val[i];
+    fprintf(stderr, "mp_lu_bench: %s\n", what);
+    exit(1);
+}
+
+void hip_ok(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) {
+        fprintf(stderr, "mp_lu_bench: %s: %s\n", what, hipGetErrorString(e));
+        exit(1);
     }
 }
 
-static double residual(const lsp_s0 &M, const double *x)
-{
-    std::vector<double> bh(M.n, 0.0);
-    for (long i = 0; i < M.nnz; i++) bh[M.row[i] - 1] += M.val[i] * x[M.col[i] - 1];
-    double num = 0, den = 0;
-    for (int i = 0; i < M.n; i++) {
-        const double d = bh[i] - M.b[i + 1];
-        num += d * d;
-        den += M.b[i + 1] * M.b[i + 1];
+// ---- the problem and its checks -------------------------------------------
+struct Problem {
+    lsp_s0 M{};
+    int n = 0;
+    std::vector<int> row, col, q;          // 0-based COO, column order
+    std::vector<int> Ap, Ai;               // q-order CSC (rows: original ids, file order)
+    std::vector<double> Ax;
+    std::vector<float> Ax32;
+    std::vector<double> b, x_cpu;          // b[0..n)
+    long lnnz_cpu = 0, unnz_cpu = 0;
+
+    // ||x - x_cpu|| / ||x_cpu||
+    double err(const std::vector<double> &x) const
+    {
+        double d2 = 0, r2 = 0;
+        for (int i = 0; i < n; i++) {
+            d2 += (x[i] - x_cpu[i]) * (x[i] - x_cpu[i]);
+            r2 += x_cpu[i] * x_cpu[i];
+        }
+        return std::sqrt(d2 / r2);
     }
-    return std::sqrt(num / den);
+    // ||A x - b|| / ||b||
+    double res(const std::vector<double> &x) const
+    {
+        std::vector<double> r(b);
+        for (size_t e = 0; e < row.size(); e++) r[row[e]] -= M.val[e] * x[col[e]];
+        double a = 0, c = 0;
+        for (int i = 0; i < n; i++) {
+            a += r[i] * r[i];
+            c += b[i] * b[i];
+        }
+        return std::sqrt(a / c);
+    }
+};
+
+// Device arrays, freed together at exit.
+struct Arena {
+    std::vector<void *> blocks;
+    template <typename T> T *get(size_t count, const void *init = nullptr)
+    {
+        void *p = nullptr;
+        hip_ok(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), "device allocation");
+        if (init) hip_ok(hipMemcpy(p, init, count * sizeof(T), hipMemcpyHostToDevice), "upload");
+        blocks.push_back(p);
+        return (T *)p;
+    }
+    ~Arena()
+    {
+        for (void *p : blocks) (void)hipFree(p);
+    }
+};
+
+// The fastest of `reps` timed calls after one warm-up call.
+struct Clock {
+    hipEvent_t a{}, b{};
+    Clock()
+    {
+        hip_ok(hipEventCreate(&a), "event");
+        hip_ok(hipEventCreate(&b), "event");
+    }
+    ~Clock()
+    {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    template <typename F> double fastest(F &&call, int reps)
+    {
+        call();
+        hip_ok(hipDeviceSynchronize(), "warm-up");
+        double best = INFINITY;
+        for (int r = 0; r < reps; r++) {
+            hip_ok(hipEventRecord(a, nullptr), "event");
+            call();
+            hip_ok(hipEventRecord(b, nullptr), "event");
+            hip_ok(hipEventSynchronize(b), "event sync");
+            float ms = 0;
+            hip_ok(hipEventElapsedTime(&ms, a, b), "event time");
+            best = std::min(best, (double)ms);
+        }
+        return best;
+    }
+};
+
+// ---- stage table ------------------------------------------------------------
+enum Store { F64 = 0, F32 = 1 };
+enum Shape { NONE, SINGLE_THREAD, WORKGROUP };
+
+struct FactorSet {
+    int *Lp, *Li, *Up, *Ui, *pinv;
+    void *Lx, *Ux, *work;                  // values and the factor's T work vector
+    long lnnz = -1, unnz = -1;
+    bool ready = false;
+};
+
+struct Stage {
+    const char *tag;
+    Store store;
+    Shape factor, solve;
+    double t_factor = -1, t_solve = -1;
+};
+
+struct Run {
+    const Problem &P;
+    Arena &dev;
+    long Lcap, Ucap;
+    int max_n_st, reps, st_reps;
+    FactorSet set[2];
+    int *Ap, *Ai, *diag, *q, *xi, *xi_wg, *wi, *lnnz, *unnz, *sing;
+    double *Ax, *b, *xo, *wd;
+    float *Ax32;
+
+    void factor(Stage &s)
+    {
+        FactorSet &F = set[s.store];
+        const int n = P.n;
+        auto launch = [&]() {
+            int rc;
+            if (s.factor == WORKGROUP)
+                rc = lu_factor_par_launch(n, Ap, Ai, Ax, diag, F.Lp, F.Li, (double *)F.Lx, Lcap, F.Up, F.Ui,
+                                          (double *)F.Ux, Ucap, F.pinv, xi_wg, lnnz, unnz, sing, nullptr);
+            else if (s.store == F64)
+                rc = lu_factor_launch(n, Ap, Ai, Ax, diag, F.Lp, F.Li, (double *)F.Lx, Lcap, F.Up, F.Ui,
+                                      (double *)F.Ux, Ucap, F.pinv, xi, (double *)F.work, lnnz, unnz, sing,
+                                      nullptr);
+            else
+                rc = lu_factor_f32_launch(n, Ap, Ai, Ax32, diag, F.Lp, F.Li, (float *)F.Lx, Lcap, F.Up, F.Ui,
+                                          (float *)F.Ux, Ucap, F.pinv, xi, (float *)F.work, lnnz, unnz, sing,
+                                          nullptr);
+            if (rc) die("factor launch failed");
+        };
+        launch();
+        int st[3];
+        hip_ok(hipMemcpy(&st[0], sing, 4, hipMemcpyDeviceToHost), "status");
+        hip_ok(hipMemcpy(&st[1], lnnz, 4, hipMemcpyDeviceToHost), "status");
+        hip_ok(hipMemcpy(&st[2], unnz, 4, hipMemcpyDeviceToHost), "status");
+        if (st[0] != -1) {
+            fprintf(stderr, "[%s factor] %s %d\n", s.tag,
+                    st[0] == -2 ? "capacity overflow" : st[0] == -3 ? "structure overflow" : "singular at column",
+                    st[0]);
+            exit(1);
+        }
+        F.lnnz = st[1];
+        F.unnz = st[2];
+        F.ready = true;
+        Clock clk;
+        s.t_factor = clk.fastest(launch, s.factor == SINGLE_THREAD ? st_reps : reps);
+    }
+
+    std::vector<double> solve(Stage &s)
+    {
+        const FactorSet &F = set[s.store];
+        const int n = P.n;
+        const bool wg = s.solve == WORKGROUP;
+        auto launch = [&]() {
+            const int rc =
+                s.store == F64
+                    ? (wg ? trisolve_par_launch : trisolve_launch)(n, F.Lp, F.Li, (double *)F.Lx, F.Up, F.Ui,
+                                                                   (double *)F.Ux, F.pinv, q, b, xo, wi, wd,
+                                                                   nullptr)
+                    : trisolve_f32_launch(n, F.Lp, F.Li, (float *)F.Lx, F.Up, F.Ui, (float *)F.Ux, F.pinv, q,
+                                          b, xo, wi, wd, wg ? 1 : 0, nullptr);
+            if (rc) die("trisolve launch failed");
+        };
+        Clock clk;
+        s.t_solve = clk.fastest(launch, reps);
+        std::vector<double> x(n);
+        hip_ok(hipMemcpy(x.data(), xo, (size_t)n * 8, hipMemcpyDeviceToHost), "download x");
+        return x;
+    }
+
+    // Why the stage cannot run here, or nullptr.
+    const char *gate(const Stage &s, std::string &why) const
+    {
+        char buf[160];
+        if (s.factor == SINGLE_THREAD && P.n > max_n_st) return "";
+        if (s.factor == WORKGROUP) {
+            int dev = 0, lim = 0;
+            hip_ok(hipGetDevice(&dev), "device");
+            hip_ok(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "attribute");
+            const size_t need = LU_PAR_SHARED_BYTES(P.n);
+            if (need > (size_t)lim && need > 160u * 1024u) {
+                snprintf(buf, sizeof buf, "n/a (needs %zu B of LDS, device limit %d B)", need, lim);
+                why = buf;
+                return why.c_str();
+            }
+        }
+        if (s.factor == NONE && !set[s.store].ready) {
+            snprintf(buf, sizeof buf, "n/a (fp32 ST factor skipped at n>%d)", max_n_st);
+            why = buf;
+            return why.c_str();
+        }
+        return nullptr;
+    }
+
+    void run(Stage &s)
+    {
+        std::string why;
+        if (const char *skip = gate(s, why)) {
+            if (*skip) printf("%s: %s\n", s.tag, skip);
+            return;
+        }
+        if (s.factor != NONE) factor(s);
+        const std::vector<double> x = solve(s);
+        if (s.factor != NONE)
+            printf("%s: factor=%.3f ms trisolve=%.3f ms  L2(vs CPU)=%.3e res=%.3e\n", s.tag, s.t_factor,
+                   s.t_solve, P.err(x), P.res(x));
+        else
+            printf("%s: %.3f ms  L2(vs CPU)=%.3e res=%.3e\n", s.tag, s.t_solve, P.err(x), P.res(x));
+    }
+};
+
+// ---- the device-resident path (luspice.h) -----------------------------------
+struct DevicePath {
+    double rf64 = -1, rf32 = -1, tri32 = -1, ir_ms = -1, err = -1, res = -1;
+    int inner = -1;
+};
+
+DevicePath device_path(const Problem &P, int reps)
+{
+    DevicePath d;
+    std::vector<double> vals(P.M.val, P.M.val + P.M.nnz);
+    lsp_circuit_in cin{P.n, (long)P.row.size(), P.row.data(), P.col.data(), vals.data(), P.q.data()};
+    lsp_batch *B = nullptr;
+    if (lsp_batch_create(&cin, 1, nullptr, &B) != LSP_OK) {
+        printf("device path: batch creation failed\n");
+        return d;
+    }
+    auto best_refactor = [&](int prec) {
+        double best = INFINITY;
+        lsp_refactor_report rr{};
+        for (int r = 0; r <= reps; r++)
+            if (lsp_batch_refactor(B, prec, &rr) == LSP_OK && r > 0) best = std::min(best, rr.ms);
+        return best;
+    };
+    d.rf64 = best_refactor(LSP_PREC_FP64);
+    d.rf32 = best_refactor(LSP_PREC_FP32);
+    std::vector<double> x(P.n);
+    d.tri32 = INFINITY;
+    for (int r = 0; r <= reps; r++) {
+        double rf = 0, fw = 0, bw = 0;
+        if (lsp_batch_trisolve(B, P.b.data(), x.data(), 0) == LSP_OK && r > 0) {
+            lsp_batch_last_times(B, &rf, &fw, &bw);
+            d.tri32 = std::min(d.tri32, fw + bw);
+        }
+    }
+    Clock clk;
+    lsp_solve_report sr{};
+    hip_ok(hipEventRecord(clk.a, nullptr), "event");
+    lsp_batch_solve(B, P.b.data(), x.data(), 0, &sr);
+    hip_ok(hipEventRecord(clk.b, nullptr), "event");
+    hip_ok(hipEventSynchronize(clk.b), "event sync");
+    float ms = 0;
+    hip_ok(hipEventElapsedTime(&ms, clk.a, clk.b), "event time");
+    d.ir_ms = ms;
+    d.inner = sr.inner_total;
+    d.err = P.err(x);
+    d.res = P.res(x);
+    printf("device path: refactor fp64=%.3f ms fp32=%.3f ms, trisolve fp32=%.3f ms, "
+           "GMRES-IR %.3f ms (%d inner) L2(vs CPU)=%.3e res=%.3e\n",
+           d.rf64, d.rf32, d.tri32, d.ir_ms, d.inner, d.err, d.res);
+    lsp_batch_destroy(B);
+    return d;
 }
 
-static double rel_l2(int n, const double *a, const double *b)
-{
-    double num = 0, den = 0;
-    for (int i = 0; i < n; i++) {
-        const double d = a[i] - b[i];
-        num += d * d;
-        den += b[i] * b[i];
+// ---- JSON summary -----------------------------------------------------------
+struct Json {
+    std::string body;
+    void add(const char *key, const char *fmt, ...) __attribute__((format(printf, 3, 4)))
+    {
+        char val[256];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(val, sizeof val, fmt, ap);
+        va_end(ap);
+        body += body.empty() ? "{" : ",";
+        body += "\"";
+        body += key;
+        body += "\":";
+        body += val;
     }
-    return std::sqrt(num / den);
-}
+};
 
-template <typename F>
-static double time_best(F &&f, int reps)
-{
-    hipEvent_t e0, e1;
-    CK(hipEventCreate(&e0));
-    CK(hipEventCreate(&e1));
-    f();
-    CK(hipDeviceSynchronize());
-    double best = 1e300;
-    for (int r = 0; r < reps; r++) {
-        CK(hipEventRecord(e0, nullptr));
-        f();
-        CK(hipEventRecord(e1, nullptr));
-        CK(hipDeviceSynchronize());
-        float ms = 0;
-        CK(hipEventElapsedTime(&ms, e0, e1));
-        best = std::min(best, (double)ms);
-    }
-    CK(hipEventDestroy(e0));
-    CK(hipEventDestroy(e1));
-    return best;
-}
+double ratio(double a, double b) { return a > 0 && b > 0 ? a / b : -1; }
 
-template <typename T>
-static T *dmalloc(size_t count)
-{
-    T *p = nullptr;
-    CK(hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T)));
-    return p;
-}
+}  // namespace
 
 int main(int argc, char **argv)
 {
@@ -124,260 +349,138 @@ int main(int argc, char **argv)
         fprintf(stderr, "usage: %s <s0.mtx> <s0.rhs> [pin] [flt] [max_n_st] [reps]\n", argv[0]);
         return 2;
     }
-    lsp_s0 M;
-    if (lsp_s0_read(argv[1], argv[2], &M)) return 1;
-    const double pin_val = argc >= 4 ? atof(argv[3]) : 0.0;
-    const char *flt = argc >= 5 ? argv[4] : nullptr;
-    const int max_n_st = argc >= 6 ? atoi(argv[5]) : 15000;
-    int reps = argc >= 7 ? atoi(argv[6]) : 0;
-    if (reps <= 0) reps = 5;
-    int st_reps = 2;
-    if (getenv("DSH_ST_REPS")) st_reps = std::max(1, atoi(getenv("DSH_ST_REPS")));
-    printf("== mp_lu_bench %s ==\nn=%d nnz=%ld pin=%.1e flt=%s max_n_st=%d reps=%d\n", argv[1],
-           M.n, M.nnz, pin_val, flt ? flt : "-", max_n_st, reps);
-    lsp_flt_pin(&M, flt, pin_val);
-    const int n = M.n;
-    // every entry must index the n x n matrix before it reaches a kernel
-    for (long i = 0; i < M.nnz; i++)
-        if (M.row[i] < 1 || M.row[i] > n || M.col[i] < 1 || M.col[i] > n) {
-            fprintf(stderr, "entry %ld (%d, %d) outside 1..%d\n", i + 1, M.row[i], M.col[i], n);
+    Problem P;
+    if (lsp_s0_read(argv[1], argv[2], &P.M)) return 1;
+    const double pin = argc > 3 ? atof(argv[3]) : 0.0;
+    const char *flt = argc > 4 ? argv[4] : nullptr;
+    const int max_n_st = argc > 5 ? atoi(argv[5]) : 15000;
+    const int reps = argc > 6 && atoi(argv[6]) > 0 ? atoi(argv[6]) : 5;
+    const char *env = getenv("DSH_ST_REPS");
+    const int st_reps = env ? std::max(1, atoi(env)) : 2;
+    printf("== mp_lu_bench %s ==\nn=%d nnz=%ld pin=%.1e flt=%s max_n_st=%d reps=%d\n", argv[1], P.M.n,
+           P.M.nnz, pin, flt ? flt : "-", max_n_st, reps);
+    lsp_flt_pin(&P.M, flt, pin);
+
+    // 0-based COO; every entry must index the n x n matrix before a kernel sees it
+    P.n = P.M.n;
+    const long nnz = P.M.nnz;
+    P.row.resize(nnz);
+    P.col.resize(nnz);
+    for (long e = 0; e < nnz; e++) {
+        P.row[e] = P.M.row[e] - 1;
+        P.col[e] = P.M.col[e] - 1;
+        if (P.row[e] < 0 || P.row[e] >= P.n || P.col[e] < 0 || P.col[e] >= P.n) {
+            fprintf(stderr, "mp_lu_bench: entry %ld (%d, %d) lies outside 1..%d\n", e + 1, P.M.row[e],
+                    P.M.col[e], P.n);
             return 1;
         }
-
-    // COLAMD on the 0-based pattern
-    std::vector<int> r0(M.nnz), c0(M.nnz), q(n);
-    for (long i = 0; i < M.nnz; i++) { r0[i] = M.row[i] - 1; c0[i] = M.col[i] - 1; }
-    if (lsp_colamd(n, M.nnz, r0.data(), c0.data(), q.data()) != 0) {
-        fprintf(stderr, "colamd failed\n");
-        return 1;
     }
+    P.b.assign(P.M.b + 1, P.M.b + 1 + P.n);
+    P.q.resize(P.n);
+    if (lsp_colamd(P.n, nnz, P.row.data(), P.col.data(), P.q.data()) != 0) die("colamd failed");
     printf("colamd ok\n");
-    std::vector<int> Ap, Ai, Ap0, Ai0;
-    std::vector<double> Ax, Ax0;
-    build_csc_q(M, q.data(), Ap, Ai, Ax);
-    build_csc_q(M, nullptr, Ap0, Ai0, Ax0);
-
-    std::vector<double> x_cpu(n);
-    long lnnz_cpu = 0, unnz_cpu = 0;
-    int rc = cpu_lu_ref(n, Ap0.data(), Ai0.data(), Ax0.data(), q.data(), M.b + 1, 1.0, x_cpu.data(),
-                        &lnnz_cpu, &unnz_cpu);
-    if (rc >= 0) {
-        fprintf(stderr, "CPU reference singular at column %d\n", rc);
-        return 1;
-    }
-    printf("CPU ref: lnnz=%ld unnz=%ld (L+U=%ld)\n", lnnz_cpu, unnz_cpu, lnnz_cpu + unnz_cpu);
-    long Lcap = 2L * M.nnz + 64, Ucap = 2L * M.nnz + 64;
-    if (lnnz_cpu > Lcap) Lcap = (long)(lnnz_cpu * 1.5) + 64;
-    if (unnz_cpu > Ucap) Ucap = (long)(unnz_cpu * 1.5) + 64;
-
-    int *d_Ap = dmalloc<int>(n + 1), *d_Ai = dmalloc<int>(M.nnz), *d_diag = dmalloc<int>(n),
-        *d_q = dmalloc<int>(n);
-    double *d_Ax = dmalloc<double>(M.nnz);
-    float *d_Ax32 = dmalloc<float>(M.nnz);
-    int *d_Lp = dmalloc<int>(n + 1), *d_Li = dmalloc<int>(Lcap), *d_Up = dmalloc<int>(n + 1),
-        *d_Ui = dmalloc<int>(Ucap);
-    double *d_Lx = dmalloc<double>(Lcap), *d_Ux = dmalloc<double>(Ucap);
-    int *d_Lp32 = dmalloc<int>(n + 1), *d_Up32 = dmalloc<int>(n + 1), *d_Li32 = dmalloc<int>(Lcap),
-        *d_Ui32 = dmalloc<int>(Ucap);
-    float *d_Lx32 = dmalloc<float>(Lcap), *d_Ux32 = dmalloc<float>(Ucap);
-    int *d_pinv = dmalloc<int>(n), *d_pinv32 = dmalloc<int>(n), *d_xi = dmalloc<int>(2 * n + 1),
-        *d_xi32 = dmalloc<int>(2 * n + 1), *d_xip = dmalloc<int>(n + 4 * LU_PAR_CAP);
-    double *d_x = dmalloc<double>(n), *d_b = dmalloc<double>(n), *d_xo = dmalloc<double>(n),
-           *d_wd = dmalloc<double>(2 * n);
-    float *d_x32 = dmalloc<float>(n);
-    int *d_wi = dmalloc<int>(n), *d_lnnz = dmalloc<int>(1), *d_unnz = dmalloc<int>(1),
-        *d_sing = dmalloc<int>(1);
-    std::vector<float> Ax32(M.nnz);
-    for (long i = 0; i < M.nnz; i++) Ax32[i] = (float)Ax[i];
-    CK(hipMemcpy(d_Ap, Ap.data(), (n + 1) * 4, hipMemcpyHostToDevice));
-    CK(hipMemcpy(d_Ai, Ai.data(), M.nnz * 4, hipMemcpyHostToDevice));
-    CK(hipMemcpy(d_Ax, Ax.data(), M.nnz * 8, hipMemcpyHostToDevice));
-    CK(hipMemcpy(d_Ax32, Ax32.data(), M.nnz * 4, hipMemcpyHostToDevice));
-    CK(hipMemcpy(d_diag, q.data(), n * 4, hipMemcpyHostToDevice));
-    CK(hipMemcpy(d_q, q.data(), n * 4, hipMemcpyHostToDevice));
-    CK(hipMemcpy(d_b, M.b + 1, n * 8, hipMemcpyHostToDevice));
-
-    auto read_status = [&](const char *tag) {
-        int sing = 0, ln = 0, un = 0;
-        CK(hipMemcpy(&sing, d_sing, 4, hipMemcpyDeviceToHost));
-        CK(hipMemcpy(&ln, d_lnnz, 4, hipMemcpyDeviceToHost));
-        CK(hipMemcpy(&un, d_unnz, 4, hipMemcpyDeviceToHost));
-        if (sing == -2) { fprintf(stderr, "[%s] capacity overflow\n", tag); exit(1); }
-        if (sing == -3) { fprintf(stderr, "[%s] structure overflow\n", tag); exit(1); }
-        if (sing >= 0) { fprintf(stderr, "[%s] singular at column %d\n", tag, sing); exit(1); }
-        return std::pair<int, int>(ln, un);
-    };
-    auto download_xo = [&]() {
-        std::vector<double> h(n);
-        CK(hipMemcpy(h.data(), d_xo, n * 8, hipMemcpyDeviceToHost));
-        return h;
-    };
-    const bool do_st = n <= max_n_st;
-    double t_f_st64 = -1, t_f_st32 = -1, t_f_par64 = -1;
-    double t_t_st64 = -1, t_t_st32 = -1, t_t_par64 = -1, t_t_par32 = -1;
-    long lnnz64 = -1, unnz64 = -1, lnnz32 = -1, unnz32 = -1;
-
-    if (do_st) {  // 1) fp64 single thread
-        auto fac = [&]() {
-            lu_factor_kernel<<<1, 1>>>(n, d_Ap, d_Ai, d_Ax, d_diag, d_Lp, d_Li, d_Lx, Lcap, d_Up,
-                                       d_Ui, d_Ux, Ucap, d_pinv, d_xi, d_x, d_lnnz, d_unnz, d_sing);
-            CK(hipGetLastError());
-        };
-        auto sol = [&]() {
-            trisolve_kernel<<<1, 1>>>(n, d_Lp, d_Li, d_Lx, d_Up, d_Ui, d_Ux, d_pinv, d_q, d_b, d_xo,
-                                      d_wi, d_wd);
-            CK(hipGetLastError());
-        };
-        fac();
-        auto p = read_status("fp64 ST factor");
-        lnnz64 = p.first;
-        unnz64 = p.second;
-        t_f_st64 = time_best(fac, st_reps);
-        t_t_st64 = time_best(sol, reps);
-        auto h = download_xo();
-        printf("fp64 ST: factor=%.3f ms trisolve=%.3f ms  L2(vs CPU)=%.3e res=%.3e\n", t_f_st64,
-               t_t_st64, rel_l2(n, h.data(), x_cpu.data()), residual(M, h.data()));
-    }
-    if (do_st) {  // 2) fp32 single thread
-        auto fac = [&]() {
-            lu_factor_f32_kernel<<<1, 1>>>(n, d_Ap, d_Ai, d_Ax32, d_diag, d_Lp32, d_Li32, d_Lx32,
-                                           Lcap, d_Up32, d_Ui32, d_Ux32, Ucap, d_pinv32, d_xi32,
-                                           d_x32, d_lnnz, d_unnz, d_sing);
-            CK(hipGetLastError());
-        };
-        auto sol = [&]() {
-            trisolve_f32_kernel<<<1, 1>>>(n, d_Lp32, d_Li32, d_Lx32, d_Up32, d_Ui32, d_Ux32,
-                                          d_pinv32, d_q, d_b, d_xo, d_wi, d_wd);
-            CK(hipGetLastError());
-        };
-        fac();
-        auto p = read_status("fp32 ST factor");
-        lnnz32 = p.first;
-        unnz32 = p.second;
-        t_f_st32 = time_best(fac, st_reps);
-        t_t_st32 = time_best(sol, reps);
-        auto h = download_xo();
-        printf("fp32 ST: factor=%.3f ms trisolve=%.3f ms  L2(vs CPU)=%.3e res=%.3e\n", t_f_st32,
-               t_t_st32, rel_l2(n, h.data(), x_cpu.data()), residual(M, h.data()));
-    }
-    {  // 3) fp64 single workgroup
-        int dev = 0, maxlds = 0;
-        CK(hipGetDevice(&dev));
-        CK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        const size_t shb = LU_PAR_SHARED_BYTES(n);
-        if (shb <= (size_t)maxlds || shb <= 160 * 1024) {
-            if (shb > 65536)
-                CK(hipFuncSetAttribute((const void *)lu_factor_par_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shb));
-            auto fac = [&]() {
-                lu_factor_par_kernel<<<1, 256, shb>>>(n, d_Ap, d_Ai, d_Ax, d_diag, d_Lp, d_Li, d_Lx,
-                                                      Lcap, d_Up, d_Ui, d_Ux, Ucap, d_pinv, d_xip,
-                                                      nullptr, d_lnnz, d_unnz, d_sing);
-                CK(hipGetLastError());
-            };
-            auto sol = [&]() {
-                trisolve_par_kernel<<<1, 256>>>(n, d_Lp, d_Li, d_Lx, d_Up, d_Ui, d_Ux, d_pinv, d_q,
-                                                d_b, d_xo, d_wi, d_wd);
-                CK(hipGetLastError());
-            };
-            fac();
-            auto p = read_status("fp64 PAR factor");
-            lnnz64 = p.first;
-            unnz64 = p.second;
-            t_f_par64 = time_best(fac, reps);
-            t_t_par64 = time_best(sol, reps);
-            auto h = download_xo();
-            printf("fp64 PAR: factor=%.3f ms trisolve=%.3f ms  L2(vs CPU)=%.3e res=%.3e\n",
-                   t_f_par64, t_t_par64, rel_l2(n, h.data(), x_cpu.data()), residual(M, h.data()));
-        } else {
-            printf("fp64 PAR: n/a (needs %zu B of LDS, device limit %d B)\n", shb, maxlds);
-        }
-    }
-    if (t_f_st32 >= 0) {  // 4) fp32 parallel trisolve on the fp32 ST factors
-        auto sol = [&]() {
-            trisolve_par_f32_kernel<<<1, 256>>>(n, d_Lp32, d_Li32, d_Lx32, d_Up32, d_Ui32, d_Ux32,
-                                                d_pinv32, d_q, d_b, d_xo, d_wi, d_wd);
-            CK(hipGetLastError());
-        };
-        t_t_par32 = time_best(sol, reps);
-        auto h = download_xo();
-        printf("fp32 PAR trisolve: %.3f ms  L2(vs CPU)=%.3e res=%.3e\n", t_t_par32,
-               rel_l2(n, h.data(), x_cpu.data()), residual(M, h.data()));
-    } else {
-        printf("fp32 PAR trisolve: n/a (fp32 ST factor skipped at n>%d)\n", max_n_st);
+    std::vector<int> src;
+    lsp_coo_to_csc(P.n, nnz, P.row.data(), P.col.data(), P.q.data(), P.Ap, P.Ai, src);
+    P.Ax.resize(nnz);
+    P.Ax32.resize(nnz);
+    for (long p = 0; p < nnz; p++) {
+        P.Ax[p] = P.M.val[src[p]];
+        P.Ax32[p] = (float)P.Ax[p];
     }
 
-    // 5) device-resident path (luspice.h)
-    double t_rf32 = -1, t_rf64 = -1, t_tri32 = -1, t_ir = -1, ir_err = -1, ir_res = -1;
-    int ir_inner = -1;
+    // the CPU reference takes the natural-order CSC and q
     {
-        std::vector<double> v0(M.nnz);
-        for (long i = 0; i < M.nnz; i++) v0[i] = M.val[i];
-        lsp_circuit_in cin{n, M.nnz, r0.data(), c0.data(), v0.data(), q.data()};
-        lsp_batch *B = nullptr;
-        if (lsp_batch_create(&cin, 1, nullptr, &B) == LSP_OK) {
-            lsp_refactor_report rr{};
-            lsp_solve_report sr{};
-            std::vector<double> xs(n);
-            double rf = 0, fw = 0, bw = 0;
-            t_rf64 = 1e300;
-            for (int r = 0; r <= reps; r++)
-                if (lsp_batch_refactor(B, LSP_PREC_FP64, &rr) == LSP_OK && r > 0)
-                    t_rf64 = std::min(t_rf64, rr.ms);
-            t_rf32 = 1e300;
-            for (int r = 0; r <= reps; r++)
-                if (lsp_batch_refactor(B, LSP_PREC_FP32, &rr) == LSP_OK && r > 0)
-                    t_rf32 = std::min(t_rf32, rr.ms);
-            t_tri32 = 1e300;
-            for (int r = 0; r <= reps; r++) {
-                if (lsp_batch_trisolve(B, M.b + 1, xs.data(), 0) == LSP_OK && r > 0) {
-                    lsp_batch_last_times(B, &rf, &fw, &bw);
-                    t_tri32 = std::min(t_tri32, fw + bw);
-                }
-            }
-            hipEvent_t e0, e1;
-            CK(hipEventCreate(&e0));
-            CK(hipEventCreate(&e1));
-            CK(hipEventRecord(e0, nullptr));
-            lsp_batch_solve(B, M.b + 1, xs.data(), 0, &sr);
-            CK(hipEventRecord(e1, nullptr));
-            CK(hipDeviceSynchronize());
-            float ms = 0;
-            CK(hipEventElapsedTime(&ms, e0, e1));
-            t_ir = ms;
-            ir_inner = sr.inner_total;
-            ir_res = residual(M, xs.data());
-            ir_err = rel_l2(n, xs.data(), x_cpu.data());
-            printf("device path: refactor fp64=%.3f ms fp32=%.3f ms, trisolve fp32=%.3f ms, "
-                   "GMRES-IR %.3f ms (%d inner) L2(vs CPU)=%.3e res=%.3e\n",
-                   t_rf64, t_rf32, t_tri32, t_ir, ir_inner, ir_err, ir_res);
-            lsp_batch_destroy(B);
-        } else {
-            printf("device path: batch creation failed\n");
+        std::vector<int> Ap0, Ai0, src0;
+        lsp_coo_to_csc(P.n, nnz, P.row.data(), P.col.data(), nullptr, Ap0, Ai0, src0);
+        std::vector<double> Ax0(nnz);
+        for (long p = 0; p < nnz; p++) Ax0[p] = P.M.val[src0[p]];
+        P.x_cpu.resize(P.n);
+        const int rc = cpu_lu_ref(P.n, Ap0.data(), Ai0.data(), Ax0.data(), P.q.data(), P.b.data(), 1.0,
+                                  P.x_cpu.data(), &P.lnnz_cpu, &P.unnz_cpu);
+        if (rc >= 0) {
+            fprintf(stderr, "mp_lu_bench: CPU reference singular at column %d\n", rc);
+            return 1;
         }
     }
+    printf("CPU ref: lnnz=%ld unnz=%ld (L+U=%ld)\n", P.lnnz_cpu, P.unnz_cpu, P.lnnz_cpu + P.unnz_cpu);
 
-    const double l64 = lnnz64 > 0 ? lnnz64 : lnnz_cpu, u64 = unnz64 > 0 ? unnz64 : unnz_cpu;
-    const double l32 = lnnz32 > 0 ? lnnz32 : lnnz_cpu, u32 = unnz32 > 0 ? unnz32 : unnz_cpu;
-    const double mem64 = (l64 + u64) * 12.0, mem32 = (l32 + u32) * 8.0;
-    const double ratio_st = (t_f_st64 > 0 && t_f_st32 > 0) ? t_f_st64 / t_f_st32 : -1;
-    const double ratio_ts_st = (t_t_st64 > 0 && t_t_st32 > 0) ? t_t_st64 / t_t_st32 : -1;
-    const double ratio_ts_par = (t_t_par64 > 0 && t_t_par32 > 0) ? t_t_par64 / t_t_par32 : -1;
-    printf("\n== SUMMARY %s n=%d nnz=%ld ==\n", argv[1], n, M.nnz);
-    printf("JSON:{\"matrix\":\"%s\",\"n\":%d,\"nnz\":%ld,\"lnnz_cpu\":%ld,\"unnz_cpu\":%ld,"
-           "\"lnnz64\":%ld,\"unnz64\":%ld,\"lnnz32\":%ld,\"unnz32\":%ld,"
-           "\"mem_fp64_mb\":%.3f,\"mem_fp32_mb\":%.3f,"
-           "\"t_factor_st64_ms\":%.3f,\"t_factor_st32_ms\":%.3f,"
-           "\"t_factor_par64_ms\":%.3f,"
-           "\"t_trisolve_st64_ms\":%.3f,\"t_trisolve_st32_ms\":%.3f,"
-           "\"t_trisolve_par64_ms\":%.3f,\"t_trisolve_par32_ms\":%.3f,"
-           "\"speedup_factor_st\":%.3f,\"speedup_trisolve_st\":%.3f,\"speedup_trisolve_par\":%.3f,"
-           "\"t_refactor_dev64_ms\":%.3f,\"t_refactor_dev32_ms\":%.3f,\"t_trisolve_dev32_ms\":%.3f,"
-           "\"t_gmres_ir_ms\":%.3f,\"ir_inner\":%d,\"ir_rel_err\":%.3e,\"ir_res\":%.3e}\n",
-           argv[1], n, M.nnz, lnnz_cpu, unnz_cpu, lnnz64, unnz64, lnnz32, unnz32, mem64 / 1e6,
-           mem32 / 1e6, t_f_st64, t_f_st32, t_f_par64, t_t_st64, t_t_st32, t_t_par64, t_t_par32,
-           ratio_st, ratio_ts_st, ratio_ts_par, t_rf64, t_rf32, t_tri32, t_ir, ir_inner, ir_err,
-           ir_res);
-    lsp_s0_free(&M);
+    Arena dev;
+    const int n = P.n;
+    const long base = 2L * nnz + 64;
+    Run R{P, dev, std::max(base, P.lnnz_cpu > base ? (long)(P.lnnz_cpu * 1.5) + 64 : base),
+          std::max(base, P.unnz_cpu > base ? (long)(P.unnz_cpu * 1.5) + 64 : base), max_n_st, reps, st_reps};
+    for (int s = 0; s < 2; s++) {
+        FactorSet &F = R.set[s];
+        const size_t v = s == F64 ? sizeof(double) : sizeof(float);
+        F.Lp = dev.get<int>(n + 1);
+        F.Up = dev.get<int>(n + 1);
+        F.Li = dev.get<int>(R.Lcap);
+        F.Ui = dev.get<int>(R.Ucap);
+        F.pinv = dev.get<int>(n);
+        F.Lx = dev.get<char>(R.Lcap * v);
+        F.Ux = dev.get<char>(R.Ucap * v);
+        F.work = dev.get<char>((size_t)n * v);
+    }
+    R.Ap = dev.get<int>(n + 1, P.Ap.data());
+    R.Ai = dev.get<int>(nnz, P.Ai.data());
+    R.Ax = dev.get<double>(nnz, P.Ax.data());
+    R.Ax32 = dev.get<float>(nnz, P.Ax32.data());
+    R.diag = dev.get<int>(n, P.q.data());           // diagrow[k] = q[k]
+    R.q = dev.get<int>(n, P.q.data());
+    R.b = dev.get<double>(n, P.b.data());
+    R.xo = dev.get<double>(n);
+    R.xi = dev.get<int>(2 * (size_t)n + 1);
+    R.xi_wg = dev.get<int>((size_t)n + 4 * LU_PAR_CAP);
+    R.wi = dev.get<int>(n);
+    R.wd = dev.get<double>(2 * (size_t)n);
+    R.lnnz = dev.get<int>(1);
+    R.unnz = dev.get<int>(1);
+    R.sing = dev.get<int>(1);
+
+    Stage stages[] = {
+        {"fp64 ST", F64, SINGLE_THREAD, SINGLE_THREAD},
+        {"fp32 ST", F32, SINGLE_THREAD, SINGLE_THREAD},
+        {"fp64 PAR", F64, WORKGROUP, WORKGROUP},
+        {"fp32 PAR trisolve", F32, NONE, WORKGROUP},
+    };
+    for (Stage &s : stages) R.run(s);
+    const Stage &st64 = stages[0], &st32 = stages[1], &wg64 = stages[2], &wg32 = stages[3];
+
+    const DevicePath d = device_path(P, reps);
+
+    const FactorSet &f64 = R.set[F64], &f32 = R.set[F32];
+    const double l64 = f64.lnnz > 0 ? f64.lnnz : P.lnnz_cpu, u64 = f64.unnz > 0 ? f64.unnz : P.unnz_cpu;
+    const double l32 = f32.lnnz > 0 ? f32.lnnz : P.lnnz_cpu, u32 = f32.unnz > 0 ? f32.unnz : P.unnz_cpu;
+    Json J;
+    J.add("matrix", "\"%s\"", argv[1]);
+    J.add("n", "%d", n);
+    J.add("nnz", "%ld", nnz);
+    J.add("lnnz_cpu", "%ld", P.lnnz_cpu);
+    J.add("unnz_cpu", "%ld", P.unnz_cpu);
+    J.add("lnnz64", "%ld", f64.lnnz);
+    J.add("unnz64", "%ld", f64.unnz);
+    J.add("lnnz32", "%ld", f32.lnnz);
+    J.add("unnz32", "%ld", f32.unnz);
+    J.add("mem_fp64_mb", "%.3f", (l64 + u64) * 12.0 / 1e6);
+    J.add("mem_fp32_mb", "%.3f", (l32 + u32) * 8.0 / 1e6);
+    J.add("t_factor_st64_ms", "%.3f", st64.t_factor);
+    J.add("t_factor_st32_ms", "%.3f", st32.t_factor);
+    J.add("t_factor_par64_ms", "%.3f", wg64.t_factor);
+    J.add("t_trisolve_st64_ms", "%.3f", st64.t_solve);
+    J.add("t_trisolve_st32_ms", "%.3f", st32.t_solve);
+    J.add("t_trisolve_par64_ms", "%.3f", wg64.t_solve);
+    J.add("t_trisolve_par32_ms", "%.3f", wg32.t_solve);
+    J.add("speedup_factor_st", "%.3f", ratio(st64.t_factor, st32.t_factor));
+    J.add("speedup_trisolve_st", "%.3f", ratio(st64.t_solve, st32.t_solve));
+    J.add("speedup_trisolve_par", "%.3f", ratio(wg64.t_solve, wg32.t_solve));
+    J.add("t_refactor_dev64_ms", "%.3f", d.rf64);
+    J.add("t_refactor_dev32_ms", "%.3f", d.rf32);
+    J.add("t_trisolve_dev32_ms", "%.3f", d.tri32);
+    J.add("t_gmres_ir_ms", "%.3f", d.ir_ms);
+    J.add("ir_inner", "%d", d.inner);
+    J.add("ir_rel_err", "%.3e", d.err);
+    J.add("ir_res", "%.3e", d.res);
+    printf("\n== SUMMARY %s n=%d nnz=%ld ==\nJSON:%s}\n", argv[1], n, nnz, J.body.c_str());
+    lsp_s0_free(&P.M);
     return 0;
 }

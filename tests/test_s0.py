@@ -160,3 +160,46 @@ def test_clis_built():
     for name in ("mp_lu_bench", "mp_spmv_bench"):
         p = os.path.join(S.BIN, name)
         assert os.path.isfile(p) and os.access(p, os.X_OK), p
+
+
+def test_reader_edge_rules(tmp_path):
+    """Rules the reference's readers have at the edges (src/spmv.cu:42-145,
+    src/mp/mp_lu_bench.cu:46-111): read_s0_to_csr drops rows outside [1, n]
+    and columns below 1 but keeps a column above n; a matrix file without any
+    "A" line is an error, one whose "A" lines all fail to parse is an empty
+    matrix; read_rhs skips the first line while lsp_s0_read reads every line
+    of the RHS file; a pin on a row that had to be appended lands on the
+    appended entry; windows line ends and blanks inside records parse."""
+    N.lib.read_rhs.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_double)]
+    N.lib.read_rhs.restype = C.c_int
+    m = tmp_path / "e.mtx"
+    m.write_text("n 3\r\nA 1 1 1.0\r\nA 4 1 9.0\nA 0 2 9.0\nA 2 5 3.0\nA 3 0 9.0\nA  3   3  -2e-3\n")
+    n, rp, ci, v = _read_csr(str(m))
+    assert n == 3 and rp.tolist() == [0, 1, 2, 3]
+    assert ci.tolist() == [0, 4, 2] and v.tolist() == [1.0, 3.0, -2e-3]
+    none = tmp_path / "none.mtx"
+    none.write_text("n 3\nB 1 1 1.0\n")
+    A = CSRMatrix()
+    assert N.lib.read_s0_to_csr(str(none).encode(), C.byref(A)) == 1
+    bad = tmp_path / "bad.mtx"
+    bad.write_text("n 3\nA 1 x 1.0\nA 2\n")
+    n, rp, ci, v = _read_csr(str(bad))
+    assert n == 3 and rp.tolist() == [0, 0, 0, 0] and ci.size == 0
+    hdr = tmp_path / "h.mtx"
+    hdr.write_text(" n 3\nA 1 1 1.0\n")
+    assert N.lib.read_s0_to_csr(str(hdr).encode(), C.byref(A)) == 1
+    r = tmp_path / "first.rhs"
+    r.write_text("RHS 1 5.0\nRHS 2 6.0\nRHS 9 7.0\n")
+    x = np.full(3, -1.0)
+    assert N.lib.read_rhs(str(r).encode(), 3, x.ctypes.data_as(C.POINTER(C.c_double))) == 0
+    assert x.tolist() == [-1.0, 6.0, -1.0]
+    M = _read_cli(str(m), str(r))
+    assert M["b"].tolist() == [5.0, 6.0, 1.0]
+    assert M["row"].tolist() == [0, 3, -1, 1, 2, 2]           # unchecked, file order
+    flt = tmp_path / "twice.flt"
+    flt.write_text("F 2\nF 2\nF 1\n")
+    m2 = tmp_path / "p2.mtx"
+    m2.write_text("n 2\nA 1 1 1.0\nA 1 1 5.0\n")
+    P = _read_cli(str(m2), str(r), str(flt), 0.25)
+    assert P["added"] == 1
+    assert P["val"].tolist() == [1.25, 5.0, 0.5] and P["row"].tolist() == [0, 0, 1]
