@@ -151,6 +151,20 @@ int  lsp_batch_estimate_bytes(const lsp_batch *b, size_t *bytes);
 int  lsp_batch_upload(lsp_batch *b);
 /* Device memory currently held by the batch. */
 int  lsp_batch_device_bytes(const lsp_batch *b, size_t *bytes);
+/* Continuous batching: a prepared batch (host phase done; call
+ * lsp_batch_upload) whose circuit i is circuit circuit_of[i] of
+ * parts[part_of[i]], moved over without repeating the symbolic phase, so a
+ * serving loop can merge newly prepared requests with the ones in flight
+ * into one batch (one refactor launch and one GMRES-IR per step for all).
+ * Every batch in parts is destroyed by the call, whatever the result (its
+ * handle is invalid afterwards); circuits not picked are dropped.  The new
+ * batch takes the options of parts[0]. */
+int  lsp_batch_adopt(lsp_batch *const *parts, int nparts, const int *part_of,
+                     const int *circuit_of, int ncircuits, lsp_batch **out);
+/* The circuits lsp_batch_refactor and lsp_batch_solve work on: those with
+ * active[c] != 0 (NULL: every circuit).  The others keep their factors and
+ * solutions untouched.  graph_step, trisolve and spmv cover every circuit. */
+int  lsp_batch_set_active(lsp_batch *b, const int *active);
 
 /* Plain device-memory helpers for callers without their own HIP runtime
  * binding (staging inputs in HBM): allocate, copy (any direction,

@@ -221,6 +221,48 @@ class Batch:
         check(N.lib.lsp_batch_upload(self._h), "lsp_batch_upload")
         self.uploaded = True
 
+    @classmethod
+    def adopt(cls, picks, upload=True) -> "Batch":
+        """A batch whose circuit i is circuit picks[i][1] of batch picks[i][0],
+        moved over with its host structures (no symbolic phase).  Every source
+        batch is consumed (closed); circuits not picked are dropped."""
+        parts = []
+        for b, _ in picks:
+            if all(b is not p for p in parts):
+                parts.append(b)
+        arr = (N.C.c_void_p * len(parts))(*[p._h for p in parts])
+        part_of = np.array([next(i for i, p in enumerate(parts) if p is b) for b, _ in picks], np.int32)
+        circ_of = np.array([c for _, c in picks], np.int32)
+        opts = parts[0].opts
+        h = N.C.c_void_p()
+        rc = N.lib.lsp_batch_adopt(arr, len(parts), N.ptr(part_of), N.ptr(circ_of), len(picks),
+                                   N.C.byref(h))
+        for p in parts:
+            p._h = None                       # destroyed by the call
+        check(rc, "lsp_batch_adopt")
+        circuits = [b.circuits[c] for b, c in picks]
+        self = cls.__new__(cls)
+        self.circuits = circuits
+        self.opts = opts
+        self._h = h
+        self.total_n = sum(c.n for c in circuits)
+        self.total_nnz = sum(c.nnz for c in circuits)
+        self.uploaded = False
+        self.voff = np.cumsum([0] + [c.n for c in circuits])
+        self.aoff = np.cumsum([0] + [c.nnz for c in circuits])
+        if upload:
+            self.upload()
+        return self
+
+    def set_active(self, mask=None):
+        """Circuits refactor() and solve() work on (None: all)."""
+        if mask is None:
+            check(N.lib.lsp_batch_set_active(self._h, None), "set_active")
+            return
+        m = np.ascontiguousarray(np.asarray(mask, bool).astype(np.int32))
+        assert m.size == len(self.circuits)
+        check(N.lib.lsp_batch_set_active(self._h, N.ptr(m)), "set_active")
+
     def estimate_bytes(self) -> int:
         v = N.C.c_size_t()
         check(N.lib.lsp_batch_estimate_bytes(self._h, N.C.byref(v)), "estimate_bytes")

@@ -87,6 +87,9 @@ _sig("lsp_batch_prepare", C.c_int, C.POINTER(CircuitIn), C.c_int, C.POINTER(Opti
 _sig("lsp_batch_estimate_bytes", C.c_int, C.c_void_p, C.POINTER(C.c_size_t))
 _sig("lsp_batch_upload", C.c_int, C.c_void_p)
 _sig("lsp_batch_device_bytes", C.c_int, C.c_void_p, C.POINTER(C.c_size_t))
+_sig("lsp_batch_adopt", C.c_int, C.POINTER(C.c_void_p), C.c_int, c_int_p, c_int_p, C.c_int,
+     C.POINTER(C.c_void_p))
+_sig("lsp_batch_set_active", C.c_int, C.c_void_p, c_int_p)
 _sig("lsp_batch_set_stream", C.c_int, C.c_void_p, C.c_void_p)
 _sig("lsp_batch_info", C.c_int, C.c_void_p, C.c_int, C.POINTER(CircuitInfo))
 _sig("lsp_batch_sizes", C.c_int, C.c_void_p, c_long_p, c_long_p)

@@ -345,6 +345,42 @@ int Batch::prepare(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
     return LSP_OK;
 }
 
+// Moves circuit picks[i].second of batch picks[i].first into circuit i of this
+// (empty) batch.  Only host structures move (pattern, pivot sequence, plan,
+// SELL layout, values); the device phase is upload_all().  Every source batch
+// is emptied: its circuits not picked are dropped.  The options are those of
+// the first source.
+int Batch::adopt(const std::vector<std::pair<Batch *, int>> &picks)
+{
+    if (!circ_.empty() || uploaded_ || picks.empty()) return LSP_E_ARG;
+    std::vector<std::pair<Batch *, int>> sorted(picks);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end()) return LSP_E_ARG;   // picked twice
+    for (const auto &pk : picks) {
+        if (!pk.first || pk.first == this || pk.second < 0 || pk.second >= (int)pk.first->circ_.size())
+            return LSP_E_ARG;
+        if (pk.first->circ_[pk.second].n <= 0) return LSP_E_ARG;
+    }
+    opt_ = picks[0].first->opt_;
+    refactor_tl_path_ = picks[0].first->refactor_tl_path_;
+    trisolve_tl_path_ = picks[0].first->trisolve_tl_path_;
+    circ_.reserve(picks.size());
+    for (const auto &pk : picks) circ_.push_back(std::move(pk.first->circ_[pk.second]));
+    for (const auto &pk : picks) pk.first->circ_.clear();
+    return LSP_OK;
+}
+
+int Batch::set_active(const int *mask)
+{
+    if (!mask) {
+        active_.clear();
+        return LSP_OK;
+    }
+    active_.assign(mask, mask + circ_.size());
+    for (int &a : active_) a = a ? 1 : 0;
+    return LSP_OK;
+}
+
 // Device bytes upload_all() and the first fp32 + fp64 refactors will hold
 // (an upper estimate used for admission; device_bytes() reports the actual).
 size_t Batch::estimate_device_bytes() const
@@ -1038,6 +1074,7 @@ int Batch::refactor(int prec, lsp_refactor_report *rep, const std::vector<int> *
     std::vector<int> ms;
     int rc = check_prec(prec);
     if (rc) return rc;
+    if (!subset && !active_.empty()) subset = &active_;
     rc = refactor_pass(prec, subset, ms);
     if (rc) return rc;
     auto in_subset = [&](int m) { return !subset || (*subset)[m]; };
@@ -1343,13 +1380,20 @@ int Batch::solve(const double *b, double *x, bool on_device, lsp_solve_report *r
     }
     rc = check(k_zero_int(d_abort_ + 1, 1, stream_), "abort reset");
     if (rc) return rc;
+    // the circuits solved: the set_active mask, else all
+    const int nc = (int)circ_.size();
+    std::vector<char> live;
+    int nlive = nc;
+    if (!active_.empty()) {
+        live.assign(active_.begin(), active_.end());
+        nlive = (int)std::count(active_.begin(), active_.end(), 1);
+    }
     lsp_solve_report r{};
-    rc = gmres_ir(db, dx, cur_prec_, nullptr, &r);
+    rc = gmres_ir(db, dx, cur_prec_, live.empty() ? nullptr : &live, &r);
     if (rc) return rc;
     // per-circuit fallback to fp64 factors when IR stalls
-    if (opt_.fallback_fp64 && cur_prec_ != LSP_PREC_FP64 && r.converged < (int)circ_.size()) {
+    if (opt_.fallback_fp64 && cur_prec_ != LSP_PREC_FP64 && r.converged < nlive) {
         // identify stalled circuits from the final residuals
-        const int nc = (int)circ_.size();
         std::vector<double> beta(nc), nb(nc);
         (void)hipMemcpy(beta.data(), d_beta_, nc * 8, hipMemcpyDeviceToHost);
         (void)hipMemcpy(nb.data(), d_nb_, nc * 8, hipMemcpyDeviceToHost);
@@ -1357,6 +1401,7 @@ int Batch::solve(const double *b, double *x, bool on_device, lsp_solve_report *r
         std::vector<char> subc(nc, 0);
         int cnt = 0;
         for (int m = 0; m < nc; m++) {
+            if (!live.empty() && !live[m]) continue;
             double rel = nb[m] > 0 ? beta[m] / nb[m] : beta[m];
             if (!(rel <= opt_.ir_outer_tol * 10 || rel <= 1e-9)) { sub[m] = 1; subc[m] = 1; cnt++; }
         }
@@ -1383,6 +1428,7 @@ int Batch::solve(const double *b, double *x, bool on_device, lsp_solve_report *r
             (void)hipMemcpy(b2.data(), d_beta_, nc * 8, hipMemcpyDeviceToHost);
             r.max_rel_res = 0;
             for (int m = 0; m < nc; m++) {
+                if (!live.empty() && !live[m]) continue;
                 double rel = sub[m] ? (nb[m] > 0 ? b2[m] / nb[m] : b2[m])
                                     : (nb[m] > 0 ? beta[m] / nb[m] : beta[m]);
                 if (!(rel == rel)) rel = INFINITY;             // a diverged circuit stays visible
@@ -1395,7 +1441,7 @@ int Batch::solve(const double *b, double *x, bool on_device, lsp_solve_report *r
         if (!rc) rc = check(hipStreamSynchronize(stream_), "sync");
         if (rc) return rc;
     }
-    r.status = r.converged == (int)circ_.size() ? LSP_OK : LSP_E_NOT_CONVERGED;
+    r.status = r.converged == nlive ? LSP_OK : LSP_E_NOT_CONVERGED;
     if (rep) *rep = r;
     return r.status;
 }

@@ -128,6 +128,11 @@ public:
     ~Batch();
     int create(const lsp_circuit_in *c, int nc, const lsp_options *opt);
     int prepare(const lsp_circuit_in *c, int nc, const lsp_options *opt);   // host symbolic only
+    // A prepared batch made of circuits moved out of other batches (their host
+    // structures, no symbolic recomputation); the sources are left empty.
+    int adopt(const std::vector<std::pair<Batch *, int>> &picks);
+    // Circuits that refactor() and solve() work on (nullptr: all).
+    int set_active(const int *mask);
     int upload_all();                                                         // device phase
     size_t estimate_device_bytes() const;
     size_t device_bytes() const;
@@ -153,7 +158,7 @@ public:
     // repivot policy of later refactors (lsp_options::repivot)
     int set_repivot(int mode)
     {
-        if (mode < 0 || mode > 2) return LSP_E_ARG;
+        if (mode < 0 || mode > 3) return LSP_E_ARG;
         opt_.repivot = mode;
         return LSP_OK;
     }
@@ -196,6 +201,7 @@ private:
     hipStream_t stream_ = nullptr;
     bool own_stream_ = false;
     std::vector<CircuitHost> circ_;
+    std::vector<int> active_;             // set_active mask (empty: all circuits)
     std::vector<DevMem> mem_;
     int cur_prec_ = -1;
     bool uploaded_ = false;              // upload_all done (cleared by release_device)

@@ -66,6 +66,43 @@ int lsp_batch_create(const lsp_circuit_in *circuits, int ncircuits, const lsp_op
 
 void lsp_batch_destroy(lsp_batch *b) { delete b; }
 
+int lsp_batch_adopt(lsp_batch *const *parts, int nparts, const int *part_of, const int *circuit_of,
+                    int ncircuits, lsp_batch **out)
+{
+    int rc = LSP_OK;
+    if (!out || !parts || nparts <= 0 || !part_of || !circuit_of || ncircuits <= 0) rc = LSP_E_ARG;
+    if (out) *out = nullptr;
+    std::vector<std::pair<lsp::Batch *, int>> picks;
+    for (int i = 0; !rc && i < ncircuits; i++) {
+        if (part_of[i] < 0 || part_of[i] >= nparts || !parts[part_of[i]]) rc = LSP_E_ARG;
+        else picks.emplace_back(&parts[part_of[i]]->impl, circuit_of[i]);
+    }
+    lsp_batch *b = nullptr;
+    if (!rc) {
+        b = new (std::nothrow) lsp_batch();
+        if (!b) rc = LSP_E_ALLOC;
+    }
+    if (!rc) rc = b->impl.adopt(picks);
+    // the sources go whatever the result (each handle once)
+    if (parts)
+        for (int p = 0; p < nparts; p++) {
+            bool seen = false;
+            for (int r = 0; r < p && !seen; r++) seen = parts[r] == parts[p];
+            if (!seen) delete parts[p];
+        }
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return LSP_OK;
+}
+
+int lsp_batch_set_active(lsp_batch *b, const int *active)
+{
+    return b ? b->impl.set_active(active) : LSP_E_ARG;
+}
+
 int lsp_batch_set_stream(lsp_batch *b, void *s)
 {
     return b ? b->impl.set_stream((hipStream_t)s) : LSP_E_ARG;

@@ -143,11 +143,12 @@ int load_prec(StaticState &S, int prec)
 {
     lsp::Batch &B = S.batch->impl;
     if (B.cur_prec() == prec) return LSP_OK;
-    B.set_repivot(0);
+    int rc = B.set_repivot(0);
+    if (rc) return rc;
     lsp_refactor_report rep{};
-    const int rc = B.refactor(prec, &rep);
-    B.set_repivot(3);
-    return rc;
+    rc = B.refactor(prec, &rep);
+    const int rc2 = B.set_repivot(3);
+    return rc ? rc : rc2;
 }
 
 }  // namespace

@@ -89,6 +89,45 @@ def test_serve_same_alone_and_mixed():
 
 
 @pytest.mark.gpu
+def test_serve_cohorts_merge_and_overlap():
+    """Continuous batching: three requests prepared together run as one cohort
+    (one launch per step for all three); a fourth that arrives while that
+    cohort is on the GPU is admitted as a cohort of its own and its steps
+    overlap the first cohort's in time.  Each result equals the request run
+    alone to 1e-12 and the fp64 CPU LU to 1e-6."""
+    print("case: cohorts merge and overlap", flush=True)
+    reqs = S.make_requests(4, rate=1e6, nmin=500, nmax=1900, smin=40, smax=40, seed=6)
+    for r in reqs[:3]:
+        r.arrival = 0.0
+    reqs[3].arrival = 0.02
+    srv = S.Server(8 << 30, slots=4, keep_results=True, gather_ms=200.0)
+    srv.log_steps = True
+    st = srv.run(reqs)
+    srv.close()
+    print(st, flush=True)
+    assert st["errors"] == 0 and st["solves"] == 160, st
+    by_cohort = {}
+    for cid, ids, t0, t1 in srv.step_log:
+        by_cohort.setdefault(cid, []).append((ids, t0, t1))
+    first = [c for c, steps in by_cohort.items() if any(len(ids) == 3 for ids, _, _ in steps)]
+    late = [c for c, steps in by_cohort.items() if all(ids == (reqs[3].rid,) for ids, _, _ in steps)]
+    assert first and late, by_cohort.keys()
+    a, b = by_cohort[first[0]], by_cohort[late[0]]
+    assert b[0][1] < a[-1][2]                 # the late cohort started before the first one ended
+    for r in reqs:
+        alone = S.Request(r.rid, 0.0, r.circuit, r.steps, r.seed)
+        s1 = S.Server(8 << 30, slots=1, keep_results=True)
+        s1.run([alone])
+        s1.close()
+        for x, y in zip(r.results, alone.results):
+            assert _rel(x, y) <= 1e-12
+        vals, rhs = r.step_inputs(r.steps - 1)
+        c = r.circuit
+        rc, xr, _, _ = O.cpu_lu_ref(c.n, c.row, c.col, vals, c.q, rhs)
+        assert _rel(r.results[-1], xr) <= 1e-6
+
+
+@pytest.mark.gpu
 def test_serve_admission_respects_budget():
     print("case: admission under a tight HBM budget", flush=True)
     reqs = S.make_requests(5, rate=1e6, nmin=1000, nmax=2000, smin=2, smax=3, seed=5)
