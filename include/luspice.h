@@ -88,6 +88,11 @@ typedef struct {
                               the FP16 factor path once faulted the GPU with no cause shown, so
                               refactor / graph_step refuse FP16 with LSP_E_ARG unless the caller
                               opts in */
+    int ir_method;         /* solve: 0 fp64 GMRES-IR (default); 1 fused Richardson steps first
+                              (lsp_batch_ir_step, one launch per step for the whole batch, from
+                              x = 0, until ||b - A x|| <= ir_outer_tol ||b||), the circuits they
+                              leave unconverged after ir_fused_steps are solved by GMRES-IR */
+    int ir_fused_steps;    /* most fused steps per solve (0 = 12) */
 } lsp_options;
 
 typedef struct {
@@ -175,6 +180,12 @@ int  lsp_dev_copy(void *dst, const void *src, size_t bytes);
 int  lsp_dev_sync(void);
 int  lsp_dev_mem_info(size_t *free_bytes, size_t *total_bytes);
 void lsp_batch_destroy(lsp_batch *b);
+/* Ruiz equilibration scales of a COO matrix (`sweeps` alternating row and
+ * column sweeps, each dividing by the square root of the largest |entry| of
+ * R A C in the row / column): R, C of length n.  The scaling of
+ * bench/test_lu_gpu_f32.cu:247-269. */
+int  lsp_ruiz(int n, long nnz, const int *row, const int *col, const double *val, int sweeps,
+              double *R, double *C);
 int  lsp_batch_set_stream(lsp_batch *b, void *hip_stream);
 int  lsp_batch_info(const lsp_batch *b, int circuit, lsp_circuit_info *info);
 int  lsp_batch_sizes(const lsp_batch *b, long *total_n, long *total_nnz);
@@ -184,6 +195,17 @@ int  lsp_batch_sizes(const lsp_batch *b, long *total_n, long *total_nnz);
 int  lsp_batch_set_values(lsp_batch *b, const double *vals, int on_device);
 /* Numeric refactorization of every circuit (one persistent launch). */
 int  lsp_batch_refactor(lsp_batch *b, int prec, lsp_refactor_report *rep);
+/* One fused iterative-refinement step for every circuit, in one launch:
+ * r = b - A x (fp64 values), x += Q U^-1 L^-1 P r with the current factors
+ * (fp64 accumulation); rel_res (optional, one per circuit) gets ||r|| / ||b||
+ * of the x passed in.  The reference runs the same step as one kernel for one
+ * matrix (src/mp/fusion_bench.cu:119-163) and adds the correction on the
+ * host. */
+int  lsp_batch_ir_step(lsp_batch *b, const double *rhs, double *x, int on_device, double *rel_res);
+/* Smallest and largest |u_kk| of one circuit's current factors: the input of
+ * the pivot guard (min |pivot| < 1e-9 max |A| -> fp64 factors), which the
+ * reference states in docs/02_precision_budget.md:82-83. */
+int  lsp_batch_pivot_range(lsp_batch *b, int circuit, double *min_abs, double *max_abs);
 /* Direct solve x = Q U^-1 L^-1 P b with the current factors. */
 int  lsp_batch_trisolve(lsp_batch *b, const double *rhs, double *x, int on_device);
 /* fp64 GMRES-IR with the current factors as right preconditioner. */

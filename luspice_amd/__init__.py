@@ -324,6 +324,23 @@ class Batch:
         out["rc"] = rc
         return x, out
 
+    def pivot_range(self, i=0):
+        """(min |u_kk|, max |u_kk|) of circuit i's current factors."""
+        lo, hi = N.C.c_double(), N.C.c_double()
+        check(N.lib.lsp_batch_pivot_range(self._h, i, N.C.byref(lo), N.C.byref(hi)), "pivot_range")
+        return lo.value, hi.value
+
+    def ir_step(self, x: np.ndarray, rhs: np.ndarray):
+        """One fused refinement step for every circuit (one launch):
+        x + Q U^-1 L^-1 P (rhs - A x) with the current factors; returns the new
+        x and ||rhs - A x|| / ||rhs|| of the x passed in, per circuit."""
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        xn = np.array(x, np.float64, copy=True)
+        rel = np.empty(len(self.circuits))
+        check(N.lib.lsp_batch_ir_step(self._h, rhs.ctypes.data_as(N.C.c_void_p),
+                                      xn.ctypes.data_as(N.C.c_void_p), 0, N.ptr(rel)), "ir_step")
+        return xn, rel
+
     def graph_step(self, prec="fp32"):
         """One Newton step on the resident b / x as a replayed HIP graph
         (refactor + GMRES cycle + residual); returns (refactor, solve) reports."""

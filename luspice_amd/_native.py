@@ -42,7 +42,7 @@ class Options(C.Structure):
                 ("tail_max", C.c_int), ("tail_density", C.c_double), ("ruiz_sweeps", C.c_int),
                 ("pivot_scaled", C.c_int), ("poll_backoff", C.c_int),
                 ("factor_shift", C.c_double), ("krylov_spmv", C.c_int), ("repivot", C.c_int),
-                ("allow_fp16", C.c_int)]
+                ("allow_fp16", C.c_int), ("ir_method", C.c_int), ("ir_fused_steps", C.c_int)]
 
 
 class RefactorReport(C.Structure):
@@ -99,6 +99,9 @@ _sig("lsp_batch_trisolve", C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int)
 _sig("lsp_batch_solve", C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
      C.POINTER(SolveReport))
 _sig("lsp_batch_spmv", C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int)
+_sig("lsp_batch_ir_step", C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, c_double_p)
+_sig("lsp_batch_pivot_range", C.c_int, C.c_void_p, C.c_int, c_double_p, c_double_p)
+_sig("lsp_ruiz", C.c_int, C.c_int, C.c_long, c_int_p, c_int_p, c_double_p, C.c_int, c_double_p, c_double_p)
 _sig("lsp_batch_export_factors", C.c_int, C.c_void_p, C.c_int, c_int_p, c_int_p, c_int_p,
      c_double_p, c_int_p, c_int_p, c_double_p)
 _sig("lsp_batch_graph_step", C.c_int, C.c_void_p, C.c_int, C.POINTER(RefactorReport), C.POINTER(SolveReport))

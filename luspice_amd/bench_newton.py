@@ -11,7 +11,9 @@ rel err <= 1e-6.
   batched: all circuits in one resident batch, one persistent refactor launch
            + one batched fp64 GMRES-IR per step, inputs in HBM, the whole step
            replayed as one HIP graph (--no-graph: eager); factor storage fp32
-           (the mixed path) and fp64, each on a batch of its own
+           (the mixed path) and fp64, each on a batch of its own; with
+           --ir-methods 1 the solve is fused Richardson steps instead (one
+           launch per step for the whole batch, GMRES-IR for what is left)
   plain:   one batch per circuit, fp64 factors, host inputs each step (timed
            over --plain-steps steps): the one-matrix-per-launch fp64 path
 
@@ -52,14 +54,14 @@ def _errors(circ, NB, last, check):
     return np.array(errs)
 
 
-def batched(circ, prec, steps, warmup, seed, graph=True, check=-1, host=None):
+def batched(circ, prec, steps, warmup, seed, graph=True, check=-1, host=None, ir_method=0):
     """Runs `steps` timed Newton steps (after `warmup`) of the whole mix in one
     resident batch with `prec` factor storage.  Returns the result record and
     the step inputs (reused by the plain path)."""
     from . import device_sync
     from . import workloads as W
     t0 = time.time()
-    NB = W.NewtonBatched(circ, steps + warmup, seed, host_inputs=host)
+    NB = W.NewtonBatched(circ, steps + warmup, seed, host_inputs=host, ir_method=ir_method)
     tails = [NB.B.info(i)["tail_T"] for i in range(len(circ))]
     _log(f"[newton] {prec} batch built {time.time() - t0:.1f}s, dense tails "
          f"{sum(t > 0 for t in tails)}, max T {max(tails)}")
@@ -95,6 +97,7 @@ def batched(circ, prec, steps, warmup, seed, graph=True, check=-1, host=None):
         "bad_pivots": int(sum(r[0].get("bad_pivots", 0) for r in reps)),
         "dense_tails": int(sum(t > 0 for t in tails)), "max_tail_T": int(max(tails)),
         "hip_graph": bool(graph),
+        "ir": "fused Richardson steps" if ir_method == 1 else "GMRES-IR",
     }
     host = NB.host
     NB.close()
@@ -126,7 +129,7 @@ def plain(circ, host, plain_steps):
 
 
 def compare(ncirc=512, steps=20, warmup=1, plain_steps=2, nmin=100, nmax=5000, seed=3,
-            precs=("fp32", "fp64"), graph=True, check=-1):
+            precs=("fp32", "fp64"), graph=True, check=-1, ir_methods=(0,)):
     """Batched (each precision in `precs`) against plain fp64 on one mix."""
     from . import workloads as W
     t0 = time.time()
@@ -137,14 +140,19 @@ def compare(ncirc=512, steps=20, warmup=1, plain_steps=2, nmin=100, nmax=5000, s
                        f"{steps} Newton steps (values x lognormal(0, 0.1), b ~ U[-1, 1])",
            "circuits": len(circ), "total_n": ntot, "steps": steps}
     host = None
-    for prec in precs:
-        rec, host = batched(circ, prec, steps, warmup, seed, graph=graph, check=check, host=host)
-        out["mixed" if prec == "fp32" else prec] = rec
-    p = plain(circ, host, plain_steps)
+    keys = []
+    for irm in ir_methods:
+        for prec in precs:
+            rec, host = batched(circ, prec, steps, warmup, seed, graph=graph, check=check, host=host,
+                                ir_method=irm)
+            key = ("mixed" if prec == "fp32" else prec) + ("_fused" if irm == 1 else "")
+            out[key] = rec
+            keys.append(key)
+    p = plain(circ, host, plain_steps) if plain_steps > 0 else None
     out["plain_fp64"] = p
-    for prec in precs:
-        key = "mixed" if prec == "fp32" else prec
-        out[key]["over_plain"] = round(out[key]["value"] / p["value"], 2)
+    for key in keys:
+        if p:
+            out[key]["over_plain"] = round(out[key]["value"] / p["value"], 2)
     if "mixed" in out and "fp64" in out:
         out["mixed_over_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 3)
     return out
@@ -163,9 +171,11 @@ def main(argv=None):
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of the HIP graph")
     ap.add_argument("--precs", nargs="+", default=["fp32", "fp64"], choices=["fp32", "fp64"],
                     help="factor storage of the batched runs")
+    ap.add_argument("--ir-methods", nargs="+", type=int, default=[0], choices=[0, 1],
+                    help="0: GMRES-IR; 1: fused Richardson steps (one launch per step for the batch)")
     args = ap.parse_args(argv)
     out = compare(args.circuits, args.steps, args.warmup, args.plain_steps, args.nmin, args.nmax,
-                  args.seed, tuple(args.precs), not args.no_graph, args.check)
+                  args.seed, tuple(args.precs), not args.no_graph, args.check, tuple(args.ir_methods))
     out["metric"] = "batched Newton refactor+solve units/s at rel err <=1e-6 (batched vs plain fp64)"
     print(json.dumps(out), flush=True)
 

@@ -108,6 +108,23 @@ struct SellDesc {
     int n;
 };
 
+// Fused IR step (irstep.hip): one circuit's arrays.
+template <typename T>
+struct IrDesc {
+    SellDesc<double> A;     // A with fp64 values, for the residual
+    const int *lrp, *lci;   // strict L by rows (positions)
+    const int *urp, *uci;   // strict U by rows
+    const T *lrow, *urow, *diag;
+    const T *tail_F;        // dense trailing block, column-major (tail_T = 0: none)
+    const int *rowat;       // position -> original row (P)
+    const int *q;           // position -> original column (Q)
+    const int *tail_lcut;   // tail_T: end of the sparse part of L row tail_k0 + r
+    const int *frow, *flvp; // rows below tail_k0 by forward level; nlv_f + 1 level pointers
+    const int *brow, *blvp; // the same for the backward solve
+    long voff;              // first entry of the circuit in the vector arenas
+    int n, tail_k0, tail_T, nlv_f, nlv_b, pad_;
+};
+
 // Status words written by the persistent kernels.
 struct KStatus {
     int abort_code;         // kErr* (0 = none)

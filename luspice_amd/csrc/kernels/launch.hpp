@@ -73,6 +73,22 @@ struct KrylovArgs {
     double inner_tol, outer_tol;
 };
 
+// One fused IR step over the circuits of a batch (irstep.hip).
+struct IrStepArgs {
+    const void *descs;      // IrDesc<T>[nmat]
+    const double *b;        // right-hand sides (arena, original row order)
+    double *x;              // solutions (arena, original column order), updated in place
+    double *r;              // residual arena (original row order)
+    double *yw;             // work arena (position order) for circuits larger than lds_n
+    int *conv;              // per circuit: 1 = converged (the launch leaves it alone)
+    double *rn2, *bn2;      // per circuit: ||b - A x||^2 and ||b||^2 at its last step
+    double tol2;            // squared relative residual tolerance (< 0: every circuit steps)
+    int lds_n;              // doubles of LDS per workgroup for the work vector
+    int first;              // 1: x starts at 0 (r = b, x is overwritten rather than updated)
+};
+template <typename T> hipError_t launch_ir_step(const IrStepArgs &A, int nmat, hipStream_t st);
+template <typename T> hipError_t ir_step_set_lds(int lds_n);
+
 // factor.hip
 template <typename T> hipError_t launch_refactor(const FactorLaunch &L, int blocks, hipStream_t st);
 template <typename T> hipError_t refactor_set_lds(int cap);

@@ -180,6 +180,35 @@ void Batch::account_trisolve()
 }
 
 // ---------------------------------------------------------------------------
+// Rows below the dense block (positions < tail_k0) grouped by forward and by
+// backward solve level: a counting sort of the plan's row levels.  The level
+// count is taken from the levels themselves (the plan's nlv_f / nlv_b count
+// the supernodal solve's levels, which can be fewer).  Row lists are padded
+// to n entries so that they share the n-sized arena offsets.
+static int by_level(const std::vector<int> &lv, int k0, int n, std::vector<int> &row, std::vector<int> &ptr)
+{
+    if (k0 < 0 || k0 > n || (int)lv.size() < k0) return LSP_E_STRUCTURE;
+    int nlv = 0;
+    for (int i = 0; i < k0; i++) {
+        if (lv[i] < 0) return LSP_E_STRUCTURE;
+        nlv = std::max(nlv, lv[i] + 1);
+    }
+    ptr.assign(nlv + 1, 0);
+    for (int i = 0; i < k0; i++) ptr[lv[i] + 1]++;
+    for (int l = 0; l < nlv; l++) ptr[l + 1] += ptr[l];
+    row.assign(n, 0);
+    std::vector<int> next(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < k0; i++) row[next[lv[i]]++] = i;
+    return LSP_OK;
+}
+
+static int ir_level_lists(CircuitHost &C)
+{
+    const LspPlan &P = C.plan;
+    int rc = by_level(P.lvf, P.tail_k0, C.n, C.frow, C.flvp);
+    return rc ? rc : by_level(P.lvb, P.tail_k0, C.n, C.brow, C.blvp);
+}
+
 static int build_circuit(CircuitHost &C, const lsp_circuit_in &in, const lsp_options &opt)
 {
     C.n = in.n;
@@ -227,7 +256,7 @@ static int build_circuit(CircuitHost &C, const lsp_circuit_in &in, const lsp_opt
     lsp_coo_to_csr(C.n, C.nnz, C.row.data(), C.col.data(), rp, ci, rsrc);
     if (lsp_build_sell(C.n, rp.data(), ci.data(), rsrc.data(), 64, opt.sell_sigma, C.sell) != 0)
         return LSP_E_STRUCTURE;
-    return LSP_OK;
+    return ir_level_lists(C);
 }
 
 template <typename T>
@@ -243,6 +272,7 @@ int Batch::ensure_factor_set(FactorSet<T> &fs)
     const int nc = (int)circ_.size();
     std::vector<FactorDesc<T>> fd(nc);
     std::vector<TriDesc<T>> td(nc);
+    std::vector<IrDesc<T>> id(nc);
     for (int m = 0; m < nc; m++) {
         const CircuitHost &C = circ_[m];
         FactorDesc<T> &F = fd[m];
@@ -300,10 +330,32 @@ int Batch::ensure_factor_set(FactorSet<T> &fs)
         D.tail_ctr = d_tail_ctr_ + 4 * m;
         D.tail_k0 = C.plan.tail_k0;
         D.tail_T = C.plan.tail_T;
+        IrDesc<T> &I = id[m];
+        I.A.off = d_soff_ + C.o_slp1;
+        I.A.rlen = d_srlen_ + C.o_n;
+        I.A.col = d_scol_ + C.o_ss;
+        I.A.val = d_sval64_ + C.o_ss;
+        I.A.perm = d_sperm_ + C.o_n;
+        I.A.nslices = C.sell.nslices;
+        I.A.n = C.n;
+        I.lrp = D.lrp; I.lci = D.lci; I.urp = D.urp; I.uci = D.uci;
+        I.lrow = D.lrow; I.urow = D.urow; I.diag = D.diag; I.tail_F = D.tail_F;
+        I.rowat = D.rowat; I.q = D.q; I.tail_lcut = D.tail_lcut;
+        I.frow = d_frow_ + C.o_n;
+        I.flvp = d_flvp_ + C.o_lf;
+        I.brow = d_brow_ + C.o_n;
+        I.blvp = d_blvp_ + C.o_lb;
+        I.voff = C.voff;
+        I.n = C.n;
+        I.tail_k0 = C.plan.tail_k0;
+        I.tail_T = C.plan.tail_T;
+        I.nlv_f = (int)C.flvp.size() - 1;
+        I.nlv_b = (int)C.blvp.size() - 1;
     }
     fs.fdesc = upload(fd);
     fs.tdesc = upload(td);
-    if (!fs.fdesc || !fs.tdesc) return LSP_E_ALLOC;
+    fs.idesc = upload(id);
+    if (!fs.fdesc || !fs.tdesc || !fs.idesc) return LSP_E_ALLOC;
     fs.ready = true;
     return LSP_OK;
 }
@@ -516,6 +568,27 @@ int Batch::upload_all()
         d_ssrc_ = upload(ssrc);
         d_dpos_ = upload(dpos);
         if (!d_dpos_) return LSP_E_ALLOC;
+        // fused IR step level lists (built by the host phase)
+        {
+            std::vector<int> frow, flvp, brow, blvp;
+            long lf = 0, lb = 0;
+            for (auto &C : circ_) {
+                C.o_lf = lf;
+                C.o_lb = lb;
+                frow.insert(frow.end(), C.frow.begin(), C.frow.end());
+                brow.insert(brow.end(), C.brow.begin(), C.brow.end());
+                flvp.insert(flvp.end(), C.flvp.begin(), C.flvp.end());
+                blvp.insert(blvp.end(), C.blvp.begin(), C.blvp.end());
+                lf += (long)C.flvp.size();
+                lb += (long)C.blvp.size();
+            }
+            d_frow_ = upload(frow); d_flvp_ = upload(flvp); d_brow_ = upload(brow); d_blvp_ = upload(blvp);
+            std::vector<int> zc(circ_.size(), 0);
+            std::vector<double> zd(circ_.size(), 0.0);
+            d_irconv_ = upload(zc); d_rn2_ = upload(zd); d_bn2_ = upload(zd);
+            if (!d_frow_ || !d_flvp_ || !d_brow_ || !d_blvp_ || !d_irconv_ || !d_rn2_ || !d_bn2_)
+                return LSP_E_ALLOC;
+        }
         if (!d_cp_ || !d_relp_ || !d_soff_ || !d_dp_ || !d_pos_ || !d_aptr_ || !d_asrc_ ||
             !d_aslot_ || !d_dst_ || !d_lrp_ || !d_lci_ || !d_urp_ || !d_uci_ || !d_rowat_ ||
             !d_q_ || !d_rel_ || !d_snend_ || !d_ru_ || !d_fcut_ || !d_bcut_ || !d_fpk_ || !d_bpk_ || !d_srlen_ || !d_scol_ || !d_sperm_ || !d_ssrc_)
@@ -1046,6 +1119,10 @@ int Batch::repivot(const std::vector<int> &cands, int *changed)
         C.sym = std::move(fresh[i].sym);
         C.plan = std::move(fresh[i].plan);
         C.sell = std::move(fresh[i].sell);
+        C.frow = std::move(fresh[i].frow);
+        C.flvp = std::move(fresh[i].flvp);
+        C.brow = std::move(fresh[i].brow);
+        C.blvp = std::move(fresh[i].blvp);
         (*changed)++;
     }
     if (*changed == 0) return LSP_OK;
@@ -1269,7 +1346,7 @@ int Batch::spmv(const double *x, double *y, bool on_device, int prec)
 
 // ---------------------------------------------------------------------------
 int Batch::gmres_ir(const double *d_bv, double *d_xv, int prec, const std::vector<char> *subset,
-                    lsp_solve_report *rep)
+                    lsp_solve_report *rep, bool keep_x)
 {
     const KrylovArgs K = kargs();
     const int nc = (int)circ_.size();
@@ -1279,6 +1356,7 @@ int Batch::gmres_ir(const double *d_bv, double *d_xv, int prec, const std::vecto
 #define LSP_TRY(expr) do { rc = (expr); if (rc) return rc; } while (0)
 #define LSP_HIP(expr, what) LSP_TRY(check((expr), what))
     // conv = 1 for circuits outside the subset, 0 inside; x = 0 inside
+    // unless keep_x (x then holds a first iterate, e.g. after fused steps)
     {
         std::vector<int> cv(nc, 0);
         if (subset) for (int m = 0; m < nc; m++) cv[m] = (*subset)[m] ? 0 : 1;
@@ -1288,7 +1366,7 @@ int Batch::gmres_ir(const double *d_bv, double *d_xv, int prec, const std::vecto
     LSP_HIP(k_activate(K, s), "activate");
     LSP_HIP(k_chunk_dots(K, d_bv, d_bv, 1, s), "dots");
     LSP_HIP(k_reduce(K, 1, 2, 0, d_nb_, s), "reduce");
-    LSP_HIP(k_chunk_fill(K, d_xv, 0.0, s), "zero x");
+    if (!keep_x) LSP_HIP(k_chunk_fill(K, d_xv, 0.0, s), "zero x");
     int outer_used = 0, inner_total = 0;
     res_hist0_.clear();
     for (int outer = 0; outer < opt_.ir_outer; outer++) {
@@ -1366,6 +1444,134 @@ int Batch::gmres_ir(const double *d_bv, double *d_xv, int prec, const std::vecto
     return LSP_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Fused IR steps (irstep.hip)
+template <typename T>
+int Batch::ir_launch(FactorSet<T> &fs, const double *d_b, double *d_x, double tol2, bool first)
+{
+    if (!fs.idesc) return LSP_E_ARG;
+    int rc = ir_prepare();
+    if (rc) return rc;
+    IrStepArgs A{};
+    A.descs = fs.idesc;
+    A.b = d_b;
+    A.x = d_x;
+    A.r = d_r_;
+    A.yw = d_ty_;
+    A.conv = d_irconv_;
+    A.rn2 = d_rn2_;
+    A.bn2 = d_bn2_;
+    A.tol2 = tol2;
+    A.lds_n = ir_lds_n_;
+    A.first = first ? 1 : 0;
+    return check(launch_ir_step<T>(A, (int)circ_.size(), stream_), "ir step");
+}
+
+// The fused step's LDS work vector: up to 8192 doubles (64 KB, two workgroups
+// per CU); larger circuits use the work arena.  Set once, before any capture.
+int Batch::ir_prepare()
+{
+    if (ir_lds_n_) return LSP_OK;
+    int maxn = 0;
+    for (const auto &C : circ_) maxn = std::max(maxn, C.n);
+    const int lds_n = std::min(8192, std::max(64, maxn));
+    int rc = check(ir_step_set_lds<float>(lds_n), "ir lds attr");
+    if (!rc) rc = check(ir_step_set_lds<double>(lds_n), "ir lds attr");
+    if (!rc) ir_lds_n_ = lds_n;
+    return rc;
+}
+
+bool Batch::use_fused(int prec) const
+{
+    return opt_.ir_method == 1 && prec != LSP_PREC_FP16;
+}
+
+int Batch::fused_steps() const
+{
+    return opt_.ir_fused_steps > 0 ? opt_.ir_fused_steps : 12;
+}
+
+int Batch::ir_launch_prec(int prec, const double *d_b, double *d_x, double tol2, bool first)
+{
+    if (prec == LSP_PREC_FP32) return ir_launch(f32_, d_b, d_x, tol2, first);
+    if (prec == LSP_PREC_FP64) return ir_launch(f64_, d_b, d_x, tol2, first);
+    return LSP_E_ARG;                        // FP16 factors are solved with their Ruiz scales by GMRES-IR
+}
+
+int Batch::fused_ir(const double *d_bv, double *d_xv, int prec, const std::vector<char> *subset, int max_steps,
+                    std::vector<int> &conv, lsp_solve_report *rep)
+{
+    const int nc = (int)circ_.size();
+    conv.assign(nc, 0);
+    std::vector<int> skip(nc, 0);
+    for (int m = 0; m < nc; m++) skip[m] = subset && !(*subset)[m] ? 1 : 0;
+    int rc = check(hipMemcpyAsync(d_irconv_, skip.data(), nc * 4, hipMemcpyHostToDevice, stream_), "conv");
+    if (rc) return rc;
+    const double tol2 = opt_.ir_outer_tol * opt_.ir_outer_tol;
+    std::vector<int> cv(nc);
+    int steps = 0, done = 0;
+    const int want = nc - (int)std::count(skip.begin(), skip.end(), 1);
+    while (steps < max_steps) {
+        rc = ir_launch_prec(prec, d_bv, d_xv, tol2, steps == 0);      // the first step starts from x = 0
+        if (rc) return rc;
+        steps++;
+        rc = check(hipMemcpyAsync(cv.data(), d_irconv_, nc * 4, hipMemcpyDeviceToHost, stream_), "conv");
+        if (!rc) rc = check(hipStreamSynchronize(stream_), "sync");
+        if (rc) return rc;
+        done = 0;
+        for (int m = 0; m < nc; m++) done += !skip[m] && cv[m];
+        if (done == want) break;
+    }
+    std::vector<double> rn(nc), bn(nc);
+    rc = check(hipMemcpyAsync(rn.data(), d_rn2_, nc * 8, hipMemcpyDeviceToHost, stream_), "rn");
+    if (!rc) rc = check(hipMemcpyAsync(bn.data(), d_bn2_, nc * 8, hipMemcpyDeviceToHost, stream_), "bn");
+    if (!rc) rc = check(hipStreamSynchronize(stream_), "sync");
+    if (rc) return rc;
+    lsp_solve_report r{};
+    r.outer = 1;
+    r.inner_total = steps;
+    for (int m = 0; m < nc; m++) {
+        if (skip[m]) continue;
+        conv[m] = cv[m];
+        r.converged += cv[m];
+        double rel = bn[m] > 0 ? std::sqrt(rn[m] / bn[m]) : std::sqrt(rn[m]);
+        if (!(rel == rel)) rel = INFINITY;
+        r.max_rel_res = std::max(r.max_rel_res, rel);
+    }
+    r.status = LSP_OK;
+    if (rep) *rep = r;
+    return LSP_OK;
+}
+
+int Batch::ir_step(const double *b, double *x, bool on_device, double *rel_res)
+{
+    if (cur_prec_ < 0 || circ_.empty()) return LSP_E_ARG;
+    const int nc = (int)circ_.size();
+    const double *db = b;
+    double *dx = x;
+    int rc;
+    if (!on_device) {
+        rc = check(hipMemcpyAsync(d_b_, b, ntot_ * 8, hipMemcpyHostToDevice, stream_), "b");
+        if (!rc) rc = check(hipMemcpyAsync(d_x_, x, ntot_ * 8, hipMemcpyHostToDevice, stream_), "x");
+        if (rc) return rc;
+        db = d_b_;
+        dx = d_x_;
+    }
+    std::vector<int> z(nc, 0);
+    rc = check(hipMemcpyAsync(d_irconv_, z.data(), nc * 4, hipMemcpyHostToDevice, stream_), "conv");
+    if (!rc) rc = ir_launch_prec(cur_prec_, db, dx, -1.0, false);   // no tolerance: every circuit steps
+    if (rc) return rc;
+    std::vector<double> rn(nc), bn(nc);
+    rc = check(hipMemcpyAsync(rn.data(), d_rn2_, nc * 8, hipMemcpyDeviceToHost, stream_), "rn");
+    if (!rc) rc = check(hipMemcpyAsync(bn.data(), d_bn2_, nc * 8, hipMemcpyDeviceToHost, stream_), "bn");
+    if (!rc && !on_device) rc = check(hipMemcpyAsync(x, d_x_, ntot_ * 8, hipMemcpyDeviceToHost, stream_), "x");
+    if (!rc) rc = check(hipStreamSynchronize(stream_), "sync");
+    if (rc) return rc;
+    if (rel_res)
+        for (int m = 0; m < nc; m++) rel_res[m] = bn[m] > 0 ? std::sqrt(rn[m] / bn[m]) : std::sqrt(rn[m]);
+    return LSP_OK;
+}
+
 int Batch::solve(const double *b, double *x, bool on_device, lsp_solve_report *rep)
 {
     if (cur_prec_ < 0) return LSP_E_ARG;
@@ -1389,8 +1595,49 @@ int Batch::solve(const double *b, double *x, bool on_device, lsp_solve_report *r
         nlive = (int)std::count(active_.begin(), active_.end(), 1);
     }
     lsp_solve_report r{};
-    rc = gmres_ir(db, dx, cur_prec_, live.empty() ? nullptr : &live, &r);
-    if (rc) return rc;
+    if (use_fused(cur_prec_)) {
+        // fused Richardson steps first; the circuits they leave are solved by GMRES-IR
+        std::vector<int> conv;
+        rc = fused_ir(db, dx, cur_prec_, live.empty() ? nullptr : &live, fused_steps(), conv, &r);
+        if (rc) return rc;
+        if (r.converged < nlive) {
+            std::vector<char> rest(nc, 0);
+            for (int m = 0; m < nc; m++) rest[m] = (live.empty() || live[m]) && !conv[m];
+            lsp_solve_report rg{};
+            rc = gmres_ir(db, dx, cur_prec_, &rest, &rg, true);
+            if (rc) return rc;
+            r.converged += rg.converged;
+            r.outer += rg.outer;
+            r.inner_total += rg.inner_total;
+            // the residual norms of every circuit, as gmres_ir's final check left them
+            std::vector<double> beta(nc), nb(nc);
+            (void)hipMemcpy(beta.data(), d_beta_, nc * 8, hipMemcpyDeviceToHost);
+            (void)hipMemcpy(nb.data(), d_nb_, nc * 8, hipMemcpyDeviceToHost);
+            std::vector<double> rn(nc), bn(nc);
+            (void)hipMemcpy(rn.data(), d_rn2_, nc * 8, hipMemcpyDeviceToHost);
+            (void)hipMemcpy(bn.data(), d_bn2_, nc * 8, hipMemcpyDeviceToHost);
+            r.max_rel_res = 0.0;
+            for (int m = 0; m < nc; m++) {
+                if (!(live.empty() || live[m])) continue;
+                double rel = rest[m] ? (nb[m] > 0 ? beta[m] / nb[m] : beta[m])
+                                     : (bn[m] > 0 ? std::sqrt(rn[m] / bn[m]) : std::sqrt(rn[m]));
+                if (!(rel == rel)) rel = INFINITY;
+                r.max_rel_res = std::max(r.max_rel_res, rel);
+            }
+            // the fp64 fallback below reads the final residuals from d_beta_ /
+            // d_nb_: give the fused circuits theirs
+            for (int m = 0; m < nc; m++)
+                if (!rest[m]) {
+                    beta[m] = std::sqrt(rn[m]);
+                    nb[m] = std::sqrt(bn[m]);
+                }
+            (void)hipMemcpy(d_beta_, beta.data(), nc * 8, hipMemcpyHostToDevice);
+            (void)hipMemcpy(d_nb_, nb.data(), nc * 8, hipMemcpyHostToDevice);
+        }
+    } else {
+        rc = gmres_ir(db, dx, cur_prec_, live.empty() ? nullptr : &live, &r);
+        if (rc) return rc;
+    }
     // per-circuit fallback to fp64 factors when IR stalls
     if (opt_.fallback_fp64 && cur_prec_ != LSP_PREC_FP64 && r.converged < nlive) {
         // identify stalled circuits from the final residuals
@@ -1540,6 +1787,14 @@ int Batch::record_step(int prec)
 #define LSP_HIP(expr, what) LSP_TRY(check((expr), what))
     LSP_TRY(launch_refactor_prec(prec, nullptr));
     LSP_HIP(k_zero_int(d_abort_ + 1, 1, s), "abort reset");
+    if (use_fused(prec)) {
+        // fused Richardson steps: a converged circuit's later launches return
+        // at once, so the recorded count only bounds the work
+        LSP_HIP(k_zero_int(d_irconv_, nc, s), "conv");
+        const double tol2 = opt_.ir_outer_tol * opt_.ir_outer_tol;
+        for (int j = 0; j < fused_steps(); j++) LSP_TRY(ir_launch_prec(prec, d_bv, d_xv, tol2, j == 0));
+        return LSP_OK;
+    }
     LSP_HIP(k_zero_int(d_conv_, nc, s), "conv");
     LSP_HIP(k_activate(K, s), "activate");
     LSP_HIP(k_chunk_dots(K, d_bv, d_bv, 1, s), "dots");
@@ -1595,6 +1850,10 @@ int Batch::graph_step(int prec, lsp_refactor_report *rrep, lsp_solve_report *sre
         else if (prec == LSP_PREC_FP16) rc = ensure_factor_set(f16_);
         else return LSP_E_ARG;
         if (rc) return rc;
+        if (use_fused(prec)) {
+            rc = ir_prepare();
+            if (rc) return rc;
+        }
         if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
         if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
         rc = check(hipStreamSynchronize(stream_), "sync");
@@ -1619,13 +1878,21 @@ int Batch::graph_step(int prec, lsp_refactor_report *rrep, lsp_solve_report *sre
     if (rc) return rc;
     cur_prec_ = prec;
     const int nc = (int)circ_.size();
-    std::vector<int> ms(kMstat * nc), jf(nc);
+    const bool fused = use_fused(prec);
+    std::vector<int> ms(kMstat * nc), jf(nc), cv(nc, 0);
     std::vector<double> beta(nc), nb(nc);
     int ab[2] = {0, 0};
     rc = check(hipMemcpyAsync(ms.data(), d_mstat_, ms.size() * 4, hipMemcpyDeviceToHost, stream_), "mstat");
-    if (!rc) rc = check(hipMemcpyAsync(beta.data(), d_beta_, nc * 8, hipMemcpyDeviceToHost, stream_), "beta");
-    if (!rc) rc = check(hipMemcpyAsync(nb.data(), d_nb_, nc * 8, hipMemcpyDeviceToHost, stream_), "nb");
-    if (!rc) rc = check(hipMemcpyAsync(jf.data(), d_jfin_, nc * 4, hipMemcpyDeviceToHost, stream_), "jfin");
+    if (fused) {
+        // squared residual norms of the last step each circuit ran
+        if (!rc) rc = check(hipMemcpyAsync(beta.data(), d_rn2_, nc * 8, hipMemcpyDeviceToHost, stream_), "rn2");
+        if (!rc) rc = check(hipMemcpyAsync(nb.data(), d_bn2_, nc * 8, hipMemcpyDeviceToHost, stream_), "bn2");
+        if (!rc) rc = check(hipMemcpyAsync(cv.data(), d_irconv_, nc * 4, hipMemcpyDeviceToHost, stream_), "conv");
+    } else {
+        if (!rc) rc = check(hipMemcpyAsync(beta.data(), d_beta_, nc * 8, hipMemcpyDeviceToHost, stream_), "beta");
+        if (!rc) rc = check(hipMemcpyAsync(nb.data(), d_nb_, nc * 8, hipMemcpyDeviceToHost, stream_), "nb");
+        if (!rc) rc = check(hipMemcpyAsync(jf.data(), d_jfin_, nc * 4, hipMemcpyDeviceToHost, stream_), "jfin");
+    }
     if (!rc) rc = check(hipMemcpyAsync(h_pin_, d_abort_, 8, hipMemcpyDeviceToHost, stream_), "abort");
     if (!rc) rc = check(hipStreamSynchronize(stream_), "graph sync");
     if (rc) return rc;
@@ -1674,13 +1941,17 @@ int Batch::graph_step(int prec, lsp_refactor_report *rrep, lsp_solve_report *sre
     sr.outer = 1;
     int conv = 0, jmax = 0;
     for (int m = 0; m < nc; m++) {
+        if (fused) {
+            beta[m] = std::sqrt(beta[m]);
+            nb[m] = std::sqrt(nb[m]);
+        }
         double rel = nb[m] > 0 ? beta[m] / nb[m] : beta[m];
         if (!(rel == rel)) rel = INFINITY;
         sr.max_rel_res = std::max(sr.max_rel_res, rel);
-        if (rel <= opt_.ir_outer_tol * 10 || rel <= 1e-9) conv++;
+        if (fused ? cv[m] != 0 : (rel <= opt_.ir_outer_tol * 10 || rel <= 1e-9)) conv++;
         jmax = std::max(jmax, jf[m]);
     }
-    sr.inner_total = jmax;
+    sr.inner_total = fused ? fused_steps() : jmax;
     sr.converged = conv;
     sr.status = LSP_OK;
     if (conv < nc || r.status != LSP_OK) {

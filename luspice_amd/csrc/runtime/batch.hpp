@@ -28,6 +28,9 @@ struct CircuitHost {
     LspGpLu sym;                         // CPU symbolic factors (fp64)
     LspPlan plan;
     LspSell sell;
+    // fused IR step: rows below the dense block grouped by forward / backward
+    // level (row lists padded to n entries; nlv + 1 level pointers each)
+    std::vector<int> frow, flvp, brow, blvp;
     long voff = 0, aoff = 0;             // vector / value arena offsets
     // offsets into the concatenated device arrays
     long o_np1 = 0;                      // n+1 sized arrays
@@ -39,6 +42,7 @@ struct CircuitHost {
     long o_sl = 0, o_slp1 = 0, o_ss = 0; // SELL slices, slices+1, slots
     long o_t = 0, o_t2 = 0, o_tb = 0;    // dense block: T, T^2, 64-row blocks
     long o_fpk = 0, o_bpk = 0;           // trisolve packets forward / backward
+    long o_lf = 0, o_lb = 0;             // fused IR step: level pointers forward / backward
 };
 
 template <typename T>
@@ -47,6 +51,7 @@ struct FactorSet {                       // per storage precision
     T *tailF = nullptr;                  // dense trailing blocks, storage precision
     FactorDesc<T> *fdesc = nullptr;      // device
     TriDesc<T> *tdesc = nullptr;         // device
+    IrDesc<T> *idesc = nullptr;          // device (fused IR step)
     bool ready = false;                  // arrays allocated
 };
 
@@ -121,6 +126,11 @@ struct DeviceArrays {
            *d_hn_ = nullptr, *d_hcol_ = nullptr;
     int *d_act_ = nullptr, *d_jfin_ = nullptr, *d_conv_ = nullptr, *d_cycled_ = nullptr,
         *d_nact_ = nullptr, *d_subset_ = nullptr;
+    // fused IR step: rows below the dense block by level, and per-circuit state
+    int *d_frow_ = nullptr, *d_flvp_ = nullptr, *d_brow_ = nullptr, *d_blvp_ = nullptr;
+    int *d_irconv_ = nullptr;
+    double *d_rn2_ = nullptr, *d_bn2_ = nullptr;
+    int ir_lds_n_ = 0;                   // doubles of LDS for the work vector (0: not set up)
 };
 
 class Batch : private DeviceArrays {
@@ -171,7 +181,15 @@ public:
 
     // device-resident pieces used by the serving loop and the bench
     int gmres_ir(const double *d_b, double *d_x, int prec, const std::vector<char> *subset,
-                 lsp_solve_report *rep);
+                 lsp_solve_report *rep, bool keep_x = false);
+    // Fused Richardson steps x += M^-1 (b - A x) from x = 0, one launch per step
+    // for every unconverged circuit, up to max_steps; conv (host) gets 1 for
+    // each circuit that met ir_outer_tol.
+    int fused_ir(const double *d_b, double *d_x, int prec, const std::vector<char> *subset, int max_steps,
+                 std::vector<int> &conv, lsp_solve_report *rep);
+    // One fused step on the given vectors for every circuit (no convergence
+    // skip); rel_res gets ||b - A x|| / ||b|| before the update, per circuit.
+    int ir_step(const double *b, double *x, bool on_device, double *rel_res);
     int launch_trisolve_prec(int prec, const double *d_b, double *d_x, const int *active,
                              const int *d_nact = nullptr);
     int launch_spmv_krylov(const double *d_xv, double *d_y, const int *active);
@@ -182,6 +200,11 @@ public:
 
 private:
     template <typename T> int ensure_factor_set(FactorSet<T> &fs);
+    template <typename T> int ir_launch(FactorSet<T> &fs, const double *d_b, double *d_x, double tol2, bool first);
+    int ir_launch_prec(int prec, const double *d_b, double *d_x, double tol2, bool first);
+    int ir_prepare();
+    bool use_fused(int prec) const;      // ir_method 1 and factors the fused step reads
+    int fused_steps() const;
     int refactor_pass(int prec, const std::vector<int> *subset, std::vector<int> &ms);
     int repivot(const std::vector<int> &cands, int *changed);
     void release_device();

@@ -136,6 +136,26 @@ int lsp_batch_trisolve(lsp_batch *b, const double *rhs, double *x, int on_device
     return b ? b->impl.trisolve(rhs, x, on_device != 0) : LSP_E_ARG;
 }
 
+int lsp_batch_pivot_range(lsp_batch *b, int circuit, double *min_abs, double *max_abs)
+{
+    return b ? b->impl.diag_stats(circuit, min_abs, max_abs) : LSP_E_ARG;
+}
+
+int lsp_ruiz(int n, long nnz, const int *row, const int *col, const double *val, int sweeps,
+             double *R, double *C)
+{
+    if (n <= 0 || nnz < 0 || sweeps < 1 || !row || !col || !val || !R || !C) return LSP_E_ARG;
+    for (long e = 0; e < nnz; e++)
+        if (row[e] < 0 || row[e] >= n || col[e] < 0 || col[e] >= n) return LSP_E_ARG;
+    lsp_ruiz_scales(n, nnz, row, col, val, sweeps, R, C);
+    return LSP_OK;
+}
+
+int lsp_batch_ir_step(lsp_batch *b, const double *rhs, double *x, int on_device, double *rel_res)
+{
+    return b && rhs && x ? b->impl.ir_step(rhs, x, on_device != 0, rel_res) : LSP_E_ARG;
+}
+
 int lsp_batch_solve(lsp_batch *b, const double *rhs, double *x, int on_device,
                     lsp_solve_report *rep)
 {

@@ -22,6 +22,19 @@ Every attempt reuses nothing from the previous one except the circuit; the
 IR parameters come from the selector's configuration (src/selector.c:124-126:
 3 restarts, 50 inner iterations, 1e-9), and the in-batch fp64 fallback is
 off, because the chain itself is the fallback.
+
+Two guards of the reference's precision budget sit on the chain:
+
+* pivot guard (docs/02_precision_budget.md:82-83): after the P-fast refactor,
+  min |u_kk| < 1e-9 max |A| marks the fp32 factors unreliable; the solve is
+  skipped and the chain moves on to fp64 factors;
+* Ruiz rule (docs/03_gmres_ir.md:60-67): Ruiz is used only when it reduces the
+  matrix's scale spread by more than 1e2 (row/column imbalance); below that
+  (an intrinsically near-singular matrix) P-safe and P-hard pivot on the
+  unscaled matrix.  The reduction is measured as the ratio of the entries'
+  dynamic range max |a| / min |a| before and after 8 Ruiz sweeps: a diagonal
+  scaling D1 A D2 that Ruiz undoes widens both the dynamic range and the
+  condition number by the same factors, and the range needs no factorisation.
 """
 from __future__ import annotations
 
@@ -32,6 +45,8 @@ import numpy as np
 from . import _native as N
 
 P_FAST, P_SAFE, P_HARD, P_SING = 0, 1, 2, 3
+PIVOT_GUARD = 1e-9          # min |pivot| < PIVOT_GUARD * max |A|: fp32 factors unreliable
+RUIZ_MIN_REDUCTION = 1e2    # Ruiz only when it shrinks the scale spread more than this
 
 
 class MatrixInfo(C.Structure):          # selector.h dsh_matrix_info
@@ -93,17 +108,40 @@ def select(circuit, singular_flag: int = 0) -> DshConfig:
     return cfg
 
 
-def path_options(p: int, cfg: DshConfig, shift: float | None = None) -> tuple[str, dict]:
-    """(factor precision, batch options) of one path of the chain."""
+def _dyn_range(v: np.ndarray) -> float:
+    a = np.abs(v)
+    a = a[a > 0]
+    return float(a.max() / a.min()) if a.size else 1.0
+
+
+def ruiz_reduction(circuit, sweeps: int = 8) -> float:
+    """How much Ruiz equilibration shrinks the scale spread of A: dynamic
+    range max |a| / min |a| of A over that of R A C (8 sweeps, the
+    reference's own sweep count in results/raw/fp16/cpu_fp16_analysis.py)."""
+    row = np.ascontiguousarray(circuit.row, np.int32)
+    col = np.ascontiguousarray(circuit.col, np.int32)
+    val = np.ascontiguousarray(circuit.val, np.float64)
+    R = np.empty(circuit.n)
+    Cs = np.empty(circuit.n)
+    N.check(N.lib.lsp_ruiz(circuit.n, int(circuit.nnz), N.ptr(row), N.ptr(col), N.ptr(val), int(sweeps),
+                           N.ptr(R), N.ptr(Cs)), "lsp_ruiz")
+    return _dyn_range(val) / _dyn_range(val * R[row] * Cs[col])
+
+
+def path_options(p: int, cfg: DshConfig, shift: float | None = None,
+                 use_ruiz: bool = True) -> tuple[str, dict]:
+    """(factor precision, batch options) of one path of the chain; use_ruiz
+    is the Ruiz rule's verdict (pivot order of P-safe / P-hard on R A C)."""
     ir = dict(ir_outer=cfg.ir_outer, ir_inner=cfg.ir_inner, ir_inner_tol=cfg.ir_rtol,
               ir_outer_tol=cfg.ir_rtol, fallback_fp64=0)
     s = cfg.shift if shift is None else shift
+    rz = 1 if use_ruiz else 0
     if p == P_FAST:
         return "fp32", ir
     if p == P_SAFE:
-        return "fp64", dict(ir, pivot_scaled=1, factor_shift=s if cfg.use_shift else 0.0)
+        return "fp64", dict(ir, pivot_scaled=rz, factor_shift=s if cfg.use_shift else 0.0)
     if p == P_HARD:
-        return "fp64", dict(ir, pivot_scaled=1, ir_outer=3 * cfg.ir_outer, ir_inner=2 * cfg.ir_inner)
+        return "fp64", dict(ir, pivot_scaled=rz, ir_outer=3 * cfg.ir_outer, ir_inner=2 * cfg.ir_inner)
     raise ValueError(f"no solve configuration for {path_name(p)}")
 
 
@@ -120,17 +158,28 @@ def solve_adaptive(circuit, rhs=None, tol: float = 1e-6, shift: float | None = N
     b = circuit.rhs if rhs is None else np.asarray(rhs, np.float64)
     cfg = select(circuit)
     p = cfg.mp_path if start is None else int(start)
+    reduction = ruiz_reduction(circuit)
+    use_ruiz = reduction > RUIZ_MIN_REDUCTION
+    amax = float(np.abs(circuit.val).max()) if circuit.nnz else 0.0
     attempts = []
     while p != P_SING:
-        prec, opts = path_options(p, cfg, shift)
+        prec, opts = path_options(p, cfg, shift, use_ruiz)
         opts.update(extra)
         rec = dict(path=path_name(p), prec=prec, factor_shift=opts.get("factor_shift", 0.0),
-                   pivot_scaled=opts.get("pivot_scaled", 0))
+                   pivot_scaled=opts.get("pivot_scaled", 0), ruiz_reduction=reduction)
         B = None
         try:
             B = batch_factory(circuit, **opts)
             rr = B.refactor(prec)
             rec.update(refactor_rc=rr["rc"], bad_pivots=rr.get("bad_pivots", 0))
+            if rr["rc"] == 0 and prec == "fp32":
+                lo, _ = B.pivot_range(0)
+                rec["min_pivot"] = lo
+                if not lo >= PIVOT_GUARD * amax:          # NaN fails it too
+                    rec["pivot_guard"] = True
+                    attempts.append(rec)
+                    p = escalate(p)
+                    continue
             if rr["rc"] == 0:
                 x, sr = B.solve(b)
                 rec.update(solve_rc=sr["rc"], rel_res=float(sr["max_rel_res"]),

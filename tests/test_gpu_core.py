@@ -214,3 +214,18 @@ def test_fp16_refused_without_opt_in(c0, c0_q):
     assert rep["rc"] == -1, rep                 # LSP_E_ARG
     assert B.refactor("fp32")["rc"] == 0
     B.close()
+
+
+def test_pivot_range_is_the_range_of_u_diagonal(c0, c0_q):
+    """lsp_batch_pivot_range (the pivot guard's input) against the diagonal of
+    the exported U (its last entry per column), including the dense block's
+    columns."""
+    print("case: c0 pivot range", flush=True)
+    B = L.Batch(L.Circuit(c0.n, c0.row, c0.col, c0.val, c0.rhs, q=c0_q))
+    assert B.info(0)["tail_T"] > 0
+    for prec, tol in (("fp64", 1e-12), ("fp32", 1e-6)):
+        assert B.refactor(prec)["rc"] == 0
+        F = B.export_factors(0)
+        d = np.abs(F["Ux"][F["Up"][1:] - 1])
+        lo, hi = B.pivot_range(0)
+        np.testing.assert_allclose([lo, hi], [d.min(), d.max()], rtol=tol, err_msg=prec)

@@ -45,7 +45,7 @@ class _Stub:
         self.script = script
 
     def __call__(self, circuit, **opts):
-        key = ("safe" if opts.get("factor_shift") else "hard") if opts.get("pivot_scaled") else "fast"
+        key = "hard" if opts["ir_outer"] > 3 else "safe" if opts.get("factor_shift") else "fast"
         _Stub.built.append((key, opts))
         s = self.script[key]
         if s == "singular":
@@ -56,6 +56,10 @@ class _Stub:
 class _StubBatch:
     def __init__(self, n, s):
         self.n, self.s = n, s
+        self.solved = False
+
+    def pivot_range(self, i=0):
+        return (1e-30, 1.0) if self.s == "tiny-pivot" else (0.5, 2.0)
 
     def refactor(self, prec):
         return {"rc": 0 if self.s != "refactor-fail" else 3, "bad_pivots": 0}
@@ -113,3 +117,60 @@ def test_chain_on_device():
         print(rep["path"], rep["attempts"], flush=True)
         r = O.residual(c2.n, c2.row, c2.col, c2.val, c2.rhs, x)
         assert r <= 1e-6
+
+
+def test_ruiz_reduction_closed_forms():
+    """A matrix whose entries all have |a| = 1 has nothing to equilibrate
+    (reduction 1); a diagonal matrix is equilibrated by the sweeps to within
+    |d|^(1/2^16) of 1, so the reduction is its dynamic range; a random
+    two-sided scaling of c0 is undone far beyond the 1e2 threshold, while c0
+    itself and a c2 circuit (intrinsic spread) stay below it."""
+    rng = np.random.default_rng(1)
+    n = 200
+    row = np.r_[np.arange(n), np.arange(n - 1), np.arange(1, n)].astype(np.int32)
+    col = np.r_[np.arange(n), np.arange(1, n), np.arange(n - 1)].astype(np.int32)
+    v = np.where(rng.random(row.size) < 0.5, -1.0, 1.0)
+    assert P.ruiz_reduction(L.Circuit(n, row, col, v, np.ones(n))) == 1.0
+    d = 10.0 ** rng.uniform(-4, 4, n)
+    dr = d.max() / d.min()
+    red = P.ruiz_reduction(L.Circuit(n, np.arange(n, dtype=np.int32), np.arange(n, dtype=np.int32), d,
+                                     np.ones(n)))
+    assert abs(red / dr - 1.0) <= 1e-3
+    c = L.mna("c0", seed=0)
+    assert P.ruiz_reduction(c) < P.RUIZ_MIN_REDUCTION
+    assert P.ruiz_reduction(L.mna("c2", n=3000, seed=2)) < P.RUIZ_MIN_REDUCTION
+    d1, d2 = 10.0 ** rng.uniform(-3, 3, c.n), 10.0 ** rng.uniform(-2, 2, c.n)
+    cs = L.Circuit(c.n, c.row, c.col, c.val * d1[c.row] * d2[c.col], c.rhs)
+    assert P.ruiz_reduction(cs) > 1e6
+
+
+def test_ruiz_rule_sets_the_pivot_order_of_the_fp64_paths():
+    cfg = P.select(L.mna("c0", seed=0))
+    for p in (P.P_SAFE, P.P_HARD):
+        assert P.path_options(p, cfg, use_ruiz=True)[1]["pivot_scaled"] == 1
+        assert P.path_options(p, cfg, use_ruiz=False)[1]["pivot_scaled"] == 0
+    _Stub.built = []
+    c = L.mna("c0", seed=0)                       # reduction < 1e2: no Ruiz
+    x, rep = P.solve_adaptive(c, batch_factory=_Stub({"fast": 1e-3, "safe": 1e-11, "hard": 1e-11}))
+    assert rep["path"] == "P-safe" and _Stub.built[1][1]["pivot_scaled"] == 0
+    rng = np.random.default_rng(4)
+    d1 = 10.0 ** rng.uniform(-3, 3, c.n)
+    cs = L.Circuit(c.n, c.row, c.col, c.val * d1[c.row], c.rhs)
+    _Stub.built = []
+    x, rep = P.solve_adaptive(cs, batch_factory=_Stub({"fast": 1e-3, "safe": 1e-11, "hard": 1e-11}))
+    assert rep["path"] == "P-safe" and _Stub.built[1][1]["pivot_scaled"] == 1
+    assert rep["attempts"][0]["ruiz_reduction"] > P.RUIZ_MIN_REDUCTION
+
+
+def test_pivot_guard_skips_the_fp32_solve():
+    """min |pivot| < 1e-9 max |A| after the P-fast refactor: the fp32 factors
+    are not used for a solve, the chain goes straight to fp64 factors."""
+    _Stub.built = []
+    c = L.mna("c0", seed=0)
+    x, rep = P.solve_adaptive(c, batch_factory=_Stub({"fast": "tiny-pivot", "safe": 1e-11, "hard": 1e-11}))
+    a = rep["attempts"]
+    assert rep["path"] == "P-safe" and len(a) == 2
+    assert a[0]["pivot_guard"] and a[0]["min_pivot"] == 1e-30 and "rel_res" not in a[0]
+    _Stub.built = []
+    x, rep = P.solve_adaptive(c, batch_factory=_Stub({"fast": 1e-11, "safe": 1e-11, "hard": 1e-11}))
+    assert rep["path"] == "P-fast" and "pivot_guard" not in rep["attempts"][0]
