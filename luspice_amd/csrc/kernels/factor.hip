@@ -238,9 +238,17 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
                 const int tri = jl - __builtin_amdgcn_readlane(j, tb - 1);  // run rows after this part
                 const int nlast = __builtin_amdgcn_readlane(nl, ta);
                 if (bw > 1) {
-                    if (bw <= 9) xc = run_triangle<8>(M.val, xc, lb, ta, tb);
-                    else if (bw <= 25) xc = run_triangle<24>(M.val, xc, lb, ta, tb);
-                    else xc = run_triangle<kTri>(M.val, xc, lb, ta, tb);
+                    if (bw <= 9) {
+                        xc = run_triangle<8>(M.val, xc, lb, ta, tb);
+                    } else if constexpr (sizeof(T) == 8) {
+                        // fp64 values: 16 triangle columns in registers at a
+                        // time (the same steps in the same order), which keeps
+                        // the kernel at 3 waves per SIMD instead of 2
+                        for (int t0 = ta; t0 < tb - 1; t0 += 16) xc = run_triangle<16>(M.val, xc, lb, t0, tb);
+                    } else {
+                        if (bw <= 25) xc = run_triangle<24>(M.val, xc, lb, ta, tb);
+                        else xc = run_triangle<kTri>(M.val, xc, lb, ta, tb);
+                    }
                 }
                 if (lane >= ta && lane < tb) x[s0 + lane] = xc;
                 // rows below this part of the run: the run's later rows, then
@@ -249,17 +257,20 @@ __device__ void factor_column(const FactorDesc<T> &M, int k, XP x,
                 // column that ends a critical chain step, covers more rows
                 // per round trip)
                 const int R = tri + nlast;
+                // (fp64 values: half the columns in flight per pass, for the
+                // register budget of 3 waves per SIMD)
+                constexpr int kB8 = sizeof(T) == 8 ? 4 : 8;
                 if (bw >= 8) {
-                    if (R > 2 * kWave) run_gemv<4, 8>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
+                    if (R > 2 * kWave) run_gemv<4, kB8>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
                     else run_gemv<1, kGemvBatch>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
                 } else if (R <= kWave) {
                     run_gemv<1, 8>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
                 } else if (bw == 1) {
                     run_gemv<8, 1>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
                 } else if (bw <= 4) {
-                    run_gemv<8, 4>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
+                    run_gemv<sizeof(T) == 8 ? 4 : 8, 4>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
                 } else {
-                    run_gemv<4, 8>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
+                    run_gemv<4, kB8>(M.val, x, map, u, xc, lb, ta, tb, tri, R, s0);
                 }
                 if (tri == 0) u += nlast;
                 wave_fence();

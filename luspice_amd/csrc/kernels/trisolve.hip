@@ -380,11 +380,13 @@ __device__ void tri_task(const TriLaunch &L, int t, int epoch)
 template <typename T, bool kForward>
 __device__ __forceinline__ void trisolve_body(const TriLaunch &L)
 {
+    // A Krylov iteration after every circuit converged: nothing to solve.
+    // Every wave of the grid reads the same count (written by an earlier
+    // launch) and returns at once: no ticket, no finish count, and the epoch
+    // stays, as no flag is written under it.
+    if (L.n_active && uniform(ld_relaxed(L.n_active)) == 0) return;
     const int epoch = read_epoch(L.epoch);
-    // a Krylov iteration after every circuit converged: nothing to solve
-    const bool idle = L.n_active && uniform(ld_relaxed(L.n_active)) == 0;
     for (;;) {
-        if (idle) break;
         const int t = wave_ticket(&L.ctr[0]);
         if (t >= L.ntask) break;
         if (aborted(L.abort_word)) continue;

@@ -119,6 +119,37 @@ def test_fp32_and_fp64_on_c2():
 
 
 @pytest.mark.gpu
+def test_fp64_fallback_after_an_ir_stall():
+    """IR held to one restart of one inner iteration cannot bring fp32
+    factors to the tolerance: the stalled circuits alone are refactored in
+    fp64 and solved again (the fallback count says how many), and every
+    circuit ends within 1e-6 of the fp64 oracle.  One circuit of the batch is
+    left inactive and must not be counted."""
+    cs = [L.mna("c2", n=1500, seed=2003), L.mna("c0", seed=0), L.mna("sized", n=700, seed=8)]
+    for c in cs:
+        c.q = L.colamd(c)
+    print("case: fp32 IR stall -> fp64 fallback, n =", [c.n for c in cs], flush=True)
+    B = L.Batch(cs, ir_outer=1, ir_inner=1, ir_outer_tol=1e-10, ir_inner_tol=1e-10, fallback_fp64=1)
+    assert B.refactor("fp32")["rc"] == 0
+    B.set_active([True, True, False])
+    rhs = np.concatenate([c.rhs for c in cs])
+    x, sr = B.solve(rhs)
+    print(sr, flush=True)
+    assert sr["fallback"] == 2 and sr["converged"] == 2 and sr["rc"] == 0
+    for i in (0, 1):
+        c = cs[i]
+        rc, xr, _, _ = O.cpu_lu_ref(c.n, c.row, c.col, c.val, c.q, c.rhs)
+        assert _rel(x[B.voff[i]:B.voff[i + 1]], xr) <= 1e-6, i
+    # without the fallback the same solve reports the stall
+    B2 = L.Batch(cs[:2], ir_outer=1, ir_inner=1, ir_outer_tol=1e-10, ir_inner_tol=1e-10, fallback_fp64=0)
+    assert B2.refactor("fp32")["rc"] == 0
+    x2, sr2 = B2.solve(np.concatenate([c.rhs for c in cs[:2]]))
+    assert sr2["rc"] != 0 and sr2["fallback"] == 0 and sr2["converged"] < 2, sr2
+    B.close()
+    B2.close()
+
+
+@pytest.mark.gpu
 def test_krylov_spmv_value_precision():
     """Arnoldi products with fp32 or FP16 SELL values (the residual stays on
     the fp64 values): config 0 with fp32 values, and an FP16 candidate (a
