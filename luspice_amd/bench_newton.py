@@ -173,7 +173,18 @@ def main(argv=None):
                     help="factor storage of the batched runs")
     ap.add_argument("--ir-methods", nargs="+", type=int, default=[0], choices=[0, 1],
                     help="0: GMRES-IR; 1: fused Richardson steps (one launch per step for the batch)")
+    ap.add_argument("--calib-mb", type=int, default=0,
+                    help="size of a device copy issued first (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args(argv)
+    if args.calib_mb > 0:
+        from . import DeviceArray, device_copy, device_sync
+        nb = args.calib_mb << 20
+        ca, cb = DeviceArray(np.zeros(nb // 8)), DeviceArray(np.zeros(nb // 8))
+        device_copy(cb.ptr, ca.ptr, nb)
+        device_sync()
+        ca.close()
+        cb.close()
+        _log(f"[newton] calibration copy {nb} B")
     out = compare(args.circuits, args.steps, args.warmup, args.plain_steps, args.nmin, args.nmax,
                   args.seed, tuple(args.precs), not args.no_graph, args.check, tuple(args.ir_methods))
     out["metric"] = "batched Newton refactor+solve units/s at rel err <=1e-6 (batched vs plain fp64)"

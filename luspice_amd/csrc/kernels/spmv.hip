@@ -23,13 +23,17 @@
 
 namespace lsp {
 
+constexpr int kSpmvBatch = 8;           // slots of a row loaded per round
+
 template <typename V>
 __global__ void __launch_bounds__(256)
 sell_spmv_kernel(const SellDesc<V> *mats, const int2 *tasks, int ntask,
                  const int64_t *voff, const double *x, double *y, const double *b,
                  const int *active)
 {
-    const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    // the task index is wave-uniform: readfirstlane lets the task and its
+    // descriptor come through scalar loads
+    const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     if (t >= ntask) return;
     const int2 tk = tasks[t];
     if (active && !active[tk.x]) return;
@@ -41,13 +45,30 @@ sell_spmv_kernel(const SellDesc<V> *mats, const int2 *tasks, int ntask,
     const double *xm = x + vo;
     const int len = M.rlen[k];
     const int64_t base = M.off[tk.y] + lane;
-    double s = 0.0;
-    for (int j = 0; j < len; j++) {
-        const int64_t e = base + (int64_t)j * kWave;
-        s += to_f64(M.val[e]) * xm[M.col[e]];
-    }
     const int row = M.perm[k];
-    y[vo + row] = b ? b[vo + row] - s : s;
+    const double bi = b ? b[vo + row] : 0.0;
+    // kSpmvBatch slots per round: every value and column id of the round is
+    // loaded, then every x entry, then the products are added in slot order
+    // (the same sum as one slot at a time), so a row of up to kSpmvBatch
+    // entries costs two dependent memory round trips instead of two per entry
+    double s = 0.0;
+    for (int j0 = 0; j0 < len; j0 += kSpmvBatch) {
+        V v[kSpmvBatch];
+        int c[kSpmvBatch];
+#pragma unroll
+        for (int u = 0; u < kSpmvBatch; u++) {
+            const int64_t e = base + (int64_t)min(j0 + u, len - 1) * kWave;
+            v[u] = M.val[e];
+            c[u] = M.col[e];
+        }
+        double xv[kSpmvBatch];
+#pragma unroll
+        for (int u = 0; u < kSpmvBatch; u++) xv[u] = xm[c[u]];
+#pragma unroll
+        for (int u = 0; u < kSpmvBatch; u++)
+            if (j0 + u < len) s += to_f64(v[u]) * xv[u];
+    }
+    y[vo + row] = b ? bi - s : s;
 }
 
 __global__ void __launch_bounds__(256)

@@ -124,6 +124,7 @@ class Server:
 
     # -- host phase on arrival ---------------------------------------------
     def _prepare(self, r: Request):
+        t_prep = time.perf_counter()
         try:
             c = r.circuit
             r.batch = Batch(Circuit(c.n, c.row, c.col, c.val, c.rhs, q=c.q), upload=False, **self.opts)
@@ -141,6 +142,7 @@ class Server:
             return
         with self.lock:
             r.ready_at = time.perf_counter()
+            r.prep_s = r.ready_at - t_prep
             self.preparing -= 1
             self.ready.append(r)
             self.lock.notify_all()
@@ -271,6 +273,9 @@ class Server:
         solves = int(lat.size)
         errors = [r.error for r in requests if r.error]
         max_res = max((sr["max_rel_res"] for r in requests for _, sr in r.reports), default=0.0)
+        prep = np.array([getattr(r, "prep_s", 0.0) for r in requests]) * 1e3
+        first = np.array([r.lat[0] for r in requests if r.lat]) * 1e3
+        later = np.array([v for r in requests for v in r.lat[1:]]) * 1e3
         return {
             "requests": len(requests), "solves": solves, "wall_s": round(wall, 3),
             "solves_per_s": round(solves / wall, 2) if wall > 0 else 0.0,
@@ -281,6 +286,13 @@ class Server:
             "cohorts": self.cohorts, "launch_steps": self.launch_steps,
             "mean_requests_per_launch": round(self.circuit_steps / max(1, self.launch_steps), 2),
             "max_requests_per_cohort": self.max_in_cohort,
+            # where the tail comes from: the host symbolic phase on arrival
+            # (step 0 includes it) against the steps of resident requests
+            "prep_ms_p50": round(float(np.percentile(prep, 50)), 2) if prep.size else None,
+            "prep_ms_max": round(float(prep.max()), 2) if prep.size else None,
+            "first_step_p99_ms": round(float(np.percentile(first, 99)), 3) if first.size else None,
+            "later_steps_p50_ms": round(float(np.percentile(later, 50)), 3) if later.size else None,
+            "later_steps_p99_ms": round(float(np.percentile(later, 99)), 3) if later.size else None,
         }
 
     def close(self):
