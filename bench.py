@@ -281,6 +281,23 @@ def main():
                                  (m9["err"] is None or m9["err"] <= 1e-6))}
     log(f"[bench] selector tolerance: {sel_tol['value']:.2f} steps/s err={m9['err']}")
 
+    # the mixed step with the refactor on fp64 columns and the fp32 factors the
+    # solves read rounded from them (lsp_options.refactor_fp64); fp32 storage
+    # for the solves, fp64 for the factorization's own columns
+    log("[bench] mixed path, refactor on fp64 columns ...")
+    Bc = L.Batch(circ, refactor_fp64=1)
+    Bc.set_rhs(c.rhs)
+    mc = measure("fp32", Bc)
+    Bc.close()
+    fp64_cols = {"value": round(args.steps / mc["wall"], 3),
+                 "ms_per_step": round(1e3 * mc["wall"] / args.steps, 3), "rel_err": mc["err"],
+                 "p50_ms": round(float(np.percentile(mc["lat"], 50)), 3),
+                 "p99_ms": round(float(np.percentile(mc["lat"], 99)), 3),
+                 "ir_inner_iters": int(np.median([r[1]["inner_total"] for r in mc["reps"]])),
+                 "pass_1e-6": bool(all(r[1]["rc"] == 0 for r in mc["reps"]) and
+                                   (mc["err"] is None or mc["err"] <= 1e-6))}
+    log(f"[bench] refactor on fp64 columns: {fp64_cols['value']:.2f} steps/s err={mc['err']}")
+
     # the north-star comparison: the batched resident mixed path against the
     # plain one-matrix-per-launch fp64 path on the Newton mix (512 circuits,
     # n in [100, 5000], seed 3, 20 steps)
@@ -337,6 +354,7 @@ def main():
                       "rel_err": m64["err"]},
         "mixed_over_fp64": round(value / (args.steps / m64["wall"]), 3),
         "mixed_at_selector_tol": sel_tol,
+        "mixed_refactor_fp64_columns": fp64_cols,
         "kernel_ms": {"refactor_kernel": round(rf_ms, 4),
                       "dense_block": round(st["dense_ms"] / max(1, st["refactor_count"]), 4),
                       "trisolve_pair": round(tri_ms, 4),

@@ -93,6 +93,10 @@ typedef struct {
                               x = 0, until ||b - A x|| <= ir_outer_tol ||b||), the circuits they
                               leave unconverged after ir_fused_steps are solved by GMRES-IR */
     int ir_fused_steps;    /* most fused steps per solve (0 = 12) */
+    int refactor_fp64;     /* with LSP_PREC_FP32: 1 = the refactor runs on fp64 columns (the
+                              fp64 factor set) and the fp32 factors the solves read are its
+                              rounding, instead of a refactor that reads its earlier columns
+                              back in fp32; a better preconditioner for one fp64 refactor */
 } lsp_options;
 
 typedef struct {

@@ -42,7 +42,8 @@ class Options(C.Structure):
                 ("tail_max", C.c_int), ("tail_density", C.c_double), ("ruiz_sweeps", C.c_int),
                 ("pivot_scaled", C.c_int), ("poll_backoff", C.c_int),
                 ("factor_shift", C.c_double), ("krylov_spmv", C.c_int), ("repivot", C.c_int),
-                ("allow_fp16", C.c_int), ("ir_method", C.c_int), ("ir_fused_steps", C.c_int)]
+                ("allow_fp16", C.c_int), ("ir_method", C.c_int), ("ir_fused_steps", C.c_int),
+                ("refactor_fp64", C.c_int)]
 
 
 class RefactorReport(C.Structure):

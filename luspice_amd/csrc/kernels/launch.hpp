@@ -157,6 +157,7 @@ struct RuizArgs {
     int pad_;
 };
 hipError_t launch_ruiz(const RuizArgs &R, int ncirc, hipStream_t st);
+hipError_t launch_round_f32(const double *in, float *out, long count, hipStream_t st);
 // Factor-only diagonal shift: dst[e] = src[e] + shift*|src[e]| for e in idx[0..nidx).
 hipError_t launch_shift(const double *src, double *dst, const long *idx, long nidx, double shift,
                         hipStream_t st);

@@ -82,4 +82,19 @@ hipError_t launch_shift(const double *src, double *dst, const long *idx, long ni
     return hipGetLastError();
 }
 
+// out[i] = (float) in[i]: the fp32 factor set as the rounding of the fp64 one
+__global__ void __launch_bounds__(256) round_f32_kernel(const double *in, float *out, long count)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (long)gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
+}
+
+hipError_t launch_round_f32(const double *in, float *out, long count, hipStream_t st)
+{
+    if (count <= 0) return hipSuccess;
+    const long want = (count + 255) / 256, blocks = want < 8192 ? want : 8192;
+    hipLaunchKernelGGL(round_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, in, out, count);
+    return hipGetLastError();
+}
+
 }  // namespace lsp

@@ -992,6 +992,18 @@ int Batch::launch_refactor_prec(int prec, const int *d_active)
         if (!rc) rc = check(launch_shift(d_ain_, d_ains_, d_sidx_, nsidx_, opt_.factor_shift, stream_), "shift");
         if (rc) return rc;
     }
+    if (prec == LSP_PREC_FP32 && opt_.refactor_fp64) {
+        // refactor on fp64 columns, then the fp32 set the solves read is its
+        // rounding (every array, so exports and pivot ranges see fp32 storage)
+        int rc = ensure_factor_set(f32_);
+        if (!rc) rc = refactor_t(f64_, d_active);
+        if (!rc) rc = check(launch_round_f32(f64_.val, f32_.val, nslot_tot_, stream_), "round val");
+        if (!rc) rc = check(launch_round_f32(f64_.lrow, f32_.lrow, nl_tot_, stream_), "round lrow");
+        if (!rc) rc = check(launch_round_f32(f64_.urow, f32_.urow, nu_tot_, stream_), "round urow");
+        if (!rc) rc = check(launch_round_f32(f64_.diag, f32_.diag, ntot_, stream_), "round diag");
+        if (!rc) rc = check(launch_round_f32(f64_.tailF, f32_.tailF, nt2_tot_, stream_), "round tail");
+        return rc;
+    }
     if (prec == LSP_PREC_FP32) return refactor_t(f32_, d_active);
     if (prec == LSP_PREC_FP64) return refactor_t(f64_, d_active);
     if (prec == LSP_PREC_FP16) {
@@ -1847,6 +1859,7 @@ int Batch::graph_step(int prec, lsp_refactor_report *rrep, lsp_solve_report *sre
         // every allocation happens before the capture
         if (prec == LSP_PREC_FP32) rc = ensure_factor_set(f32_);
         else if (prec == LSP_PREC_FP64) rc = ensure_factor_set(f64_);
+        if (!rc && prec == LSP_PREC_FP32 && opt_.refactor_fp64) rc = ensure_factor_set(f64_);
         else if (prec == LSP_PREC_FP16) rc = ensure_factor_set(f16_);
         else return LSP_E_ARG;
         if (rc) return rc;

@@ -229,3 +229,28 @@ def test_pivot_range_is_the_range_of_u_diagonal(c0, c0_q):
         d = np.abs(F["Ux"][F["Up"][1:] - 1])
         lo, hi = B.pivot_range(0)
         np.testing.assert_allclose([lo, hi], [d.min(), d.max()], rtol=tol, err_msg=prec)
+
+
+def test_refactor_fp64_columns_gives_the_rounded_fp64_factors(c0, c0_q):
+    """refactor_fp64: the fp32 factors are fp32 values within one rounding
+    (2^-24 relative) of an fp64 refactor's factors (two fp64 refactors may
+    differ in the last bits: dependencies are applied in the order their
+    flags are seen), and GMRES-IR on them meets the pass line."""
+    print("case: c0 fp32 factors rounded from the fp64 refactor", flush=True)
+    circ = L.Circuit(c0.n, c0.row, c0.col, c0.val, c0.rhs, q=c0_q)
+    B64 = L.Batch(circ)
+    assert B64.refactor("fp64")["rc"] == 0
+    F64 = B64.export_factors(0)
+    B = L.Batch(circ, refactor_fp64=1)
+    assert B.refactor("fp32")["rc"] == 0
+    F = B.export_factors(0)
+    for k in ("pinv", "Lp", "Li", "Up", "Ui"):
+        assert np.array_equal(F[k], F64[k]), k
+    for k in ("Lx", "Ux"):
+        assert np.array_equal(F[k], F[k].astype(np.float32).astype(np.float64)), k
+        np.testing.assert_allclose(F[k], F64[k], rtol=2.0 ** -24 * (1 + 1e-6),
+                                   atol=1e-13 * np.abs(F64[k]).max(), err_msg=k)
+    x, sr = B.solve(c0.rhs)
+    err = O.rel_l2(x, _oracle_x(c0, c0_q))
+    print("rel err", err, sr, flush=True)
+    assert sr["rc"] == 0 and err <= 1e-6
