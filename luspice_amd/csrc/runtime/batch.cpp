@@ -372,6 +372,9 @@ int Batch::prepare(const lsp_circuit_in *cin, int nc, const lsp_options *opt)
     if (!cin || nc <= 0 || !circ_.empty()) return LSP_E_ARG;
     if (opt) opt_ = *opt; else lsp_options_default(&opt_);
     if (opt_.repivot < 0 || opt_.repivot > 3) return LSP_E_ARG;
+    if (opt_.ir_method < 0 || opt_.ir_method > 1 ||
+        opt_.ir_fused_steps < 0 || opt_.refactor_fp64 < 0 || opt_.refactor_fp64 > 1)
+        return LSP_E_ARG;
     refactor_tl_path_ = getenv("LSP_REFACTOR_TIMELINE");     // diagnostic dumps
     trisolve_tl_path_ = getenv("LSP_TRISOLVE_TIMELINE");
     circ_.resize(nc);
@@ -1859,9 +1862,10 @@ int Batch::graph_step(int prec, lsp_refactor_report *rrep, lsp_solve_report *sre
         // every allocation happens before the capture
         if (prec == LSP_PREC_FP32) rc = ensure_factor_set(f32_);
         else if (prec == LSP_PREC_FP64) rc = ensure_factor_set(f64_);
-        if (!rc && prec == LSP_PREC_FP32 && opt_.refactor_fp64) rc = ensure_factor_set(f64_);
         else if (prec == LSP_PREC_FP16) rc = ensure_factor_set(f16_);
         else return LSP_E_ARG;
+        // the fp64-column refactor of the mixed path writes both sets
+        if (!rc && prec == LSP_PREC_FP32 && opt_.refactor_fp64) rc = ensure_factor_set(f64_);
         if (rc) return rc;
         if (use_fused(prec)) {
             rc = ir_prepare();

@@ -50,10 +50,11 @@ def test_newton_batched_matches_oracle():
 
 
 @pytest.mark.gpu
-def test_graph_step_matches_eager():
-    print("case: 12-circuit Newton batch, graph replay vs eager, 4 steps", flush=True)
+@pytest.mark.parametrize("opts", [{}, {"refactor_fp64": 1}])
+def test_graph_step_matches_eager(opts):
+    print("case: 12-circuit Newton batch, graph replay vs eager, 4 steps", opts, flush=True)
     circ = W.newton_mix(12, 100, 2000, seed=5)
-    NB = W.NewtonBatched(circ, 4, seed=5)
+    NB = W.NewtonBatched(circ, 4, seed=5, **opts)
     for s in range(4):
         rr, sr = NB.run_step(s)                      # eager, with its host synchronisation
         assert rr["rc"] == 0 and sr["rc"] == 0, (rr, sr)
