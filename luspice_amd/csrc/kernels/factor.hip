@@ -49,12 +49,19 @@ __device__ __forceinline__ void run_gemv(const T *val, XP x, const LSP_GLOBAL ui
     const int lane = lane_id();
     const int bw = tb - ta;
     for (int i0 = 0; i0 < R; i0 += G * kWave) {
-        int i[G];
+        int i[G], slot[G];
         double acc[G];
 #pragma unroll
         for (int g = 0; g < G; g++) {
             i[g] = min(i0 + g * kWave + lane, R - 1);
             acc[g] = 0.0;
+        }
+        // fp64 values: the target slots first, so their map loads overlap the
+        // value loads (measured on c1: 3.92 -> 3.69 ms; with fp32 values the
+        // same order measured slower, 3.55 -> 3.70 ms, so they load after)
+        if constexpr (sizeof(T) == 8) {
+#pragma unroll
+            for (int g = 0; g < G; g++) slot[g] = i[g] < tri ? s0 + tb + i[g] : (int)map[u + (i[g] - tri)];
         }
         for (int q0 = 0; q0 < bw; q0 += B) {
             T v[B][G];
@@ -77,8 +84,8 @@ __device__ __forceinline__ void run_gemv(const T *val, XP x, const LSP_GLOBAL ui
 #pragma unroll
         for (int g = 0; g < G; g++) {
             if (i0 + g * kWave + lane < R) {
-                const int slot = i[g] < tri ? s0 + tb + i[g] : (int)map[u + (i[g] - tri)];
-                x[slot] -= acc[g];
+                if constexpr (sizeof(T) != 8) slot[g] = i[g] < tri ? s0 + tb + i[g] : (int)map[u + (i[g] - tri)];
+                x[slot[g]] -= acc[g];
             }
         }
     }
@@ -97,10 +104,14 @@ __device__ __forceinline__ bool run_gemv_unset(const T *val, XP x, const LSP_GLO
 {
     const int lane = lane_id();
     for (int i0 = 0; i0 < R; i0 += G * kWave) {
-        int i[G];
+        int i[G], slot[G];
         T v[G];
 #pragma unroll
         for (int g = 0; g < G; g++) i[g] = min(i0 + g * kWave + lane, R - 1);
+        // the rows' slots do not depend on the predecessor: their loads are
+        // in flight while its values are polled, not after they arrive
+#pragma unroll
+        for (int g = 0; g < G; g++) slot[g] = (int)map[u + i[g]];
         for (int round = 0;; round++) {
 #pragma unroll
             for (int g = 0; g < G; g++) v[g] = ld_coherent_val(gp(val) + (uint32_t)base, (uint32_t)i[g]);
@@ -117,7 +128,7 @@ __device__ __forceinline__ bool run_gemv_unset(const T *val, XP x, const LSP_GLO
         }
 #pragma unroll
         for (int g = 0; g < G; g++) {
-            if (i0 + g * kWave + lane < R) x[(int)map[u + i[g]]] -= to_f64(v[g]) * xj;
+            if (i0 + g * kWave + lane < R) x[slot[g]] -= to_f64(v[g]) * xj;
         }
     }
     return true;
