@@ -133,13 +133,13 @@ __device__ bool tri_backward(const TriDesc<T> &M, int s, int e, double *z, doubl
         if (bw <= 8) zt = run_triangle_upper<8, false>(M.val, num, rd, dpc, run, bw);
         else if (bw <= 24) zt = run_triangle_upper<24, false>(M.val, num, rd, dpc, run, bw);
         else zt = run_triangle_upper<kWave, false>(M.val, num, rd, dpc, run, bw);
+        if (lane < bw) st_coherent(&z[row], zt);
+        store_complete();
         if (lane < bw) {
-            st_coherent(&z[row], zt);
-            const int qo = M.q[row];
+            st_relaxed(&flags[row], epoch);
+            const int qo = M.q[row];                   // the solution, after the flag
             x[qo] = cs ? zt * cs[qo] : zt;
         }
-        store_complete();
-        if (lane < bw) st_relaxed(&flags[row], epoch);
     }
     return true;
 }
@@ -173,6 +173,20 @@ __device__ void tri_packet(const TriLaunch &L, const TriDesc<T> &M, int o, long 
     }
     const int *ci = kForward ? M.lci : M.uci;
     const T *rv = kForward ? M.lrow : M.urow;
+    double *y = L.y + vo;
+    double *z = L.z + vo;
+    const bool lead = rc >= 0 && l == 0;
+    // b (forward), y and the diagonal (backward) before the waits
+    double pre = 0.0, dia = 1.0;
+    if (lead) {
+        if (kForward) {
+            const int ro = M.rowat[i];
+            pre = L.rs ? L.b[vo + ro] * L.rs[vo + ro] : L.b[vo + ro];
+        } else {
+            pre = y[i];
+            if (!part) dia = to_f64(M.diag[i]);
+        }
+    }
     int jj[kTriPacketLane];
     T vv[kTriPacketLane];
     const int *fl[kTriPacketLane];
@@ -185,8 +199,6 @@ __device__ void tri_packet(const TriLaunch &L, const TriDesc<T> &M, int o, long 
         fl[h] = on ? &flags[jj[h]] : nullptr;
     }
     if (!wave_wait_n<kTriPacketLane>(fl, epoch, L.abort_word)) return;
-    double *y = L.y + vo;
-    double *z = L.z + vo;
     const double *src = kForward ? y : z;
     double sv[kTriPacketLane];
 #pragma unroll
@@ -195,23 +207,16 @@ __device__ void tri_packet(const TriLaunch &L, const TriDesc<T> &M, int o, long 
 #pragma unroll
     for (int h = 0; h < kTriPacketLane; h++) acc += to_f64(vv[h]) * sv[h];
     for (int off = g >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
-    const bool lead = rc >= 0 && l == 0;
+    const double r = kForward || part ? pre - acc : (pre - acc) / dia;
+    if (lead) st_coherent(kForward ? &y[i] : &z[i], r);
+    store_complete();
     if (lead) {
-        if (kForward) {
-            const int ro = M.rowat[i];
-            const double bi = L.rs ? L.b[vo + ro] * L.rs[vo + ro] : L.b[vo + ro];
-            st_coherent(&y[i], bi - acc);
-        } else if (part) {
-            st_coherent(&z[i], y[i] - acc);
-        } else {
-            const double zi = (y[i] - acc) / to_f64(M.diag[i]);
-            st_coherent(&z[i], zi);
+        st_relaxed(&flags[i], part ? -epoch : epoch);
+        if (!kForward && !part) {                      // the solution, after the flag
             const int qo = M.q[i];
-            L.x[vo + qo] = L.cs ? zi * L.cs[vo + qo] : zi;
+            L.x[vo + qo] = L.cs ? r * L.cs[vo + qo] : r;
         }
     }
-    store_complete();
-    if (lead) st_relaxed(&flags[i], part ? -epoch : epoch);
 }
 
 // Packet of small supernode triangles (header -g, g = 4, 8 or 16): up to
@@ -283,13 +288,13 @@ __device__ void tri_tpacket(const TriLaunch &L, const TriDesc<T> &M, int o, long
             }
         }
         const double zt = num * rd;
+        if (on) st_coherent(&z[row], zt);
+        store_complete();
         if (on) {
-            st_coherent(&z[row], zt);
-            const int qo = M.q[row];
+            st_relaxed(&flags[row], epoch);
+            const int qo = M.q[row];                   // the solution, after the flag
             L.x[vo + qo] = L.cs ? zt * L.cs[vo + qo] : zt;
         }
-        store_complete();
-        if (on) st_relaxed(&flags[row], epoch);
     }
 }
 
@@ -333,6 +338,18 @@ __device__ void tri_task(const TriLaunch &L, int t, int epoch)
         e0 = part ? M.bcut[i] : M.urp[i];
         e1 = M.urp[i + 1];
     }
+    // what the row's result needs besides its sum -- b (forward), y and the
+    // diagonal (backward) -- is loaded before the waits, off the chain
+    double pre = 0.0, dia = 1.0;
+    if (lane == 0) {
+        if (kForward) {
+            const int ro = M.rowat[i];
+            pre = L.rs ? L.b[vo + ro] * L.rs[vo + ro] : L.b[vo + ro];
+        } else {
+            pre = y[i];
+            if (!part) dia = to_f64(M.diag[i]);
+        }
+    }
     double acc = 0.0;
     bool ok = true;
     for (int b0 = e0; b0 < e1 && ok; b0 += kGather * kWave) {
@@ -357,20 +374,14 @@ __device__ void tri_task(const TriLaunch &L, int t, int epoch)
     if (!ok) return;
     acc = wave_sum(acc);
     if (lane == 0) {
-        if (kForward) {
-            const int ro = M.rowat[i];
-            const double bi = L.rs ? L.b[vo + ro] * L.rs[vo + ro] : L.b[vo + ro];
-            st_coherent(&y[i], bi - acc);
-        } else if (part) {
-            st_coherent(&z[i], y[i] - acc);
-        } else {
-            const double zi = (y[i] - acc) / to_f64(M.diag[i]);
-            st_coherent(&z[i], zi);
-            const int qo = M.q[i];
-            L.x[vo + qo] = L.cs ? zi * L.cs[vo + qo] : zi;
-        }
+        const double r = kForward || part ? pre - acc : (pre - acc) / dia;
+        st_coherent(kForward ? &y[i] : &z[i], r);
         store_complete();
         st_relaxed(&flags[i], part ? -epoch : epoch);
+        if (!kForward && !part) {                      // the solution, after the flag
+            const int qo = M.q[i];
+            L.x[vo + qo] = L.cs ? r * L.cs[vo + qo] : r;
+        }
     }
 }
 
