@@ -46,11 +46,11 @@ __device__ bool tri_forward(const TriDesc<T> &M, int s, int e, double *y, int *f
     for (int b0 = 0; b0 < w; b0 += kWave) {
         const int bw = min(kWave, w - b0);
         const int row = s + b0 + lane;
-        if (!wave_wait(lane < bw ? &flags[row] : nullptr, -epoch, abort_word)) return false;
         const int rr = s + b0 + min(lane, bw - 1);
-        double p = lane < bw ? ld_coherent_val(&y[row]) : 0.0;
-        // earlier blocks, 64 columns at a time: lane q holds y of column q of
-        // the chunk (one load), broadcast by readlane
+        // earlier blocks (solved by this task), 64 columns at a time: lane q
+        // holds y of column q of the chunk (one load), broadcast by readlane.
+        // They do not depend on this block's partial sums, so they are
+        // accumulated before the wait for those.
         constexpr int SB = kSolveBatch;
         double acc = 0.0;
         for (int q0 = 0; q0 < b0; q0 += kWave) {
@@ -68,6 +68,8 @@ __device__ bool tri_forward(const TriDesc<T> &M, int s, int e, double *y, int *f
                     if (h0 + h < qn) acc = fma(to_f64(v[h]), bcast_f64(yq, h0 + h), acc);
             }
         }
+        if (!wave_wait(lane < bw ? &flags[row] : nullptr, -epoch, abort_word)) return false;
+        double p = lane < bw ? ld_coherent_val(&y[row]) : 0.0;
         p -= acc;
         const int lb = dp[rr] + 1;
         if (bw == 1) {
@@ -97,11 +99,10 @@ __device__ bool tri_backward(const TriDesc<T> &M, int s, int e, double *z, doubl
         const int bw = min(kWave, b1);
         const int base = s + b1 - bw;
         const int row = base + lane;
-        if (!wave_wait(lane < bw ? &flags[row] : nullptr, -epoch, abort_word)) return false;
         const int rr = base + min(lane, bw - 1);
-        double num = lane < bw ? ld_coherent_val(&z[row]) : 0.0;
-        // later blocks, 64 columns at a time: lane q holds z of column q of
-        // the chunk (one load), broadcast by readlane
+        // later blocks (solved by this task), 64 columns at a time: lane q
+        // holds z of column q of the chunk (one load), broadcast by readlane;
+        // accumulated before the wait for this block's partial sums
         constexpr int SB = kSolveBatch;
         const int c0 = base + bw;                        // first column of the later blocks
         const int cnt = e - c0;
@@ -125,10 +126,12 @@ __device__ bool tri_backward(const TriDesc<T> &M, int s, int e, double *z, doubl
                     if (h0 + h < qn && in[h]) acc = fma(to_f64(v[h]), bcast_f64(zq, h0 + h), acc);
             }
         }
-        num -= acc;
         const double rd = 1.0 / to_f64(M.diag[rr]);
         const int dpc = dp[rr];
         const int run = ru[rr];
+        if (!wave_wait(lane < bw ? &flags[row] : nullptr, -epoch, abort_word)) return false;
+        double num = lane < bw ? ld_coherent_val(&z[row]) : 0.0;
+        num -= acc;
         double zt;
         if (bw <= 8) zt = run_triangle_upper<8, false>(M.val, num, rd, dpc, run, bw);
         else if (bw <= 24) zt = run_triangle_upper<24, false>(M.val, num, rd, dpc, run, bw);
