@@ -54,26 +54,37 @@ __device__ __forceinline__ double bcast(double v, int src)
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal block: LU of the 64x64 block and the inverses of both factors.
-// 256 threads; thread (r, s) = (tid % 16, tid / 16) keeps elements (r + 16a,
-// s + 16b), a, b < 4, in registers.  A partial last block is padded with
-// identity rows/columns.
-//   * elimination, step c: the owners of row c and column c publish them in
-//     LDS (double-buffered by step parity, one barrier per step); every thread
-//     forms l_i = A(i,c) / A(c,c) and updates its trailing elements.  The
-//     same row operations applied to I build L^-1 (Gauss-Jordan): row c of
-//     L^-1 has no entry right of c and the trailing part of A none left of
-//     c, so one published row carries both.
-//   * U^-1 in the same steps, by columns: V = U^-1 solves V U = I, so
-//     V(i,c) = (delta_ic - Y(i,c)) / U(c,c) with Y(i,j) = sum_{k<c} V(i,k)
-//     U(k,j).  Row c of U is final (and published) at step c, so the owners
-//     of column c publish delta - Y(:,c) beside column c of A, every thread
-//     scales it by the pivot reciprocal and adds V(i,c) U(c,j) into Y(i,j),
-//     j > c.  No second pass of 64 barrier steps is needed.
-// Outputs: L\U over the block in S, L^-1 (unit lower) and U^-1 (upper) into
-// the job's workspace W (column-major, leading dimension 64), and the column
-// maxima of |L| for the pivot test.
-constexpr int kWs = 2 * kNb * kNb;
+// Diagonal block: LU of the 64x64 block and the inverses of both factors,
+// blocked by 16-column sub-panels (256 threads).  Thread (r, s) = (tid % 16,
+// tid / 16) keeps elements (r + 16a, s + 16b), a, b < 4, of A, of X = L^-1
+// and of Y = U^-1 in registers; a partial last block is padded with identity
+// rows and columns.  For each sub-panel k (columns k0 = 16k .. k0 + 15):
+//   1. the owners publish, in LDS, the panel columns (rows >= k0), the block
+//      row [R | I | A12] (R: rows k0.. of X left of k0; A12: rows k0.. of A
+//      right of the panel) and delta - Yacc over the panel columns;
+//   2. wave 0 factors the 64 x 16 panel, one row per lane: step c broadcasts
+//      the pivot row by readlane, every lane below it forms l = A(i, c) / p
+//      (times the reciprocal) and updates its row -- the same operations, in
+//      the same order, as the unblocked right-looking elimination, so the
+//      factor is the same bit for bit;
+//   3. wave 1 applies L11^-1 to the block row, one column per lane: D R, D =
+//      L11^-1 and U12 = L11^-1 A12 in one forward substitution; wave 2 solves
+//      V U11 = delta - Yacc, one row per lane, for the panel columns of Y;
+//   4. every thread takes its final panel values and applies the rank-16
+//      updates: A22 -= L21 U12, the rows of X below the panel -= L21 [D R | D]
+//      (X <- M_k^-1 X for the elementary block factor M_k of L), and
+//      Yacc(:, right of the panel) += V U12.
+// Four barriers per sub-panel instead of one per column.  Outputs: L\U over
+// the block in S, L^-1 (unit lower) and U^-1 (upper) into the job's
+// workspace W (column-major, leading dimension 64), and the column maxima of
+// |L| for the pivot test.
+constexpr int kDgLs = 80;                  // row stride of the panel and delta - Yacc buffers
+constexpr int kDgRs = 68;                  // row stride of the block-row buffer
+constexpr int kDgLp = 0;                   // [16][80]: A(i, k0 + q) at q * 80 + i
+constexpr int kDgRow = 16 * kDgLs;         // [16][68]: block row at q * 68 + j
+constexpr int kDgW = kDgRow + 16 * kDgRs;  // [16][80]: delta - Yacc(i, k0 + q), then V, at q * 80 + i
+constexpr int kDgRp = kDgW + 16 * kDgLs;   // [16]: pivot reciprocals of the sub-panel
+constexpr size_t kDiagLds = sizeof(double) * (kDgRp + 16);
 
 __device__ __forceinline__ double max16(double v)
 {
@@ -84,15 +95,16 @@ __device__ __forceinline__ double max16(double v)
 
 // The block body, run by all 256 threads of a workgroup: panel p of job J,
 // its input read from src (column-major, leading dimension lds: the block in
-// S itself, or a copy in LDS that the caller has made visible with a barrier).
-__device__ __forceinline__ void diag_block(const DenseJob &J, int p, const double *src, size_t lds)
+// S itself, or a copy in LDS that the caller has made visible with a
+// barrier); sm is kDiagLds bytes of LDS, which may overlap src.
+__device__ __forceinline__ void diag_block(const DenseJob &J, int p, const double *src, size_t lds, double *sm)
 {
     const int tid = threadIdx.x;
     const int r = tid & 15, s = tid >> 4;
+    const int lane = tid & 63, wid = tid >> 6;
     const int b = min(kNb, J.T - p);
     const size_t ld = (size_t)J.T;
     double *Sp = J.S + (size_t)p * ld + p;
-    __shared__ double bc[2][4][kNb];                   // [parity][row c of A | row c of L^-1 | column c | delta - Y(:,c)]
     double A[4][4], X[4][4], Y[4][4];
 #pragma unroll
     for (int a = 0; a < 4; a++) {
@@ -106,101 +118,129 @@ __device__ __forceinline__ void diag_block(const DenseJob &J, int p, const doubl
             Y[a][bb] = 0.0;
         }
     }
-    // The 16-step blocks are unrolled (register indices c/16 are compile-time
-    // constants); the steps inside a block stay a loop.  Masks are applied to
-    // the broadcast operands (row multipliers zero for rows <= c, row values
-    // zero outside their columns), so each element costs one fma; column c is
-    // then overwritten with l and -l.  Every LDS operand of a step is read
-    // before any is used (one LDS round trip per step).  The reciprocal of the
-    // next pivot is taken one step ahead by every thread: the owner of
-    // A(c+1, c+1) publishes it with step c's row and column, so after the
-    // barrier each thread forms fma(-A(c+1,c) / A(c,c), U(c,c+1), A(c+1,c+1))
-    // -- the owner's own update of that element, bit for bit -- and divides
-    // while its updates run; no division or extra LDS read sits between two
-    // barriers on one wave alone.
-    __shared__ double nxt[2];
-    if (tid == 0) nxt[1] = A[0][0];
-    __syncthreads();
-    double rp = 1.0 / nxt[1];
+    __syncthreads();                                   // src read by every thread: sm may overlap it
+    double *Lp = sm + kDgLp, *Rw = sm + kDgRow, *Wv = sm + kDgW, *Rp = sm + kDgRp;
 #pragma unroll
-    for (int ca = 0; ca < 4; ca++)
-#pragma unroll 1
-    for (int cl = 0; cl < 16; cl++) {
-        const int c = 16 * ca + cl;
-        const int par = c & 1;
-        const int cn1 = ca < 3 ? ca + 1 : 3;
-        if (r == (c & 15)) {
+    for (int kb = 0; kb < 4; kb++) {
+        const int k0 = 16 * kb;
+        // 1. publish
 #pragma unroll
-            for (int bb = 0; bb < 4; bb++) {
-                bc[par][0][s + 16 * bb] = A[ca][bb];
-                bc[par][1][s + 16 * bb] = X[ca][bb];
-            }
-        }
-        if (s == (c & 15)) {
+        for (int a = kb; a < 4; a++) Lp[s * kDgLs + r + 16 * a] = A[a][kb];
 #pragma unroll
-            for (int a = 0; a < 4; a++) {
-                const int i = r + 16 * a;
-                bc[par][2][i] = A[a][ca];
-                bc[par][3][i] = i < c ? -Y[a][ca] : (i == c ? 1.0 : 0.0);
-            }
-        }
-        // A(c+1, c+1) before this step's update: register (cn, cn) of thread
-        // ((c+1) % 16, (c+1) % 16), cn = (c+1) / 16
-        if (c + 1 < kNb && r == ((c + 1) & 15) && s == ((c + 1) & 15))
-            nxt[par] = cl < 15 ? A[ca][ca] : A[cn1][cn1];
+        for (int bb = 0; bb < 4; bb++)
+            Rw[r * kDgRs + s + 16 * bb] = bb < kb ? X[kb][bb] : bb == kb ? (r == s ? 1.0 : 0.0) : A[kb][bb];
+#pragma unroll
+        for (int a = 0; a <= kb; a++) Wv[s * kDgLs + r + 16 * a] = (a == kb && r == s ? 1.0 : 0.0) - Y[a][kb];
         __syncthreads();
-        double rowv[4], xrow[4], colv[4], wv[4];
+        // 2. panel LU on wave 0 (rows above k0 are read and left alone)
+        if (wid == 0) {
+            double v[16];
 #pragma unroll
-        for (int bb = 0; bb < 4; bb++) {
-            rowv[bb] = bc[par][0][s + 16 * bb];
-            xrow[bb] = bc[par][1][s + 16 * bb];
-        }
+            for (int q = 0; q < 16; q++) v[q] = Lp[q * kDgLs + lane];
 #pragma unroll
-        for (int a = 0; a < 4; a++) {
-            colv[a] = bc[par][2][r + 16 * a];
-            wv[a] = bc[par][3][r + 16 * a];
-        }
-        const int cn = c + 1 < kNb ? c + 1 : c;
-        const double a_nn = nxt[par], l_n = bc[par][2][cn], u_n = bc[par][0][cn];
-        const double rpc = rp;
-        double ur[4], xr[4], l[4], vc[4];
-        // masks: only the block that holds index c needs a run-time test
+            for (int q = 0; q < 16; q++) {
+                const int c = k0 + q;
+                const double rp = 1.0 / bcast(v[q], c);
+                const double l = lane > c ? v[q] * rp : 0.0;
+                if (lane > c) v[q] = l;
 #pragma unroll
-        for (int bb = 0; bb < 4; bb++) {
-            const int j = s + 16 * bb;
-            ur[bb] = bb > ca ? rowv[bb] : bb < ca ? 0.0 : (j > c ? rowv[bb] : 0.0);
-            xr[bb] = bb < ca ? xrow[bb] : bb > ca ? 0.0 : (j < c ? xrow[bb] : 0.0);
-        }
-#pragma unroll
-        for (int a = 0; a < 4; a++) {
-            const int i = r + 16 * a;
-            l[a] = a > ca ? colv[a] * rpc : a < ca ? 0.0 : (i > c ? colv[a] * rpc : 0.0);
-            vc[a] = wv[a] * rpc;                         // V(i, c); zero below the diagonal
-        }
-        rp = 1.0 / fma(-(l_n * rpc), u_n, a_nn);          // unused after the last step
-        // Register blocks whose operand is zero for every thread at this step
-        // are skipped (ca is a compile-time index here): the multipliers l
-        // vanish in row blocks a < ca, the U^-1 column in a > ca, the U row in
-        // column blocks bb < ca, the L^-1 row in bb > ca.  fma(0, x, y) = y, so
-        // the result is unchanged.
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int bb = 0; bb < 4; bb++) {
-                if (a >= ca && bb >= ca) A[a][bb] = fma(-l[a], ur[bb], A[a][bb]);
-                if (a >= ca && bb <= ca) X[a][bb] = fma(-l[a], xr[bb], X[a][bb]);
-                if (a <= ca && bb >= ca) Y[a][bb] = fma(vc[a], ur[bb], Y[a][bb]);
+                for (int j = q + 1; j < 16; j++) v[j] = fma(-l, bcast(v[j], c), v[j]);
+                if (lane == 0) Rp[q] = rp;
             }
-        if (s == (c & 15)) {
+            if (lane >= k0) {
 #pragma unroll
-            for (int a = 0; a < 4; a++) {
-                if (r + 16 * a > c) {
-                    A[a][ca] = l[a];
-                    X[a][ca] = -l[a];
+                for (int q = 0; q < 16; q++) Lp[q * kDgLs + lane] = v[q];
+            }
+        }
+        __syncthreads();
+        // 3. triangular solves with the panel's diagonal block; the next
+        //    column of L11 / row of U11 is loaded while the current one is used
+        if (wid == 1) {
+            double w[16], lc[2][16];
+#pragma unroll
+            for (int q = 0; q < 16; q++) w[q] = Rw[q * kDgRs + lane];
+#pragma unroll
+            for (int q = 1; q < 16; q++) lc[0][q] = Lp[k0 + q];
+#pragma unroll
+            for (int qp = 0; qp < 15; qp++) {
+                if (qp + 1 < 15) {
+#pragma unroll
+                    for (int q = qp + 2; q < 16; q++) lc[(qp + 1) & 1][q] = Lp[(qp + 1) * kDgLs + k0 + q];
                 }
-                Y[a][ca] = vc[a];
+#pragma unroll
+                for (int q = qp + 1; q < 16; q++) w[q] = fma(-lc[qp & 1][q], w[qp], w[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < 16; q++) Rw[q * kDgRs + lane] = w[q];
+        } else if (wid == 2) {
+            if (lane < k0 + 16) {                      // rows below hold no V
+                double w[16], uc[2][16], rq[2];
+#pragma unroll
+                for (int q = 0; q < 16; q++) w[q] = Wv[q * kDgLs + lane];
+                rq[0] = Rp[0];
+#pragma unroll
+                for (int j = 1; j < 16; j++) uc[0][j] = Lp[j * kDgLs + k0];
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    if (q + 1 < 16) {
+                        rq[(q + 1) & 1] = Rp[q + 1];
+#pragma unroll
+                        for (int j = q + 2; j < 16; j++) uc[(q + 1) & 1][j] = Lp[j * kDgLs + k0 + q + 1];
+                    }
+                    w[q] = w[q] * rq[q & 1];
+#pragma unroll
+                    for (int j = q + 1; j < 16; j++) w[j] = fma(-w[q], uc[q & 1][j], w[j]);
+                }
+#pragma unroll
+                for (int q = 0; q < 16; q++) Wv[q * kDgLs + lane] = w[q];
             }
         }
+        __syncthreads();
+        // 4. final panel values, then the rank-16 updates (operands of step
+        //    q + 1 loaded while step q is applied)
+#pragma unroll
+        for (int a = kb; a < 4; a++) A[a][kb] = Lp[s * kDgLs + r + 16 * a];
+#pragma unroll
+        for (int bb = 0; bb < 4; bb++) {
+            const double v = Rw[r * kDgRs + s + 16 * bb];
+            if (bb <= kb) X[kb][bb] = v;
+            else A[kb][bb] = v;
+        }
+#pragma unroll
+        for (int a = 0; a <= kb; a++) Y[a][kb] = Wv[s * kDgLs + r + 16 * a];
+        if (kb < 3) {
+            double lq[2][4], rq[2][4], vq[2][4];
+#pragma unroll
+            for (int a = kb + 1; a < 4; a++) lq[0][a] = Lp[r + 16 * a];
+#pragma unroll
+            for (int bb = 0; bb < 4; bb++) rq[0][bb] = Rw[s + 16 * bb];
+#pragma unroll
+            for (int a = 0; a <= kb; a++) vq[0][a] = Wv[r + 16 * a];
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int cb = q & 1, nb = cb ^ 1;
+                if (q + 1 < 16) {
+#pragma unroll
+                    for (int a = kb + 1; a < 4; a++) lq[nb][a] = Lp[(q + 1) * kDgLs + r + 16 * a];
+#pragma unroll
+                    for (int bb = 0; bb < 4; bb++) rq[nb][bb] = Rw[(q + 1) * kDgRs + s + 16 * bb];
+#pragma unroll
+                    for (int a = 0; a <= kb; a++) vq[nb][a] = Wv[(q + 1) * kDgLs + r + 16 * a];
+                }
+#pragma unroll
+                for (int a = kb + 1; a < 4; a++)
+#pragma unroll
+                    for (int bb = 0; bb < 4; bb++) {
+                        if (bb > kb) A[a][bb] = fma(-lq[cb][a], rq[cb][bb], A[a][bb]);
+                        else X[a][bb] = fma(-lq[cb][a], rq[cb][bb], X[a][bb]);
+                    }
+#pragma unroll
+                for (int a = 0; a <= kb; a++)
+#pragma unroll
+                    for (int bb = kb + 1; bb < 4; bb++) Y[a][bb] = fma(vq[cb][a], rq[cb][bb], Y[a][bb]);
+            }
+        }
+        __syncthreads();                               // LDS reads done before the next sub-panel's writes
     }
     double *W = J.W;
 #pragma unroll
@@ -232,9 +272,10 @@ __device__ __forceinline__ void diag_block(const DenseJob &J, int p, const doubl
 
 __global__ void __launch_bounds__(256) dense_diag_kernel(const DenseJob *jobs, const int *active, int p)
 {
+    extern __shared__ double lds[];
     const DenseJob J = jobs[blockIdx.x];
     if (p >= J.T || (active && !active[J.mat])) return;
-    diag_block(J, p, J.S + (size_t)p * J.T + p, (size_t)J.T);
+    diag_block(J, p, J.S + (size_t)p * J.T + p, (size_t)J.T, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -243,6 +284,7 @@ __global__ void __launch_bounds__(256) dense_diag_kernel(const DenseJob *jobs, c
 // As[k*64 + i] = left(i, k), Bs[k*65 + j] = right(k, j).  Consecutive lanes
 // read consecutive LDS words.
 constexpr size_t kGemmLds = sizeof(double) * (kNb * 64 + kNb * kBsLd);
+static_assert(kDiagLds <= kGemmLds, "tile 0 of the update factors the next diagonal block in its own LDS");
 
 __device__ __forceinline__ void tile_mm(const double *As, const double *Bs, int kmax, double (&acc)[4][4])
 {
@@ -356,7 +398,7 @@ __global__ void __launch_bounds__(256) dense_trsm_kernel(const DenseJob *jobs, c
 // goes through LDS so that the read-modify-write of S runs down columns.
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(256) dense_gemm_kernel(const DenseJob *jobs, const int *active, int p)
+__global__ void __launch_bounds__(256, 2) dense_gemm_kernel(const DenseJob *jobs, const int *active, int p)
 {
     const DenseJob J = jobs[blockIdx.y];
     if (p >= J.T || (active && !active[J.mat])) return;
@@ -453,7 +495,7 @@ __global__ void __launch_bounds__(256) dense_gemm_kernel(const DenseJob *jobs, c
     // panel's TRSM launch follows this kernel.
     if (tile == 0) {
         __syncthreads();
-        diag_block(J, q0, Cs, kBsLd);
+        diag_block(J, q0, Cs, kBsLd, lds);
     }
 }
 
@@ -672,7 +714,7 @@ hipError_t dense_set_lds()
 hipError_t launch_dense_lu(const DenseJob *jobs, int njobs, int maxT, const int *active, hipStream_t st)
 {
     if (maxT <= 0) return hipSuccess;
-    hipLaunchKernelGGL(dense_diag_kernel, dim3(njobs), dim3(256), 0, st, jobs, active, 0);
+    hipLaunchKernelGGL(dense_diag_kernel, dim3(njobs), dim3(256), kDiagLds, st, jobs, active, 0);
     for (int p = 0; p < maxT; p += kNb) {
         const int rem = maxT - p - min(kNb, maxT - p);
         if (rem > 0) {
