@@ -10,12 +10,17 @@
 //
 // Refactorization (fixed pivot order; no pivot search, as in the sparse part):
 //   the persistent refactor kernel applies every column outside the block and
-//   leaves S = A22 - L21 U12 (fp64, column-major T x T).  Then, for each
-//   64-column panel p:
-//     dense_diag_kernel   S[p:p+64, p:p+64] = L11 \ U11, and L11^-1, U11^-1
-//     dense_trsm_kernel   L21 = A21 U11^-1 (64-row tiles) and
+//   leaves S = A22 - L21 U12 (fp64, column-major T x T).  Then, for the
+//   64-column panels p:
+//     dense_diag_kernel   (panel 0) S[p:p+64, p:p+64] = L11 \ U11, and
+//                         L11^-1, U11^-1
+//     dense_trsm_kernel   (panel 0) L21 = A21 U11^-1 (64-row tiles) and
 //                         U12 = L11^-1 A12 (64-column tiles), as tile products
-//     dense_gemm_kernel   S22 -= L21 U12 (64x64 tiles, fp64 FMA, 4x4 per thread)
+//     dense_gemm_kernel   S22 -= L21 U12 (64x64 tiles, fp64 matrix cores); its
+//                         first tile then factors the next panel's diagonal
+//                         block, and the next panel's L21 / U12 tiles wait
+//                         for it and multiply by its inverses, so each later
+//                         panel costs one launch
 //   dense_finish_kernel  applies the pivot test (|pivot| >= ptol max_i |x_i|,
 //   i.e. ptol * max |L(:,c)| <= 1), scatters the block into the sparse factor
 //   arrays and writes a dense copy in storage precision for the solves.
@@ -250,8 +255,9 @@ __device__ __forceinline__ void diag_block(const DenseJob &J, int p, const doubl
         for (int bb = 0; bb < 4; bb++) {
             const int j = s + 16 * bb;
             if (i < b && j < b) Sp[(size_t)j * ld + i] = A[a][bb];
-            W[j * kNb + i] = i > j ? X[a][bb] : (i == j ? 1.0 : 0.0);
-            W[kNb * kNb + j * kNb + i] = i <= j ? Y[a][bb] : 0.0;
+            // W is read by other workgroups of the same update launch
+            st_coherent(&W[j * kNb + i], i > j ? X[a][bb] : (i == j ? 1.0 : 0.0));
+            st_coherent(&W[kNb * kNb + j * kNb + i], i <= j ? Y[a][bb] : 0.0);
         }
     }
     // column maxima of |L| inside the block: the 16 threads of one s are
@@ -275,6 +281,7 @@ __global__ void __launch_bounds__(256) dense_diag_kernel(const DenseJob *jobs, c
     extern __shared__ double lds[];
     const DenseJob J = jobs[blockIdx.x];
     if (p >= J.T || (active && !active[J.mat])) return;
+    if (threadIdx.x == 0) st_relaxed(J.flag, 0);      // no later panel final yet (read by the next launches)
     diag_block(J, p, J.S + (size_t)p * J.T + p, (size_t)J.T, lds);
 }
 
@@ -398,7 +405,65 @@ __global__ void __launch_bounds__(256) dense_trsm_kernel(const DenseJob *jobs, c
 // goes through LDS so that the read-modify-write of S runs down columns.
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(256, 2) dense_gemm_kernel(const DenseJob *jobs, const int *active, int p)
+// acc = As * Bs over k < kmax (a multiple of 4), As[k*64 + i], Bs[k*65 + j]
+__device__ __forceinline__ void mfma_tile(const double *As, const double *Bs, int kmax, v4d (&acc)[2][2])
+{
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    const int rb = (w & 1) * 32, cb = (w >> 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) acc[x][y] = (v4d){0.0, 0.0, 0.0, 0.0};
+    for (int k4 = 0; k4 < kmax; k4 += 4) {
+        const int k = k4 + lk;
+        double av[2], bv[2];
+#pragma unroll
+        for (int x = 0; x < 2; x++) av[x] = As[k * 64 + rb + 16 * x + li];
+#pragma unroll
+        for (int y = 0; y < 2; y++) bv[y] = Bs[k * kBsLd + cb + 16 * y + li];
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 2; y++) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], acc[x][y], 0, 0, 0);
+    }
+}
+
+// The product to Cs[j*65 + i] (the caller has finished reading As / Bs if
+// they overlap Cs).
+__device__ __forceinline__ void acc_to_lds(double *Cs, const v4d (&acc)[2][2])
+{
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    const int rb = (w & 1) * 32, cb = (w >> 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = rb + 16 * x + 4 * r + lk;
+                const int j = cb + 16 * y + li;
+                Cs[j * kBsLd + i] = acc[x][y][r];
+            }
+}
+
+// Update launch for panel p (columns p .. q0 - 1, q0 = p + 64): tile (ti, tj)
+// of the trailing matrix from (q0, q0).  Tiles go in the order
+//   (0, 0), the next panel's diagonal block: updated, then factored here
+//        (look-ahead, while the other tiles run), W written agent-coherent,
+//        and J.flag set to q0 / 64;
+//   the next panel's edge tiles, column (ti, 0) and row (0, tj): updated,
+//        then, once J.flag says the diagonal block is factored, turned into
+//        L21 = A21 U11^-1 and U12 = L11^-1 A12 with the inverses in W;
+//   the interior tiles (ti, tj >= 1): updated.
+// An edge tile only waits on tile 0 of its job, which comes first in the
+// grid (dispatch order), so it is resident or done.  The wait is bounded
+// and raises the abort word if it gives up.
+__global__ void __launch_bounds__(256, 2) dense_gemm_kernel(const DenseJob *jobs, const int *active, int p,
+                                                            int *abort_word)
 {
     const DenseJob J = jobs[blockIdx.y];
     if (p >= J.T || (active && !active[J.mat])) return;
@@ -409,8 +474,19 @@ __global__ void __launch_bounds__(256, 2) dense_gemm_kernel(const DenseJob *jobs
     const int nt = (rem + 63) / 64;
     const int tile = blockIdx.x;
     if (tile >= nt * nt) return;
-    const int r0 = q0 + (tile % nt) * 64;
-    const int c0 = q0 + (tile / nt) * 64;
+    const int ni = nt - 1;
+    int ti = 0, tj = 0;
+    if (tile > 2 * ni) {
+        ti = 1 + (tile - 1 - 2 * ni) % ni;
+        tj = 1 + (tile - 1 - 2 * ni) / ni;
+    } else if (tile > ni) {
+        tj = tile - ni;
+    } else {
+        ti = tile;
+    }
+    const int edge = tile == 0 ? 0 : tj == 0 ? 1 : ti == 0 ? 2 : 0;   // 1: L21 tile, 2: U12 tile
+    const int r0 = q0 + ti * 64;
+    const int c0 = q0 + tj * 64;
     extern __shared__ double lds[];
     double *As = lds;                                  // As[k*64 + i] = L21(r0+i, p+k)
     double *Bs = lds + kNb * 64;                       // Bs[k*65 + j] = U12(p+k, c0+j)
@@ -439,38 +515,11 @@ __global__ void __launch_bounds__(256, 2) dense_gemm_kernel(const DenseJob *jobs
         }
     }
     __syncthreads();
-    const int lane = tid & 63, w = tid >> 6;
-    const int rb = (w & 1) * 32, cb = (w >> 1) * 32;
-    const int li = lane & 15, lk = lane >> 4;
     v4d acc[2][2];
-#pragma unroll
-    for (int x = 0; x < 2; x++)
-#pragma unroll
-        for (int y = 0; y < 2; y++) acc[x][y] = (v4d){0.0, 0.0, 0.0, 0.0};
-    for (int k4 = 0; k4 < b4; k4 += 4) {
-        const int k = k4 + lk;
-        double av[2], bv[2];
-#pragma unroll
-        for (int x = 0; x < 2; x++) av[x] = As[k * 64 + rb + 16 * x + li];
-#pragma unroll
-        for (int y = 0; y < 2; y++) bv[y] = Bs[k * kBsLd + cb + 16 * y + li];
-#pragma unroll
-        for (int x = 0; x < 2; x++)
-#pragma unroll
-            for (int y = 0; y < 2; y++) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], acc[x][y], 0, 0, 0);
-    }
+    mfma_tile(As, Bs, b4, acc);
     __syncthreads();                                   // As/Bs reads done: reuse LDS for the product
     double *Cs = lds;                                  // Cs[j*65 + i]
-#pragma unroll
-    for (int x = 0; x < 2; x++)
-#pragma unroll
-        for (int y = 0; y < 2; y++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int i = rb + 16 * x + 4 * r + lk;
-                const int j = cb + 16 * y + li;
-                Cs[j * kBsLd + i] = acc[x][y][r];
-            }
+    acc_to_lds(Cs, acc);
     __syncthreads();
     double cv[16];
 #pragma unroll
@@ -486,16 +535,76 @@ __global__ void __launch_bounds__(256, 2) dense_gemm_kernel(const DenseJob *jobs
         const int j = e >> 6, i = e & 63;
         const int rr = r0 + i, c = c0 + j;
         const double v = cv[u] - Cs[j * kBsLd + i];
-        if (rr < J.T && c < J.T) J.S[(size_t)c * ld + rr] = v;
+        cv[u] = v;
+        if (!edge && rr < J.T && c < J.T) J.S[(size_t)c * ld + rr] = v;
         if (tile == 0) Cs[j * kBsLd + i] = v;           // element read by this thread alone above
     }
-    // Look-ahead: tile 0 is the next panel's diagonal block, now final (every
-    // earlier panel's update has reached it), so this workgroup factors it
-    // while the other tiles of this update are still running; the next
-    // panel's TRSM launch follows this kernel.
     if (tile == 0) {
+        // the next panel's diagonal block is final (every earlier panel's
+        // update has reached it): factor it while the other tiles run
         __syncthreads();
         diag_block(J, q0, Cs, kBsLd, lds);
+        store_complete();                              // this wave's W stores acknowledged
+        __syncthreads();
+        if (tid == 0) st_relaxed(J.flag, q0 / kNb);
+        return;
+    }
+    if (!edge) return;
+    // Edge tile of the next panel: the updated tile in LDS as the operand it
+    // will be (L21: left, As layout; U12: right, Bs layout), then the wait
+    // for the diagonal block's inverses.
+    __syncthreads();                                   // Cs reads done
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int e = tid + 256 * u;
+        const int j = e >> 6, i = e & 63;
+        if (edge == 1) As[j * 64 + i] = cv[u];         // left(i, k = j)
+        else Bs[i * kBsLd + j] = cv[u];                // right(k = i, j)
+    }
+    int ok = 1;
+    if (tid < kWave)
+        ok = wave_wait(tid == 0 ? J.flag : nullptr, q0 / kNb, abort_word, nullptr, -1, kSolveBackoff);
+    if (!__syncthreads_and(ok)) return;
+    const double *W = J.W;
+    {
+        double wv[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) wv[u] = ld_coherent_val(&W[(edge == 1 ? kNb * kNb : 0) + tid + 256 * u]);
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int e = tid + 256 * u;
+            const int j = e >> 6, k = e & 63;
+            if (edge == 1) Bs[k * kBsLd + j] = wv[u];  // U11^-1(k, j)
+            else As[e] = wv[u];                        // L11^-1(i = k, j): As[j*64 + i]
+        }
+    }
+    __syncthreads();
+    mfma_tile(As, Bs, kNb, acc);
+    __syncthreads();
+    acc_to_lds(Cs, acc);
+    __syncthreads();
+    // the panel's columns (L21) or rows (U12) of this tile, and |L| maxima
+    double am[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int e = tid + 256 * u;
+        const int j = e >> 6, i = e & 63;
+        const int rr = r0 + i, c = c0 + j;
+        const double v = Cs[j * kBsLd + i];
+        const bool in = rr < J.T && c < J.T;
+        if (in) J.S[(size_t)c * ld + rr] = v;
+        am[u] = in ? fabs(v) : 0.0;
+    }
+    if (edge == 1) {
+        // column j = e >> 6 = tid / 64 + 4u: a wave holds 64 rows of one column
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            double m = am[u];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, kWave));
+            const int j = (tid >> 6) + 4 * u;
+            if ((tid & 63) == 0) atomicMax(&J.colmax[q0 + j], dbits(m));
+        }
     }
 }
 
@@ -711,7 +820,8 @@ hipError_t dense_set_lds()
 // block is factored by tile 0 of the preceding trailing update.  For a job
 // whose T is below maxT the update kernel returns before its tiles once
 // p + 64 >= T, so each of its panels is factored exactly once.
-hipError_t launch_dense_lu(const DenseJob *jobs, int njobs, int maxT, const int *active, hipStream_t st)
+hipError_t launch_dense_lu(const DenseJob *jobs, int njobs, int maxT, const int *active, int *abort_word,
+                           hipStream_t st)
 {
     if (maxT <= 0) return hipSuccess;
     hipLaunchKernelGGL(dense_diag_kernel, dim3(njobs), dim3(256), kDiagLds, st, jobs, active, 0);
@@ -719,9 +829,10 @@ hipError_t launch_dense_lu(const DenseJob *jobs, int njobs, int maxT, const int 
         const int rem = maxT - p - min(kNb, maxT - p);
         if (rem > 0) {
             const int nt = (rem + kNb - 1) / kNb;
-            hipLaunchKernelGGL(dense_trsm_kernel, dim3(2 * nt, njobs), dim3(256), kGemmLds, st, jobs, active, p);
+            if (p == 0)
+                hipLaunchKernelGGL(dense_trsm_kernel, dim3(2 * nt, njobs), dim3(256), kGemmLds, st, jobs, active, p);
             hipLaunchKernelGGL(dense_gemm_kernel, dim3(nt * nt, njobs), dim3(256), kGemmLds, st, jobs,
-                               active, p);
+                               active, p, abort_word);
         }
     }
     return hipGetLastError();

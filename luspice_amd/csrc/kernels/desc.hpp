@@ -92,6 +92,7 @@ struct DenseJob {
     double *S;              // T x T, column-major, leading dimension T
     double *W;              // 2 x 64 x 64 workspace: inverses of the current diagonal block
     unsigned long long *colmax;
+    int *flag;              // panel whose diagonal block (and W) is final, during an update launch
     int T;
     int mat;                // matrix index (descriptor, mstat, active mask)
 };

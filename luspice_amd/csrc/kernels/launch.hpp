@@ -98,7 +98,8 @@ hipError_t launch_fill_unset(const void *mats, const int *active, int nmat, long
 
 // dense trailing blocks (dense.hip)
 hipError_t dense_set_lds();
-hipError_t launch_dense_lu(const DenseJob *jobs, int njobs, int maxT, const int *active, hipStream_t st);
+hipError_t launch_dense_lu(const DenseJob *jobs, int njobs, int maxT, const int *active, int *abort_word,
+                           hipStream_t st);
 template <typename T>
 hipError_t launch_dense_finish(const DenseJob *jobs, int njobs, int maxT, const void *mats,
                                const int *active, double ptol, double stol, hipStream_t st);

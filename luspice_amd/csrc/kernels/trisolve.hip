@@ -68,7 +68,8 @@ __device__ bool tri_forward(const TriDesc<T> &M, int s, int e, double *y, int *f
                     if (h0 + h < qn) acc = fma(to_f64(v[h]), bcast_f64(yq, h0 + h), acc);
             }
         }
-        if (!wave_wait(lane < bw ? &flags[row] : nullptr, -epoch, abort_word)) return false;
+        if (!wave_wait(lane < bw ? &flags[row] : nullptr, -epoch, abort_word, nullptr, -1, kSolveBackoff))
+            return false;
         double p = lane < bw ? ld_coherent_val(&y[row]) : 0.0;
         p -= acc;
         const int lb = dp[rr] + 1;
@@ -129,7 +130,8 @@ __device__ bool tri_backward(const TriDesc<T> &M, int s, int e, double *z, doubl
         const double rd = 1.0 / to_f64(M.diag[rr]);
         const int dpc = dp[rr];
         const int run = ru[rr];
-        if (!wave_wait(lane < bw ? &flags[row] : nullptr, -epoch, abort_word)) return false;
+        if (!wave_wait(lane < bw ? &flags[row] : nullptr, -epoch, abort_word, nullptr, -1, kSolveBackoff))
+            return false;
         double num = lane < bw ? ld_coherent_val(&z[row]) : 0.0;
         num -= acc;
         double zt;
@@ -254,7 +256,8 @@ __device__ void tri_tpacket(const TriLaunch &L, const TriDesc<T> &M, int o, long
             const int off = on && q < l ? l - q : 0;
             tv[q] = ld_plain(gp(M.val), (uint32_t)(dp[j] + off));
         }
-        if (!wave_wait(on ? &flags[row] : nullptr, -epoch, L.abort_word)) return;
+        if (!wave_wait(on ? &flags[row] : nullptr, -epoch, L.abort_word, nullptr, -1, kSolveBackoff))
+            return;
         double *y = L.y + vo;
         double p = on ? ld_coherent_val(&y[row]) : 0.0;
 #pragma unroll
@@ -280,7 +283,8 @@ __device__ void tri_tpacket(const TriLaunch &L, const TriDesc<T> &M, int o, long
             tv[q] = ld_plain(gp(M.val), (uint32_t)(dp[j] - (in[q] ? q - l : 0)));
         }
         const double rd = on ? 1.0 / to_f64(M.diag[row]) : 1.0;
-        if (!wave_wait(on ? &flags[row] : nullptr, -epoch, L.abort_word)) return;
+        if (!wave_wait(on ? &flags[row] : nullptr, -epoch, L.abort_word, nullptr, -1, kSolveBackoff))
+            return;
         double *z = L.z + vo;
         double num = on ? ld_coherent_val(&z[row]) : 0.0;
 #pragma unroll

@@ -620,7 +620,8 @@ int Batch::upload_all()
         int ntail = 0;
         for (const auto &C : circ_) ntail += C.plan.tail_T > 0;
         d_tailW_ = (double *)dalloc((size_t)std::max(ntail, 1) * 2 * 64 * 64 * sizeof(double));
-        if (!d_tailS_ || !d_colmax_ || !d_tailW_) return LSP_E_ALLOC;
+        d_dflag_ = upload(std::vector<int>(std::max(ntail, 1), 0));
+        if (!d_tailS_ || !d_colmax_ || !d_tailW_ || !d_dflag_) return LSP_E_ALLOC;
         for (int m = 0; m < nc; m++) {
             const CircuitHost &C = circ_[m];
             const LspPlan &P = C.plan;
@@ -631,6 +632,7 @@ int Batch::upload_all()
                 J.S = d_tailS_ + C.o_t2;
                 J.W = d_tailW_ + jobs.size() * 2 * 64 * 64;
                 J.colmax = d_colmax_ + C.o_t;
+                J.flag = d_dflag_ + jobs.size();
                 J.T = P.tail_T;
                 J.mat = m;
                 jobs.push_back(J);
@@ -976,7 +978,7 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
     rc = check(launch_refactor<T>(L, fblocks_, stream_), "refactor launch");
     if (!capturing_) (void)hipEventRecord(ev_[1], stream_);      // the persistent kernel alone
     if (!rc && njobs_) {
-        rc = check(launch_dense_lu(d_djobs_, njobs_, max_tail_T_, d_active, stream_), "dense lu");
+        rc = check(launch_dense_lu(d_djobs_, njobs_, max_tail_T_, d_active, d_abort_, stream_), "dense lu");
         if (!rc)
             rc = check(launch_dense_finish<T>(d_djobs_, njobs_, max_tail_T_, fs.fdesc, d_active,
                                               opt_.refactor_ptol, L.stol, stream_), "dense finish");

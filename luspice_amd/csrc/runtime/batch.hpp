@@ -106,6 +106,7 @@ struct DeviceArrays {
         *d_tail_ctr_ = nullptr, *d_tails_ = nullptr;
     double *d_tailS_ = nullptr;
     unsigned long long *d_colmax_ = nullptr;
+    int *d_dflag_ = nullptr;                   // per dense job: the last diagonal block factored
     DenseJob *d_djobs_ = nullptr;
     int njobs_ = 0, max_tail_T_ = 0;
     long nt_tot_ = 0, nt2_tot_ = 0, ntb_tot_ = 0;
