@@ -279,31 +279,38 @@ __device__ __forceinline__ int wave_wait_prefix(const int *flag, int epoch, int 
     }
 }
 
-// wave_wait over up to N flags per lane (null = nothing to wait for): each
-// round issues the outstanding polls together, so N chunks of a long
-// dependency list cost one round trip instead of N.
-//
 // The solves back off only to two sleeps (~0.2 us) between polls: their
 // waits are short and their chains long (c1 pair 1.346 -> 1.313 ms against
 // the factorization's ~0.9 us cap, no change on a 512-circuit batch).
 constexpr int kSolveBackoff = 1;
 
+// Wave-collective wait on values instead of flags: every lane loads the
+// values its non-null pointers name (agent-coherent) until none of them holds
+// the unset marker, and returns them in v (0 for null pointers).  Each round
+// issues the outstanding loads together, so N chunks of a long dependency
+// list cost one round trip instead of N.  A value is
+// written once per solve, as a whole, over the marker, so the first value
+// that is not the marker is the final one: the producer needs neither its
+// store acknowledgement nor a flag, and the consumer has the data with the
+// poll that sees it.  Returns false when the wait was abandoned.
 template <int N>
-__device__ __forceinline__ bool wave_wait_n(const int *const (&flag)[N], int epoch, int *abort_word)
+__device__ __forceinline__ bool wave_wait_values(const double *const (&p)[N], double (&v)[N], int *abort_word)
 {
     unsigned ok = 0;
 #pragma unroll
-    for (int g = 0; g < N; g++)
-        if (flag[g] == nullptr) ok |= 1u << g;
+    for (int g = 0; g < N; g++) {
+        v[g] = 0.0;
+        if (p[g] == nullptr) ok |= 1u << g;
+    }
     const unsigned full = (1u << N) - 1;
     int backoff = 0;
     for (int round = 0;; round++) {
-        int seen[N];
-#pragma unroll
-        for (int g = 0; g < N; g++) seen[g] = (ok >> g) & 1 ? epoch : ld_relaxed(flag[g]);
 #pragma unroll
         for (int g = 0; g < N; g++)
-            if (seen[g] == epoch) ok |= 1u << g;
+            if (!((ok >> g) & 1)) v[g] = ld_coherent_val(p[g]);
+#pragma unroll
+        for (int g = 0; g < N; g++)
+            if (!((ok >> g) & 1) && !Unset<double>::is(v[g])) ok |= 1u << g;
         if (__all(ok == full)) return true;
         if ((round & 15) == 15 && aborted(abort_word)) return false;
         if (round > kRoundLimit) {
@@ -313,6 +320,14 @@ __device__ __forceinline__ bool wave_wait_n(const int *const (&flag)[N], int epo
         for (int s = 0; s <= backoff; s++) __builtin_amdgcn_s_sleep(4);
         if (backoff < kSolveBackoff) backoff = 2 * backoff + 1;
     }
+}
+__device__ __forceinline__ bool wave_wait_value(const double *p, double &v, int *abort_word)
+{
+    const double *pp[1] = {p};
+    double vv[1];
+    const bool ok = wave_wait_values<1>(pp, vv, abort_word);
+    v = vv[0];
+    return ok;
 }
 
 // Makes data published before the observed flags visible to this wave.

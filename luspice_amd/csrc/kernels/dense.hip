@@ -29,13 +29,14 @@
 //
 // Triangular solves: rows of the block are solved after the rows above it
 // (forward) and before them (backward):
-//   tail_fwd_gather_kernel  y_T = b_T - L21 y  (one wave per row)
-//   tail_trsv_kernel<lower> y_T = L22^-1 y_T
-//   tail_trsv_kernel<upper> z_T = U22^-1 y_T, x[q] = z, row flags for the
-//                           sparse backward solve
+//   tail_fwd_gather_kernel  yp_T = b_T - L21 y  (one wave per row)
+//   tail_trsv_kernel<lower> y_T = L22^-1 yp_T
+//   tail_trsv_kernel<upper> z_T = U22^-1 y_T, x[q] = z
 // A trsv launch has one workgroup per 64-row block; blocks take tickets in
-// solve order and wait (bounded, epoch flags) only on blocks ticketed
-// earlier, accumulating each finished block's contribution as it arrives.
+// solve order and wait (bounded) only on blocks ticketed earlier,
+// accumulating each finished block's contribution as it arrives.  Blocks
+// hand over their values as the sparse solves do: y and z are unset before
+// the solve and a waiting block polls the values themselves.
 #include "common.hpp"
 #include "launch.hpp"
 
@@ -663,11 +664,12 @@ __global__ void __launch_bounds__(64) dense_finish_kernel(const DenseJob *jobs, 
 }
 
 // ---------------------------------------------------------------------------
-// Forward solve, rows of the block: y[k0 + r] = b[rowat] - L(k0 + r, :k0) y.
+// Forward solve, rows of the block: yp[k0 + r] = b[rowat] - L(k0 + r, :k0) y.
 template <typename T>
 __global__ void __launch_bounds__(256) tail_fwd_gather_kernel(const void *mats, const int *tails,
                                                               const long *voff, const int *active,
-                                                              const double *b, double *y, const double *rs)
+                                                              const double *b, const double *y, double *yp,
+                                                              const double *rs)
 {
     const int mi = tails[blockIdx.y];
     if (active && !active[mi]) return;
@@ -683,27 +685,26 @@ __global__ void __launch_bounds__(256) tail_fwd_gather_kernel(const void *mats, 
     acc = wave_sum(acc);
     if (lane == 0) {
         const int ro = M.rowat[i];
-        y[vo + i] = (rs ? b[vo + ro] * rs[vo + ro] : b[vo + ro]) - acc;
+        yp[vo + i] = (rs ? b[vo + ro] * rs[vo + ro] : b[vo + ro]) - acc;
     }
 }
 
 // Dense triangular solve of the block rows (see the file comment).
-//   lower: y_T <- L22^-1 y_T (unit diagonal), in place
-//   upper: z_T <- U22^-1 y_T, x[q] = z, bdone[row] = epoch of the sparse solve
+//   lower: y_T <- L22^-1 yp_T (unit diagonal)
+//   upper: z_T <- U22^-1 y_T, x[q] = z (the sparse backward solve, a later
+//          launch, reads these z values)
 // Waves 1..3 accumulate the contributions of the blocks solved before this
 // one: per block the 64 x 64 factor slab (one row per lane, in registers) is
-// loaded before its flag is awaited, so the wait overlaps the loads, and the
-// solved values arrive with one coherent load and are broadcast by readlane.
+// loaded before the block's values are awaited, so the wait overlaps the
+// loads; the polls return the solved values, broadcast by readlane.
 // Meanwhile wave 0 holds the diagonal block (and, upper, the reciprocals of
 // its diagonal), so after the last contribution only the 64-step solve and
 // the stores remain on the chain from block to block.
-constexpr int kTailBackoff = 1;    // few waiters: poll every ~0.1-0.2 us
-
 template <typename T, bool kLower>
 __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const int *tails,
                                                         const long *voff, const int *active,
-                                                        double *y, double *z, double *x,
-                                                        const int *bwd_epoch, int *abort_word, const double *cs)
+                                                        double *y, const double *yp, double *z, double *x,
+                                                        int *abort_word, const double *cs)
 {
     const int mi = tails[blockIdx.y];
     const TriDesc<T> M = ((const TriDesc<T> *)mats)[mi];
@@ -714,8 +715,6 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
     const bool live = !(active && !active[mi]);
     const int nblk = (M.tail_T + kNb - 1) / kNb;
     if (live) {
-        const int epoch = read_epoch(&M.tail_ctr[2]);            // block flags
-        const int bep = kLower ? 0 : read_epoch(bwd_epoch);     // row flags of the sparse solve
         if (threadIdx.x == 0) s_ticket = atomicAdd(&M.tail_ctr[0], 1);
         __syncthreads();
         const int t = uniform(s_ticket);
@@ -741,7 +740,7 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
                     rd = 1.0 / to_f64(M.tail_F[(size_t)rr * ld + rr]);
                     qo = M.q[k0 + rr];
                 }
-                yb = y[vo + k0 + rr];
+                yb = kLower ? yp[vo + k0 + rr] : y[vo + k0 + rr];
             } else {
                 for (int s = wv - 1; s < t && ok; s += 3) {
                     const int jb = kLower ? s : nblk - 1 - s;
@@ -750,10 +749,9 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
                     T fv[kNb];
 #pragma unroll
                     for (int c = 0; c < kNb; c++) fv[c] = M.tail_F[(size_t)(c0 + min(c, bj - 1)) * ld + rr];
-                    ok = wave_wait(lane == 0 ? &M.tail_flag[jb] : nullptr, epoch, abort_word, nullptr, -1,
-                                   kTailBackoff);
+                    double sv;                                       // zero beyond bj
+                    ok = wave_wait_value(lane < bj ? &src[vo + k0 + c0 + lane] : nullptr, sv, abort_word);
                     if (!ok) break;
-                    const double sv = lane < bj ? ld_coherent_val(&src[vo + k0 + c0 + lane]) : 0.0;
 #pragma unroll
                     for (int c = 0; c < kNb; c++) {
                         const double yc = bcast(sv, c);              // zero beyond bj
@@ -787,9 +785,6 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
                         x[vo + qo] = cs ? tr * cs[vo + qo] : tr;
                     }
                 }
-                store_complete();
-                if (!kLower && lane < bi) st_relaxed(&M.bdone[k0 + r0 + lane], bep);
-                if (lane == 0) st_relaxed(&M.tail_flag[ib], epoch);
             }
         }
         // the last workgroup of this matrix re-arms its ticket counter
@@ -798,7 +793,6 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
             if (fin == (int)gridDim.x - 1) {
                 st_relaxed(&M.tail_ctr[0], 0);
                 st_relaxed(&M.tail_ctr[1], 0);
-                advance_epoch(&M.tail_ctr[2], epoch);
             }
         }
     }
@@ -852,30 +846,30 @@ template hipError_t launch_dense_finish<__half>(const DenseJob *, int, int, cons
 
 template <typename T>
 hipError_t launch_tail_solve(const void *mats, const int *tails, int ntails, int maxT, const long *voff,
-                             const int *active, const double *b, double *y, double *z, double *x,
-                             bool forward, const int *bwd_epoch, int *abort_word, const double *rs,
-                             const double *cs, hipStream_t st)
+                             const int *active, const double *b, double *y, double *yp, double *z,
+                             double *x, bool forward, int *abort_word, const double *rs, const double *cs,
+                             hipStream_t st)
 {
     const int nblk = (maxT + kNb - 1) / kNb;
     if (forward) {
         hipLaunchKernelGGL(tail_fwd_gather_kernel<T>, dim3((maxT + 3) / 4, ntails), dim3(256), 0, st, mats,
-                           tails, voff, active, b, y, rs);
+                           tails, voff, active, b, y, yp, rs);
         hipLaunchKernelGGL((tail_trsv_kernel<T, true>), dim3(nblk, ntails), dim3(256), 0, st, mats, tails,
-                           voff, active, y, z, x, bwd_epoch, abort_word, cs);
+                           voff, active, y, yp, z, x, abort_word, cs);
     } else {
         hipLaunchKernelGGL((tail_trsv_kernel<T, false>), dim3(nblk, ntails), dim3(256), 0, st, mats, tails,
-                           voff, active, y, z, x, bwd_epoch, abort_word, cs);
+                           voff, active, y, yp, z, x, abort_word, cs);
     }
     return hipGetLastError();
 }
 template hipError_t launch_tail_solve<float>(const void *, const int *, int, int, const long *, const int *,
-                                             const double *, double *, double *, double *, bool, const int *, int *,
-                                             const double *, const double *, hipStream_t);
+                                              const double *, double *, double *, double *, double *, bool,
+                                              int *, const double *, const double *, hipStream_t);
 template hipError_t launch_tail_solve<double>(const void *, const int *, int, int, const long *, const int *,
-                                              const double *, double *, double *, double *, bool, const int *, int *,
-                                              const double *, const double *, hipStream_t);
+                                              const double *, double *, double *, double *, double *, bool,
+                                              int *, const double *, const double *, hipStream_t);
 template hipError_t launch_tail_solve<__half>(const void *, const int *, int, int, const long *, const int *,
-                                              const double *, double *, double *, double *, bool, const int *, int *,
-                                              const double *, const double *, hipStream_t);
+                                              const double *, double *, double *, double *, double *, bool,
+                                              int *, const double *, const double *, hipStream_t);
 
 }  // namespace lsp

@@ -67,7 +67,6 @@ struct TriDesc {
     const T *lrow, *urow, *diag;
     const int *rowat;       // position -> original row (P)
     const int *q;           // position -> original column (Q)
-    int *fdone, *bdone;     // per-row epoch flags
     // supernodes (rows below tail_k0)
     const T *val;           // column storage of L and U (FactorDesc::val)
     const int *dp;          // n diagonal slots (absolute)
@@ -79,8 +78,7 @@ struct TriDesc {
     // dense trailing block: rows [tail_k0, n) are solved with tail_F
     const T *tail_F;        // tail_T^2, column-major, unit-L below / U on and above
     const int *tail_lcut;   // tail_T: end of the sparse part of L row tail_k0 + r
-    int *tail_flag;         // ceil(tail_T / 64) block flags (epoch-tagged)
-    int *tail_ctr;          // 4 ints: block ticket counter, exit counter, epoch, pad
+    int *tail_ctr;          // 4 ints: block ticket counter, exit counter, pad
     int n;
     int tail_k0;
     int tail_T;

@@ -37,12 +37,12 @@ struct TriLaunch {
     const int *active;      // per-matrix mask (nullptr = all)
     const double *b;        // right-hand sides (original row order)
     double *x;              // solutions (original column order)
-    double *y, *z;          // work vectors (position order)
+    double *y, *z;          // work vectors (position order), unset before the solve
+    double *yp, *zp;        // partial sums of supernode rows (position order), unset before the solve
     const double *rs, *cs;  // optional row / column scales (Ruiz): b_s = R b, x = C z
     const int *n_active;    // optional: when *n_active == 0 no circuit is active, tasks are skipped
     long long *timeline;    // optional diagnostic, 2 per task: ticket granted, task finished
     int ntask;
-    int *epoch;             // device epoch counter (see FactorLaunch)
     int *ctr;
     int *abort_word;
     int total_waves;
@@ -105,13 +105,14 @@ hipError_t launch_dense_finish(const DenseJob *jobs, int njobs, int maxT, const 
                                const int *active, double ptol, double stol, hipStream_t st);
 template <typename T>
 hipError_t launch_tail_solve(const void *mats, const int *tails, int ntails, int maxT, const long *voff,
-                             const int *active, const double *b, double *y, double *z, double *x,
-                             bool forward, const int *bwd_epoch, int *abort_word, const double *rs,
-                             const double *cs, hipStream_t st);
+                             const int *active, const double *b, double *y, double *yp, double *z,
+                             double *x, bool forward, int *abort_word, const double *rs, const double *cs,
+                             hipStream_t st);
 
 // trisolve.hip
 template <typename T>
 hipError_t launch_trisolve_half(const TriLaunch &L, bool forward, int blocks, hipStream_t st);
+hipError_t launch_solve_unset(const TriLaunch &L, long n, hipStream_t st);
 
 // spmv.hip
 template <typename V>

@@ -7,20 +7,25 @@
 // fp64 accumulation, pb = P b, x[q[k]] = z[k]).  The reference's
 // level-scheduled form (src/mp/fusion_bench.cu:80-116) runs one block with a
 // barrier per level; here rows are tasks in a global queue sorted by level,
-// each row waits only for the rows it reads (epoch flags), and every CU works
-// on the front of the dependency DAG at once.
+// each row waits only for the values it reads, and every CU works on the
+// front of the dependency DAG at once.
 //
 //   forward   y[i] = b[rowat[i]] - sum_{j<i} L(i,j) y[j]
 //   backward  z[i] = (y[i] - sum_{j>i} U(i,j) z[j]) / U(i,i),  x[q[i]] = z[i]
 //
 // One wave per row: lanes stride over the row's entries (coalesced index and
-// value loads), wait on their own dependency flags in parallel, and the wave
+// value loads), wait on their own dependency values in parallel, and the wave
 // sums in fp64 with a DPP reduction.  Rows of a supernode (consecutive
 // columns whose L patterns nest, so their diagonal block is dense) first
 // gather only their entries outside it, in parallel; one task per supernode
 // then solves the dense block in registers, so the solve has one dependency
-// level per supernode instead of one per row.  Solved values are stored agent-coherent
-// and flagged once acknowledged; readers load them agent-coherent, so no L2
+// level per supernode instead of one per row.
+//
+// Hand-off by data: before a solve every work value (y, z and the partial
+// sums yp, zp) is set to the unset marker; a task stores each value once,
+// agent-coherent, and its readers load the values they need until none is
+// the marker (wave_wait_values).  No flag, no store acknowledgement on the
+// producer's side, and the poll that sees a value already holds it.  No L2
 // write-back or invalidate is needed (see common.hpp).
 #include "common.hpp"
 #include "launch.hpp"
@@ -32,13 +37,13 @@ constexpr int kGather = 4;              // 64-entry chunks of a row per wait rou
 
 constexpr int kSolveBatch = 32;         // dense-part values in flight per pass of a triangle task
 
-// Forward triangle task of supernode D = [s, e): its rows hold partial sums
-// p_i = b_i - sum over L entries left of s (flag -epoch); y_D = L_DD^-1 p in
+// Forward triangle task of supernode D = [s, e): its rows' partial sums
+// p_i = b_i - sum over L entries left of s are in yp; y_D = L_DD^-1 p in
 // 64-row blocks, each block first subtracting the earlier blocks' rows
 // (L(i, j) = val[dp[j] + i - j]) and then solving its unit-lower triangle in
 // registers.  Returns false when a wait was abandoned.
 template <typename T>
-__device__ bool tri_forward(const TriDesc<T> &M, int s, int e, double *y, int *flags, int epoch, int *abort_word)
+__device__ bool tri_forward(const TriDesc<T> &M, int s, int e, double *y, const double *yp, int *abort_word)
 {
     const int lane = lane_id();
     const LSP_GLOBAL int *dp = gp(M.dp);
@@ -68,9 +73,8 @@ __device__ bool tri_forward(const TriDesc<T> &M, int s, int e, double *y, int *f
                     if (h0 + h < qn) acc = fma(to_f64(v[h]), bcast_f64(yq, h0 + h), acc);
             }
         }
-        if (!wave_wait(lane < bw ? &flags[row] : nullptr, -epoch, abort_word, nullptr, -1, kSolveBackoff))
-            return false;
-        double p = lane < bw ? ld_coherent_val(&y[row]) : 0.0;
+        double p;
+        if (!wave_wait_value(lane < bw ? &yp[row] : nullptr, p, abort_word)) return false;
         p -= acc;
         const int lb = dp[rr] + 1;
         if (bw == 1) {
@@ -78,19 +82,18 @@ __device__ bool tri_forward(const TriDesc<T> &M, int s, int e, double *y, int *f
         else if (bw <= 25) p = run_triangle<24, false>(M.val, p, lb, 0, bw);
         else p = run_triangle<kTri, false>(M.val, p, lb, 0, bw);
         if (lane < bw) st_coherent(&y[row], p);
-        store_complete();
-        if (lane < bw) st_relaxed(&flags[row], epoch);
+        if (b0 + kWave < w) store_complete();          // the next block reads these back
     }
     return true;
 }
 
-// Backward triangle task of D = [s, e): partial sums q_i = y_i - sum over U
-// entries right of e (in z, flag -epoch); z_D = U_DD^-1 q in 64-row blocks
+// Backward triangle task of D = [s, e): its rows' partial sums q_i = y_i -
+// sum over U entries right of e are in zp; z_D = U_DD^-1 q in 64-row blocks
 // from the bottom, each block first subtracting the later blocks' rows
 // (U(i, j) = val[dp[j] - (j - i)] within column j's run ru[j], else 0).
 template <typename T>
-__device__ bool tri_backward(const TriDesc<T> &M, int s, int e, double *z, double *x, const double *cs,
-                             int *flags, int epoch, int *abort_word)
+__device__ bool tri_backward(const TriDesc<T> &M, int s, int e, double *z, const double *zp, double *x,
+                             const double *cs, int *abort_word)
 {
     const int lane = lane_id();
     const LSP_GLOBAL int *dp = gp(M.dp);
@@ -130,21 +133,19 @@ __device__ bool tri_backward(const TriDesc<T> &M, int s, int e, double *z, doubl
         const double rd = 1.0 / to_f64(M.diag[rr]);
         const int dpc = dp[rr];
         const int run = ru[rr];
-        if (!wave_wait(lane < bw ? &flags[row] : nullptr, -epoch, abort_word, nullptr, -1, kSolveBackoff))
-            return false;
-        double num = lane < bw ? ld_coherent_val(&z[row]) : 0.0;
+        double num;
+        if (!wave_wait_value(lane < bw ? &zp[row] : nullptr, num, abort_word)) return false;
         num -= acc;
         double zt;
         if (bw <= 8) zt = run_triangle_upper<8, false>(M.val, num, rd, dpc, run, bw);
         else if (bw <= 24) zt = run_triangle_upper<24, false>(M.val, num, rd, dpc, run, bw);
         else zt = run_triangle_upper<kWave, false>(M.val, num, rd, dpc, run, bw);
-        if (lane < bw) st_coherent(&z[row], zt);
-        store_complete();
         if (lane < bw) {
-            st_relaxed(&flags[row], epoch);
-            const int qo = M.q[row];                   // the solution, after the flag
+            st_coherent(&z[row], zt);
+            const int qo = M.q[row];
             x[qo] = cs ? zt * cs[qo] : zt;
         }
+        if (b1 - kWave > 0) store_complete();          // the next block reads these back
     }
     return true;
 }
@@ -155,7 +156,7 @@ __device__ bool tri_backward(const TriDesc<T> &M, int s, int e, double *z, doubl
 // rows' flags are polled together, each lane group sums with g-lane
 // shuffles, and every row is written as its own task would be.
 template <typename T, bool kForward>
-__device__ void tri_packet(const TriLaunch &L, const TriDesc<T> &M, int o, long vo, int *flags, int epoch)
+__device__ void tri_packet(const TriLaunch &L, const TriDesc<T> &M, int o, long vo)
 {
     const int lane = lane_id();
     const LSP_GLOBAL int *pk = gp(kForward ? M.fpk : M.bpk) + o;
@@ -180,6 +181,7 @@ __device__ void tri_packet(const TriLaunch &L, const TriDesc<T> &M, int o, long 
     const T *rv = kForward ? M.lrow : M.urow;
     double *y = L.y + vo;
     double *z = L.z + vo;
+    const double *src = kForward ? y : z;
     const bool lead = rc >= 0 && l == 0;
     // b (forward), y and the diagonal (backward) before the waits
     double pre = 0.0, dia = 1.0;
@@ -192,32 +194,26 @@ __device__ void tri_packet(const TriLaunch &L, const TriDesc<T> &M, int o, long 
             if (!part) dia = to_f64(M.diag[i]);
         }
     }
-    int jj[kTriPacketLane];
     T vv[kTriPacketLane];
-    const int *fl[kTriPacketLane];
+    const double *sp[kTriPacketLane];
 #pragma unroll
     for (int h = 0; h < kTriPacketLane; h++) {
         const int e = e0 + l + h * g;
         const bool on = e < e1;
-        jj[h] = on ? ci[e] : 0;
         vv[h] = on ? rv[e] : T(0);
-        fl[h] = on ? &flags[jj[h]] : nullptr;
+        sp[h] = on ? &src[ci[e]] : nullptr;
     }
-    if (!wave_wait_n<kTriPacketLane>(fl, epoch, L.abort_word)) return;
-    const double *src = kForward ? y : z;
     double sv[kTriPacketLane];
-#pragma unroll
-    for (int h = 0; h < kTriPacketLane; h++) sv[h] = fl[h] ? ld_coherent_val(&src[jj[h]]) : 0.0;
+    if (!wave_wait_values<kTriPacketLane>(sp, sv, L.abort_word)) return;
     double acc = 0.0;
 #pragma unroll
     for (int h = 0; h < kTriPacketLane; h++) acc += to_f64(vv[h]) * sv[h];
     for (int off = g >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
     const double r = kForward || part ? pre - acc : (pre - acc) / dia;
-    if (lead) st_coherent(kForward ? &y[i] : &z[i], r);
-    store_complete();
     if (lead) {
-        st_relaxed(&flags[i], part ? -epoch : epoch);
-        if (!kForward && !part) {                      // the solution, after the flag
+        if (part) st_coherent(kForward ? &L.yp[vo + i] : &L.zp[vo + i], r);
+        else st_coherent(kForward ? &y[i] : &z[i], r);
+        if (!kForward && !part) {                      // the solution
             const int qo = M.q[i];
             L.x[vo + qo] = L.cs ? r * L.cs[vo + qo] : r;
         }
@@ -232,7 +228,7 @@ __device__ void tri_packet(const TriLaunch &L, const TriDesc<T> &M, int o, long 
 constexpr int kTriPacketRows = 16;      // widest supernode in a triangle packet
 
 template <typename T, bool kForward>
-__device__ void tri_tpacket(const TriLaunch &L, const TriDesc<T> &M, int o, long vo, int *flags, int epoch)
+__device__ void tri_tpacket(const TriLaunch &L, const TriDesc<T> &M, int o, long vo)
 {
     const int lane = lane_id();
     const LSP_GLOBAL int *pk = gp(kForward ? M.fpk : M.bpk) + o;
@@ -256,10 +252,8 @@ __device__ void tri_tpacket(const TriLaunch &L, const TriDesc<T> &M, int o, long
             const int off = on && q < l ? l - q : 0;
             tv[q] = ld_plain(gp(M.val), (uint32_t)(dp[j] + off));
         }
-        if (!wave_wait(on ? &flags[row] : nullptr, -epoch, L.abort_word, nullptr, -1, kSolveBackoff))
-            return;
-        double *y = L.y + vo;
-        double p = on ? ld_coherent_val(&y[row]) : 0.0;
+        double p;
+        if (!wave_wait_value(on ? &L.yp[vo + row] : nullptr, p, L.abort_word)) return;
 #pragma unroll
         for (int q = 0; q < kTriPacketRows - 1; q++) {
             if (q < g - 1) {
@@ -267,9 +261,7 @@ __device__ void tri_tpacket(const TriLaunch &L, const TriDesc<T> &M, int o, long
                 if (on && q < l) p = fma(-to_f64(tv[q]), yq, p);
             }
         }
-        if (on) st_coherent(&y[row], p);
-        store_complete();
-        if (on) st_relaxed(&flags[row], epoch);
+        if (on) st_coherent(&L.y[vo + row], p);
     } else {
         // U(s + l, s + q) = val[dp[s + q] - (q - l)] for l < q within column
         // s + q's run ru, zero above it
@@ -283,10 +275,8 @@ __device__ void tri_tpacket(const TriLaunch &L, const TriDesc<T> &M, int o, long
             tv[q] = ld_plain(gp(M.val), (uint32_t)(dp[j] - (in[q] ? q - l : 0)));
         }
         const double rd = on ? 1.0 / to_f64(M.diag[row]) : 1.0;
-        if (!wave_wait(on ? &flags[row] : nullptr, -epoch, L.abort_word, nullptr, -1, kSolveBackoff))
-            return;
-        double *z = L.z + vo;
-        double num = on ? ld_coherent_val(&z[row]) : 0.0;
+        double num;
+        if (!wave_wait_value(on ? &L.zp[vo + row] : nullptr, num, L.abort_word)) return;
 #pragma unroll
         for (int q = kTriPacketRows - 1; q >= 0; q--) {
             if (q < g) {
@@ -295,11 +285,9 @@ __device__ void tri_tpacket(const TriLaunch &L, const TriDesc<T> &M, int o, long
             }
         }
         const double zt = num * rd;
-        if (on) st_coherent(&z[row], zt);
-        store_complete();
         if (on) {
-            st_relaxed(&flags[row], epoch);
-            const int qo = M.q[row];                   // the solution, after the flag
+            st_coherent(&L.z[vo + row], zt);
+            const int qo = M.q[row];                   // the solution
             L.x[vo + qo] = L.cs ? zt * L.cs[vo + qo] : zt;
         }
     }
@@ -307,7 +295,7 @@ __device__ void tri_tpacket(const TriLaunch &L, const TriDesc<T> &M, int o, long
 
 // One task of the solve (ticket t).
 template <typename T, bool kForward>
-__device__ void tri_task(const TriLaunch &L, int t, int epoch)
+__device__ void tri_task(const TriLaunch &L, int t)
 {
     const int lane = lane_id();
     const TriDesc<T> *mats = (const TriDesc<T> *)L.mats;
@@ -317,19 +305,18 @@ __device__ void tri_task(const TriLaunch &L, int t, int epoch)
     const long vo = L.voff[tk.x];
     double *y = L.y + vo;
     double *z = L.z + vo;
-    int *flags = kForward ? M.fdone : M.bdone;
     const int code = tk.y;
     if (code >= kTriPacket) {
         const int o = code - kTriPacket;
-        if (uniform(gp(kForward ? M.fpk : M.bpk)[o]) < 0) tri_tpacket<T, kForward>(L, M, o, vo, flags, epoch);
-        else tri_packet<T, kForward>(L, M, o, vo, flags, epoch);
+        if (uniform(gp(kForward ? M.fpk : M.bpk)[o]) < 0) tri_tpacket<T, kForward>(L, M, o, vo);
+        else tri_packet<T, kForward>(L, M, o, vo);
         return;
     }
     if (code < 0) {
         const int s0 = -code - 1;
         const int e0 = M.snend[s0];
-        if (kForward) tri_forward<T>(M, s0, e0, y, flags, epoch, L.abort_word);
-        else tri_backward<T>(M, s0, e0, z, L.x + vo, L.cs ? L.cs + vo : nullptr, flags, epoch, L.abort_word);
+        if (kForward) tri_forward<T>(M, s0, e0, y, L.yp + vo, L.abort_word);
+        else tri_backward<T>(M, s0, e0, z, L.zp + vo, L.x + vo, L.cs ? L.cs + vo : nullptr, L.abort_word);
         return;
     }
     const int i = code >> 1;
@@ -358,34 +345,26 @@ __device__ void tri_task(const TriLaunch &L, int t, int epoch)
         }
     }
     double acc = 0.0;
-    bool ok = true;
-    for (int b0 = e0; b0 < e1 && ok; b0 += kGather * kWave) {
-        int jj[kGather];
-        const int *fl[kGather];
+    for (int b0 = e0; b0 < e1; b0 += kGather * kWave) {
+        const double *sp[kGather];
         T vv[kGather];
 #pragma unroll
         for (int g = 0; g < kGather; g++) {
             const int e = b0 + g * kWave + lane;
-            jj[g] = e < e1 ? ci[e] : 0;
             vv[g] = e < e1 ? rv[e] : T(0);
-            fl[g] = e < e1 ? &flags[jj[g]] : nullptr;
+            sp[g] = e < e1 ? &src[ci[e]] : nullptr;
         }
-        ok = wave_wait_n<kGather>(fl, epoch, L.abort_word);
-        if (!ok) break;
         double sv[kGather];
-#pragma unroll
-        for (int g = 0; g < kGather; g++) sv[g] = fl[g] ? ld_coherent_val(&src[jj[g]]) : 0.0;
+        if (!wave_wait_values<kGather>(sp, sv, L.abort_word)) return;
 #pragma unroll
         for (int g = 0; g < kGather; g++) acc += to_f64(vv[g]) * sv[g];
     }
-    if (!ok) return;
     acc = wave_sum(acc);
     if (lane == 0) {
         const double r = kForward || part ? pre - acc : (pre - acc) / dia;
-        st_coherent(kForward ? &y[i] : &z[i], r);
-        store_complete();
-        st_relaxed(&flags[i], part ? -epoch : epoch);
-        if (!kForward && !part) {                      // the solution, after the flag
+        if (part) st_coherent(kForward ? &L.yp[vo + i] : &L.zp[vo + i], r);
+        else st_coherent(kForward ? &y[i] : &z[i], r);
+        if (!kForward && !part) {                      // the solution
             const int qo = M.q[i];
             L.x[vo + qo] = L.cs ? r * L.cs[vo + qo] : r;
         }
@@ -400,16 +379,14 @@ __device__ __forceinline__ void trisolve_body(const TriLaunch &L)
 {
     // A Krylov iteration after every circuit converged: nothing to solve.
     // Every wave of the grid reads the same count (written by an earlier
-    // launch) and returns at once: no ticket, no finish count, and the epoch
-    // stays, as no flag is written under it.
+    // launch) and returns at once: no ticket and no finish count.
     if (L.n_active && uniform(ld_relaxed(L.n_active)) == 0) return;
-    const int epoch = read_epoch(L.epoch);
     for (;;) {
         const int t = wave_ticket(&L.ctr[0]);
         if (t >= L.ntask) break;
         if (aborted(L.abort_word)) continue;
         const long long tg = L.timeline ? (long long)wall_clock64() : 0;
-        tri_task<T, kForward>(L, t, epoch);
+        tri_task<T, kForward>(L, t);
         if (L.timeline && lane_id() == 0) {
             // diagnostic: ticket granted, task finished (wall clock)
             L.timeline[2L * t] = tg;
@@ -418,10 +395,9 @@ __device__ __forceinline__ void trisolve_body(const TriLaunch &L)
     }
     if (lane_id() == 0) {
         const int fin = atomicAdd(&L.ctr[1], 1);
-        if (fin == L.total_waves - 1) {
+        if (fin == L.total_waves - 1) {               // the last wave out re-arms the queue
             st_relaxed(&L.ctr[0], 0);
             st_relaxed(&L.ctr[1], 0);
-            advance_epoch(L.epoch, epoch);
         }
     }
 }
@@ -435,6 +411,29 @@ template <typename T>
 __global__ void __launch_bounds__(256) trisolve_bwd_kernel(TriLaunch L)
 {
     trisolve_body<T, false>(L);
+}
+
+// Sets the solve's work values (y, yp, z, zp) to the unset marker; skipped
+// when no circuit is active (the solves then return at once as well).
+__global__ void __launch_bounds__(256) solve_unset_kernel(double *y, double *yp, double *z, double *zp, long n,
+                                                          const int *n_active)
+{
+    if (n_active && uniform(ld_relaxed(n_active)) == 0) return;
+    const double u = Unset<double>::value();
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) {
+        y[i] = u;
+        yp[i] = u;
+        z[i] = u;
+        zp[i] = u;
+    }
+}
+
+hipError_t launch_solve_unset(const TriLaunch &L, long n, hipStream_t st)
+{
+    const long blocks = std::max(1L, std::min(4096L, (n + 255) / 256));
+    hipLaunchKernelGGL(solve_unset_kernel, dim3((unsigned)blocks), dim3(256), 0, st, L.y, L.yp, L.z, L.zp, n,
+                       L.n_active);
+    return hipGetLastError();
 }
 
 template <typename T>

@@ -313,8 +313,6 @@ int Batch::ensure_factor_set(FactorSet<T> &fs)
         D.diag = F.diag;
         D.rowat = d_rowat_ + C.o_n;
         D.q = d_q_ + C.o_n;
-        D.fdone = d_fdone_ + C.o_n;
-        D.bdone = d_bdone_ + C.o_n;
         D.n = C.n;
         D.tail_F = fs.tailF + C.o_t2;
         D.val = F.val;
@@ -326,7 +324,6 @@ int Batch::ensure_factor_set(FactorSet<T> &fs)
         D.fpk = d_fpk_ + C.o_fpk;
         D.bpk = d_bpk_ + C.o_bpk;
         D.tail_lcut = d_tail_lcut_ + C.o_t;
-        D.tail_flag = d_tail_flag_ + C.o_tb;
         D.tail_ctr = d_tail_ctr_ + 4 * m;
         D.tail_k0 = C.plan.tail_k0;
         D.tail_T = C.plan.tail_T;
@@ -496,15 +493,15 @@ int Batch::upload_all()
 
     // ---- concatenate structures ----
     long o_np1 = 0, o_n = 0, o_slot = 0, o_a = 0, o_rel = 0, o_l = 0, o_u = 0, o_sl = 0,
-         o_slp1 = 0, o_ss = 0, o_t = 0, o_t2 = 0, o_tb = 0, o_fpk = 0, o_bpk = 0;
+         o_slp1 = 0, o_ss = 0, o_t = 0, o_t2 = 0, o_fpk = 0, o_bpk = 0;
     for (auto &C : circ_) {
         C.o_fpk = o_fpk; C.o_bpk = o_bpk;
         o_fpk += (long)C.plan.fpk.size(); o_bpk += (long)C.plan.bpk.size();
         C.o_np1 = o_np1; C.o_n = o_n; C.o_slot = o_slot; C.o_a = o_a; C.o_rel = o_rel;
         C.o_l = o_l; C.o_u = o_u; C.o_sl = o_sl; C.o_slp1 = o_slp1; C.o_ss = o_ss;
-        C.o_t = o_t; C.o_t2 = o_t2; C.o_tb = o_tb;
+        C.o_t = o_t; C.o_t2 = o_t2;
         const long T = C.plan.tail_T;
-        o_t += T; o_t2 += T * T; o_tb += (T + 63) / 64;
+        o_t += T; o_t2 += T * T;
         C.voff = o_n;
         C.aoff = o_a;
         o_np1 += C.n + 1; o_n += C.n; o_slot += C.plan.cp[C.n]; o_a += C.nnz;
@@ -513,7 +510,7 @@ int Batch::upload_all()
     }
     ntot_ = o_n; nnz_tot_ = o_a; nslot_tot_ = o_slot; nl_tot_ = o_l; nu_tot_ = o_u;
     nsell_tot_ = o_ss;
-    nt_tot_ = o_t; nt2_tot_ = o_t2; ntb_tot_ = o_tb;
+    nt_tot_ = o_t; nt2_tot_ = o_t2;
     {
         std::vector<int64_t> cp, relp, soff;
         std::vector<int> dp, pos, aptr, asrc, aslot, dst, lrp, lci, urp, uci, rowat, q, snend, ru, fcut, bcut, fpk, bpk;
@@ -600,7 +597,7 @@ int Batch::upload_all()
     // flags start at 0; epochs start at 1
     {
         std::vector<int> zeros(ntot_, 0);
-        d_done_ = upload(zeros); d_fdone_ = upload(zeros); d_bdone_ = upload(zeros);
+        d_done_ = upload(zeros);
         std::vector<int> z6(8, 0);
         d_ctr_ = upload(z6); d_abort_ = upload(z6);
         std::vector<int> e1(4, 1);                 // epochs start at 1, flags at 0
@@ -608,7 +605,7 @@ int Batch::upload_all()
         std::vector<int> ms(kMstat * nc, 0);
         d_mstat_ = upload(ms);
         d_dbg_ = upload(z6);
-        if (!d_done_ || !d_fdone_ || !d_bdone_ || !d_ctr_ || !d_abort_ || !d_mstat_ || !d_epoch_)
+        if (!d_done_ || !d_ctr_ || !d_abort_ || !d_mstat_ || !d_epoch_)
             return LSP_E_ALLOC;
     }
     // dense trailing blocks
@@ -645,11 +642,8 @@ int Batch::upload_all()
         d_tail_lcut_ = upload(tlc);
         d_djobs_ = upload(jobs);
         d_tails_ = upload(tails);
-        std::vector<int> zb(ntb_tot_, 0), zc(4 * nc, 0);
-        for (int m = 0; m < nc; m++) zc[4 * m + 2] = 1;          // block-flag epoch
-        d_tail_flag_ = upload(zb);
-        d_tail_ctr_ = upload(zc);
-        if (!d_tail_ds_ || !d_tail_lcut_ || !d_djobs_ || !d_tails_ || !d_tail_flag_ || !d_tail_ctr_)
+        d_tail_ctr_ = upload(std::vector<int>(4 * nc, 0));
+        if (!d_tail_ds_ || !d_tail_lcut_ || !d_djobs_ || !d_tails_ || !d_tail_ctr_)
             return LSP_E_ALLOC;
         if (njobs_) {
             rc = check(dense_set_lds(), "dense lds attr");
@@ -826,6 +820,7 @@ int Batch::upload_all()
         d_r_ = (double *)dalloc(ntot_ * 8); d_w_ = (double *)dalloc(ntot_ * 8);
         d_z_ = (double *)dalloc(ntot_ * 8); d_t_ = (double *)dalloc(ntot_ * 8);
         d_ty_ = (double *)dalloc(ntot_ * 8); d_tz_ = (double *)dalloc(ntot_ * 8);
+        d_typ_ = (double *)dalloc(ntot_ * 8); d_tzp_ = (double *)dalloc(ntot_ * 8);
         std::vector<long> voff(nc);
         std::vector<int4> chunks;
         std::vector<int> cbeg(nc + 1, 0);
@@ -848,7 +843,7 @@ int Batch::upload_all()
         d_act_ = (int *)dalloc(nc * 4); d_jfin_ = (int *)dalloc(nc * 4);
         d_conv_ = (int *)dalloc(nc * 4); d_cycled_ = (int *)dalloc(nc * 4);
         d_nact_ = (int *)dalloc(16); d_subset_ = (int *)dalloc(nc * 4);
-        if (!d_b_ || !d_x_ || !d_r_ || !d_w_ || !d_z_ || !d_t_ || !d_ty_ || !d_tz_ ||
+        if (!d_b_ || !d_x_ || !d_r_ || !d_w_ || !d_z_ || !d_t_ || !d_ty_ || !d_tz_ || !d_typ_ || !d_tzp_ ||
             !d_voff_ || !d_chunks_ || !d_cbeg_ || !d_act_)
             return LSP_E_ALLOC;
         if (!h_pin_ && hipHostMalloc((void **)&h_pin_, 4096, hipHostMallocDefault) != hipSuccess)
@@ -1223,8 +1218,9 @@ int Batch::trisolve_t(FactorSet<T> &fs, const double *d_b, double *d_x, const in
     Lf.x = d_x;
     Lf.y = d_ty_;
     Lf.z = d_tz_;
+    Lf.yp = d_typ_;
+    Lf.zp = d_tzp_;
     Lf.ntask = (int)nfw_tot_;
-    Lf.epoch = d_epoch_ + 1;
     Lf.ctr = d_ctr_ + 2;
     Lf.abort_word = d_abort_ + 1;
     Lf.total_waves = tblocks_f_ * 4;
@@ -1241,20 +1237,20 @@ int Batch::trisolve_t(FactorSet<T> &fs, const double *d_b, double *d_x, const in
     Lb.timeline = Lf.timeline ? Lf.timeline + 2 * nfw_tot_ : nullptr;
     Lb.task = d_bwtask_;
     Lb.ntask = (int)nbw_tot_;
-    Lb.epoch = d_epoch_ + 2;
     Lb.ctr = d_ctr_ + 4;
     Lb.total_waves = tblocks_b_ * 4;
     if (!capturing_) (void)hipEventRecord(ev_[2], stream_);
-    int rc = check(launch_trisolve_half<T>(Lf, true, tblocks_f_, stream_), "trisolve fwd");
+    int rc = check(launch_solve_unset(Lf, ntot_, stream_), "solve unset");
+    if (!rc) rc = check(launch_trisolve_half<T>(Lf, true, tblocks_f_, stream_), "trisolve fwd");
     if (!rc && njobs_)
         rc = check(launch_tail_solve<T>(fs.tdesc, d_tails_, njobs_, max_tail_T_, d_voff_, d_active, d_b,
-                                        d_ty_, d_tz_, d_x, true, d_epoch_ + 2, d_abort_ + 1, Lf.rs, Lf.cs,
+                                        d_ty_, d_typ_, d_tz_, d_x, true, d_abort_ + 1, Lf.rs, Lf.cs,
                                         stream_),
                    "tail fwd");
     if (!capturing_) (void)hipEventRecord(ev_[4], stream_);
     if (!rc && njobs_)
         rc = check(launch_tail_solve<T>(fs.tdesc, d_tails_, njobs_, max_tail_T_, d_voff_, d_active, d_b,
-                                        d_ty_, d_tz_, d_x, false, d_epoch_ + 2, d_abort_ + 1, Lf.rs, Lf.cs,
+                                        d_ty_, d_typ_, d_tz_, d_x, false, d_abort_ + 1, Lf.rs, Lf.cs,
                                         stream_),
                    "tail bwd");
     if (!rc) rc = check(launch_trisolve_half<T>(Lb, false, tblocks_b_, stream_), "trisolve bwd");

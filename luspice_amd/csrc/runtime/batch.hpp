@@ -40,7 +40,7 @@ struct CircuitHost {
     long o_rel = 0;                      // update map
     long o_l = 0, o_u = 0;               // strict L / U entry arrays
     long o_sl = 0, o_slp1 = 0, o_ss = 0; // SELL slices, slices+1, slots
-    long o_t = 0, o_t2 = 0, o_tb = 0;    // dense block: T, T^2, 64-row blocks
+    long o_t = 0, o_t2 = 0;              // dense block: T, T^2
     long o_fpk = 0, o_bpk = 0;           // trisolve packets forward / backward
     long o_lf = 0, o_lb = 0;             // fused IR step: level pointers forward / backward
 };
@@ -67,7 +67,7 @@ struct DeviceArrays {
     int *d_dp_ = nullptr, *d_pos_ = nullptr, *d_aptr_ = nullptr, *d_asrc_ = nullptr,
         *d_aslot_ = nullptr, *d_dst_ = nullptr, *d_lrp_ = nullptr, *d_lci_ = nullptr,
         *d_urp_ = nullptr, *d_uci_ = nullptr, *d_rowat_ = nullptr, *d_q_ = nullptr,
-        *d_done_ = nullptr, *d_fdone_ = nullptr, *d_bdone_ = nullptr, *d_srlen_ = nullptr,
+        *d_done_ = nullptr, *d_srlen_ = nullptr,
         *d_scol_ = nullptr, *d_sperm_ = nullptr, *d_ssrc_ = nullptr;
     uint16_t *d_rel_ = nullptr;
     int *d_snend_ = nullptr, *d_ru_ = nullptr, *d_fcut_ = nullptr, *d_bcut_ = nullptr;
@@ -102,21 +102,21 @@ struct DeviceArrays {
     int fblocks_ = 0, tblocks_f_ = 0, tblocks_b_ = 0;
     long width_f_ = 0, width_tf_ = 0, width_tb_ = 0;   // widest merged level
     // dense trailing blocks
-    int *d_tail_ds_ = nullptr, *d_tail_lcut_ = nullptr, *d_tail_flag_ = nullptr,
-        *d_tail_ctr_ = nullptr, *d_tails_ = nullptr;
+    int *d_tail_ds_ = nullptr, *d_tail_lcut_ = nullptr, *d_tail_ctr_ = nullptr, *d_tails_ = nullptr;
     double *d_tailS_ = nullptr;
     unsigned long long *d_colmax_ = nullptr;
     int *d_dflag_ = nullptr;                   // per dense job: the last diagonal block factored
     DenseJob *d_djobs_ = nullptr;
     int njobs_ = 0, max_tail_T_ = 0;
-    long nt_tot_ = 0, nt2_tot_ = 0, ntb_tot_ = 0;
+    long nt_tot_ = 0, nt2_tot_ = 0;
     long nfw_tot_ = 0, nbw_tot_ = 0;     // row tasks of the sparse solves
     FactorSet<float> f32_;
     FactorSet<double> f64_;
     FactorSet<__half> f16_;
     // vectors and Krylov state
     double *d_b_ = nullptr, *d_x_ = nullptr, *d_r_ = nullptr, *d_w_ = nullptr,
-           *d_z_ = nullptr, *d_t_ = nullptr, *d_ty_ = nullptr, *d_tz_ = nullptr, *d_V_ = nullptr,
+           *d_z_ = nullptr, *d_t_ = nullptr, *d_ty_ = nullptr, *d_tz_ = nullptr, *d_typ_ = nullptr,
+           *d_tzp_ = nullptr, *d_V_ = nullptr,
            *d_Z_ = nullptr;
     long *d_voff_ = nullptr;
     int4 *d_chunks_ = nullptr;
