@@ -48,7 +48,9 @@ void lsp_gp_solve(const LspGpLu &F, const int *q, const double *b, double *x);
 // ---------------------------------------------------------------------------
 // Static structure for the GPU numeric refactorization and trisolve.
 constexpr int kTriPacketBase = 1 << 30;   // trisolve task codes at or above: packets
+constexpr int kTriWideBase = 1 << 29;     // codes in [kTriWideBase, kTriPacketBase): wide supernode blocks
 constexpr int kTriPacketPerLane = 4;      // entries per lane of a packet row
+constexpr int kTriBlockRows = 64;         // rows of one triangle task (one wave, one row per lane)
 
 struct LspPlan {
     int n = 0;
@@ -99,7 +101,12 @@ struct LspPlan {
     // 16 or 32, with g kTriPacketPerLane >= the row's entry count), up to
     // 64 / g rows.  Triangle tasks of supernodes of at most 16 rows are
     // packed as [-g, count, supernode start x count], g >= the width.
+    // Supernodes wider than kTriBlockRows are solved by one task per 64-row
+    // block (code kTriWideBase + x): twide[2x], twide[2x + 1] = (s, b), the
+    // b-th block of the supernode at s in solve order (from the top forward,
+    // from the bottom backward); block b waits for blocks < b.
     std::vector<int> fwd_task, bwd_task;
+    std::vector<int> twide;
     std::vector<int> fwd_key, bwd_key;    // ordering keys (merge across circuits)
     std::vector<int> fpk, bpk;            // packets (see above)
     std::vector<int> fcut;                // n: end of row i's L entries left of its supernode

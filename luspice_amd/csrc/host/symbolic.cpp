@@ -149,6 +149,7 @@ int lsp_build_plan(int n, const int *Ap, const int *Ai, const int *src_index,
                    const LspGpLu &F, const int *q, LspPlan &P)
 {
     P = LspPlan();
+    if (n >= kTriWideBase / 2) return -1;        // row task codes 2i + 1 stay below the block codes
     P.n = n;
     P.nnz_a = Ap[n];
     P.pinv = F.pinv;
@@ -371,6 +372,18 @@ static void plan_tri_supernodes(LspPlan &P)
         for (int64_t t = P.dp[i] - 1; t >= P.cp[i] && P.pos[t] >= sst[i] && P.pos[t] == i - 1 - r; t--) r++;
         P.ru[i] = r;
     }
+    // blocks of the supernodes wider than one triangle task
+    std::vector<int> wide0(k0, -1);
+    P.twide.clear();
+    for (int s = 0; s < k0; s = P.snend[s]) {
+        const int w = P.snend[s] - s;
+        if (w <= kTriBlockRows) continue;
+        wide0[s] = (int)P.twide.size() / 2;
+        for (int b = 0; b * kTriBlockRows < w; b++) {
+            P.twide.push_back(s);
+            P.twide.push_back(b);
+        }
+    }
     // F(D) = 1 + max F over the supernodes D's rows read outside D
     auto build = [&](bool fwd, std::vector<int> &task, std::vector<int> &key, std::vector<int> &pk, int &nlv) {
         std::vector<int> F(k0, 0);
@@ -402,16 +415,19 @@ static void plan_tri_supernodes(LspPlan &P)
                 items.push_back({2 * F[s] + 1, 2 * i});
             } else {
                 items.push_back({2 * F[s], 2 * i + 1});
-                if (i == s) items.push_back({2 * F[s] + 1, -(s + 1)});
+                if (i == s && w <= kTriBlockRows) items.push_back({2 * F[s] + 1, -(s + 1)});
+                if (i == s && w > kTriBlockRows)
+                    for (int b = 0; b * kTriBlockRows < w; b++)
+                        items.push_back({2 * F[s] + 1, kTriWideBase + wide0[s] + b});
             }
         }
         std::stable_sort(items.begin(), items.end(),
                          [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first < b.first; });
         // Pack the short row tasks of each key: rows with at most
         // g kTriPacketPerLane entries (g = 4, 8, 16, 32 lanes) go 64 / g to a
-        // packet, so one ticket serves up to 16 rows.  Tasks of one key are
-        // independent, so any grouping inside a key keeps the order a valid
-        // schedule.
+        // packet, so one ticket serves up to 16 rows.  Rows and triangles of
+        // one key are independent, so any grouping of them keeps the order a
+        // valid schedule; the blocks of a wide supernode stay in their order.
         auto entries = [&](int code) {
             const int i = code >> 1;
             const bool part = code & 1;
@@ -446,6 +462,10 @@ static void plan_tri_supernodes(LspPlan &P)
             for (auto &c : tcls) c.clear();
             for (size_t t = a; t < b; t++) {
                 const int code = items[t].second;
+                if (code >= kTriWideBase) {
+                    out.push_back(items[t]);
+                    continue;
+                }
                 if (code >= 0) {
                     const int ne = entries(code);
                     const int k = kTriPacketPerLane;

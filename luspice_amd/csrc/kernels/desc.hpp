@@ -19,9 +19,11 @@ enum : int {
 // Status words per matrix written by the refactor (FactorDesc::mstat).
 constexpr int kMstat = 8;
 
-// Trisolve task codes at or above this value name a packet of short rows;
-// each lane of a packet row gathers up to kTriPacketLane of its entries.
+// Trisolve task codes at or above kTriPacket name a packet of short rows
+// (each lane of a packet row gathers up to kTriPacketLane of its entries);
+// codes in [kTriWide, kTriPacket) name one 64-row block of a wide supernode.
 constexpr int kTriPacket = 1 << 30;
+constexpr int kTriWide = 1 << 29;
 constexpr int kTriPacketLane = 4;
 
 // Numeric refactorization over the static column patterns.
@@ -67,6 +69,7 @@ struct TriDesc {
     const T *lrow, *urow, *diag;
     const int *rowat;       // position -> original row (P)
     const int *q;           // position -> original column (Q)
+    const int *twide;       // (supernode start, block) pairs of the wide-supernode block tasks
     // supernodes (rows below tail_k0)
     const T *val;           // column storage of L and U (FactorDesc::val)
     const int *dp;          // n diagonal slots (absolute)

@@ -37,117 +37,105 @@ constexpr int kGather = 4;              // 64-entry chunks of a row per wait rou
 
 constexpr int kSolveBatch = 32;         // dense-part values in flight per pass of a triangle task
 
-// Forward triangle task of supernode D = [s, e): its rows' partial sums
-// p_i = b_i - sum over L entries left of s are in yp; y_D = L_DD^-1 p in
-// 64-row blocks, each block first subtracting the earlier blocks' rows
-// (L(i, j) = val[dp[j] + i - j]) and then solving its unit-lower triangle in
-// registers.  Returns false when a wait was abandoned.
+// Forward triangle task: block b (rows s + 64b .. s + 64b + bw - 1) of the
+// supernode D = [s, e).  Its rows' partial sums p_i = b_i - sum over L
+// entries left of s are in yp; the earlier blocks of D, each solved by a task
+// of its own (ticketed before this one), are subtracted first (L(i, j) =
+// val[dp[j] + i - j]), 32 columns at a time, their L values loaded before
+// their y values are awaited; then the block's unit-lower triangle is solved
+// in registers.  A supernode of at most 64 rows is one block.
 template <typename T>
-__device__ bool tri_forward(const TriDesc<T> &M, int s, int e, double *y, const double *yp, int *abort_word)
+__device__ void tri_forward(const TriDesc<T> &M, int s, int b, double *y, const double *yp, int *abort_word)
 {
     const int lane = lane_id();
     const LSP_GLOBAL int *dp = gp(M.dp);
-    const int w = e - s;
-    for (int b0 = 0; b0 < w; b0 += kWave) {
-        const int bw = min(kWave, w - b0);
-        const int row = s + b0 + lane;
-        const int rr = s + b0 + min(lane, bw - 1);
-        // earlier blocks (solved by this task), 64 columns at a time: lane q
-        // holds y of column q of the chunk (one load), broadcast by readlane.
-        // They do not depend on this block's partial sums, so they are
-        // accumulated before the wait for those.
-        constexpr int SB = kSolveBatch;
-        double acc = 0.0;
-        for (int q0 = 0; q0 < b0; q0 += kWave) {
-            const int qn = min(kWave, b0 - q0);
-            const double yq = ld_coherent_val(&y[s + q0 + min(lane, qn - 1)]);
-            for (int h0 = 0; h0 < qn; h0 += SB) {
-                T v[SB];
+    const int w = M.snend[s] - s;
+    const int b0 = kWave * b;
+    const int bw = min(kWave, w - b0);
+    const int row = s + b0 + lane;
+    const int rr = s + b0 + min(lane, bw - 1);
+    constexpr int SB = kSolveBatch;
+    double acc = 0.0;
+    for (int q0 = 0; q0 < b0; q0 += kWave) {            // earlier blocks are full
+        double yq = 0.0;
+        for (int h0 = 0; h0 < kWave; h0 += SB) {
+            T v[SB];
 #pragma unroll
-                for (int h = 0; h < SB; h++) {
-                    const int jq = s + q0 + min(h0 + h, qn - 1);
-                    v[h] = ld_plain(gp(M.val), (uint32_t)(dp[jq] + (rr - jq)));   // written by the refactor
-                }
-#pragma unroll
-                for (int h = 0; h < SB; h++)
-                    if (h0 + h < qn) acc = fma(to_f64(v[h]), bcast_f64(yq, h0 + h), acc);
+            for (int h = 0; h < SB; h++) {
+                const int jq = s + q0 + h0 + h;
+                v[h] = ld_plain(gp(M.val), (uint32_t)(dp[jq] + (rr - jq)));   // written by the refactor
             }
+            if (h0 == 0 && !wave_wait_value(&y[s + q0 + lane], yq, abort_word)) return;
+#pragma unroll
+            for (int h = 0; h < SB; h++) acc = fma(to_f64(v[h]), bcast_f64(yq, h0 + h), acc);
         }
-        double p;
-        if (!wave_wait_value(lane < bw ? &yp[row] : nullptr, p, abort_word)) return false;
-        p -= acc;
-        const int lb = dp[rr] + 1;
-        if (bw == 1) {
-        } else if (bw <= 9) p = run_triangle<8, false>(M.val, p, lb, 0, bw);
-        else if (bw <= 25) p = run_triangle<24, false>(M.val, p, lb, 0, bw);
-        else p = run_triangle<kTri, false>(M.val, p, lb, 0, bw);
-        if (lane < bw) st_coherent(&y[row], p);
-        if (b0 + kWave < w) store_complete();          // the next block reads these back
     }
-    return true;
+    double p;
+    if (!wave_wait_value(lane < bw ? &yp[row] : nullptr, p, abort_word)) return;
+    p -= acc;
+    const int lb = dp[rr] + 1;
+    if (bw == 1) {
+    } else if (bw <= 9) p = run_triangle<8, false>(M.val, p, lb, 0, bw);
+    else if (bw <= 25) p = run_triangle<24, false>(M.val, p, lb, 0, bw);
+    else p = run_triangle<kTri, false>(M.val, p, lb, 0, bw);
+    if (lane < bw) st_coherent(&y[row], p);
 }
 
-// Backward triangle task of D = [s, e): its rows' partial sums q_i = y_i -
-// sum over U entries right of e are in zp; z_D = U_DD^-1 q in 64-row blocks
-// from the bottom, each block first subtracting the later blocks' rows
-// (U(i, j) = val[dp[j] - (j - i)] within column j's run ru[j], else 0).
+// Backward triangle task: block b from the bottom of D = [s, e).  Its rows'
+// partial sums q_i = y_i - sum over U entries right of e are in zp; the
+// later blocks of D (solved by earlier tasks) are subtracted first (U(i, j) =
+// val[dp[j] - (j - i)] within column j's run ru[j], else 0), then the block's
+// upper triangle is solved and the solution written.
 template <typename T>
-__device__ bool tri_backward(const TriDesc<T> &M, int s, int e, double *z, const double *zp, double *x,
+__device__ void tri_backward(const TriDesc<T> &M, int s, int b, double *z, const double *zp, double *x,
                              const double *cs, int *abort_word)
 {
     const int lane = lane_id();
     const LSP_GLOBAL int *dp = gp(M.dp);
     const LSP_GLOBAL int *ru = gp(M.ru);
-    const int w = e - s;
-    for (int b1 = w; b1 > 0; b1 -= kWave) {
-        const int bw = min(kWave, b1);
-        const int base = s + b1 - bw;
-        const int row = base + lane;
-        const int rr = base + min(lane, bw - 1);
-        // later blocks (solved by this task), 64 columns at a time: lane q
-        // holds z of column q of the chunk (one load), broadcast by readlane;
-        // accumulated before the wait for this block's partial sums
-        constexpr int SB = kSolveBatch;
-        const int c0 = base + bw;                        // first column of the later blocks
-        const int cnt = e - c0;
-        double acc = 0.0;
-        for (int q0 = 0; q0 < cnt; q0 += kWave) {
-            const int qn = min(kWave, cnt - q0);
-            const double zq = ld_coherent_val(&z[c0 + q0 + min(lane, qn - 1)]);
-            for (int h0 = 0; h0 < qn; h0 += SB) {
-                T v[SB];
-                bool in[SB];
+    const int w = M.snend[s] - s;
+    const int b1 = w - kWave * b;                        // rows [s, s + b1) not yet solved
+    const int bw = min(kWave, b1);
+    const int base = s + b1 - bw;
+    const int row = base + lane;
+    const int rr = base + min(lane, bw - 1);
+    const int c0 = base + bw;                            // first column of the later blocks
+    constexpr int SB = kSolveBatch;
+    double acc = 0.0;
+    for (int q0 = 0; q0 < kWave * b; q0 += kWave) {     // later blocks are full
+        double zq = 0.0;
+        for (int h0 = 0; h0 < kWave; h0 += SB) {
+            T v[SB];
+            bool in[SB];
 #pragma unroll
-                for (int h = 0; h < SB; h++) {
-                    const int jq = c0 + q0 + min(h0 + h, qn - 1);
-                    const int d = jq - rr;               // >= 1
-                    const int rj = ru[jq];
-                    in[h] = d <= rj;
-                    v[h] = ld_plain(gp(M.val), (uint32_t)(dp[jq] - min(d, rj)));
-                }
-#pragma unroll
-                for (int h = 0; h < SB; h++)
-                    if (h0 + h < qn && in[h]) acc = fma(to_f64(v[h]), bcast_f64(zq, h0 + h), acc);
+            for (int h = 0; h < SB; h++) {
+                const int jq = c0 + q0 + h0 + h;
+                const int d = jq - rr;                   // >= 1
+                const int rj = ru[jq];
+                in[h] = d <= rj;
+                v[h] = ld_plain(gp(M.val), (uint32_t)(dp[jq] - min(d, rj)));
             }
+            if (h0 == 0 && !wave_wait_value(&z[c0 + q0 + lane], zq, abort_word)) return;
+#pragma unroll
+            for (int h = 0; h < SB; h++)
+                if (in[h]) acc = fma(to_f64(v[h]), bcast_f64(zq, h0 + h), acc);
         }
-        const double rd = 1.0 / to_f64(M.diag[rr]);
-        const int dpc = dp[rr];
-        const int run = ru[rr];
-        double num;
-        if (!wave_wait_value(lane < bw ? &zp[row] : nullptr, num, abort_word)) return false;
-        num -= acc;
-        double zt;
-        if (bw <= 8) zt = run_triangle_upper<8, false>(M.val, num, rd, dpc, run, bw);
-        else if (bw <= 24) zt = run_triangle_upper<24, false>(M.val, num, rd, dpc, run, bw);
-        else zt = run_triangle_upper<kWave, false>(M.val, num, rd, dpc, run, bw);
-        if (lane < bw) {
-            st_coherent(&z[row], zt);
-            const int qo = M.q[row];
-            x[qo] = cs ? zt * cs[qo] : zt;
-        }
-        if (b1 - kWave > 0) store_complete();          // the next block reads these back
     }
-    return true;
+    const double rd = 1.0 / to_f64(M.diag[rr]);
+    const int dpc = dp[rr];
+    const int run = ru[rr];
+    double num;
+    if (!wave_wait_value(lane < bw ? &zp[row] : nullptr, num, abort_word)) return;
+    num -= acc;
+    double zt;
+    if (bw <= 8) zt = run_triangle_upper<8, false>(M.val, num, rd, dpc, run, bw);
+    else if (bw <= 24) zt = run_triangle_upper<24, false>(M.val, num, rd, dpc, run, bw);
+    else zt = run_triangle_upper<kWave, false>(M.val, num, rd, dpc, run, bw);
+    if (lane < bw) {
+        st_coherent(&z[row], zt);
+        const int qo = M.q[row];
+        x[qo] = cs ? zt * cs[qo] : zt;
+    }
 }
 
 // Packet task (code kTriPacket + o): up to 64 / g row tasks of one level,
@@ -312,11 +300,15 @@ __device__ void tri_task(const TriLaunch &L, int t)
         else tri_packet<T, kForward>(L, M, o, vo);
         return;
     }
-    if (code < 0) {
-        const int s0 = -code - 1;
-        const int e0 = M.snend[s0];
-        if (kForward) tri_forward<T>(M, s0, e0, y, L.yp + vo, L.abort_word);
-        else tri_backward<T>(M, s0, e0, z, L.zp + vo, L.x + vo, L.cs ? L.cs + vo : nullptr, L.abort_word);
+    if (code < 0 || code >= kTriWide) {
+        int s0 = -code - 1, bk = 0;                      // supernode, block in solve order
+        if (code >= 0) {
+            const LSP_GLOBAL int *tw = gp(M.twide) + 2 * (code - kTriWide);
+            s0 = uniform(tw[0]);
+            bk = uniform(tw[1]);
+        }
+        if (kForward) tri_forward<T>(M, s0, bk, y, L.yp + vo, L.abort_word);
+        else tri_backward<T>(M, s0, bk, z, L.zp + vo, L.x + vo, L.cs ? L.cs + vo : nullptr, L.abort_word);
         return;
     }
     const int i = code >> 1;
@@ -373,7 +365,8 @@ __device__ void tri_task(const TriLaunch &L, int t)
 
 // Task codes (see the plan): 2i = row i solved whole, 2i+1 = row i's entries
 // outside its supernode into a partial sum, -(s+1) = supernode triangle,
-// kTriPacket + o = packet of short rows.
+// kTriWide + x = one 64-row block of a wide supernode, kTriPacket + o =
+// packet of short rows or small triangles.
 template <typename T, bool kForward>
 __device__ __forceinline__ void trisolve_body(const TriLaunch &L)
 {

@@ -313,6 +313,7 @@ int Batch::ensure_factor_set(FactorSet<T> &fs)
         D.diag = F.diag;
         D.rowat = d_rowat_ + C.o_n;
         D.q = d_q_ + C.o_n;
+        D.twide = d_twide_ + C.o_tw;
         D.n = C.n;
         D.tail_F = fs.tailF + C.o_t2;
         D.val = F.val;
@@ -493,10 +494,10 @@ int Batch::upload_all()
 
     // ---- concatenate structures ----
     long o_np1 = 0, o_n = 0, o_slot = 0, o_a = 0, o_rel = 0, o_l = 0, o_u = 0, o_sl = 0,
-         o_slp1 = 0, o_ss = 0, o_t = 0, o_t2 = 0, o_fpk = 0, o_bpk = 0;
+         o_slp1 = 0, o_ss = 0, o_t = 0, o_t2 = 0, o_fpk = 0, o_bpk = 0, o_tw = 0;
     for (auto &C : circ_) {
-        C.o_fpk = o_fpk; C.o_bpk = o_bpk;
-        o_fpk += (long)C.plan.fpk.size(); o_bpk += (long)C.plan.bpk.size();
+        C.o_fpk = o_fpk; C.o_bpk = o_bpk; C.o_tw = o_tw;
+        o_fpk += (long)C.plan.fpk.size(); o_bpk += (long)C.plan.bpk.size(); o_tw += (long)C.plan.twide.size();
         C.o_np1 = o_np1; C.o_n = o_n; C.o_slot = o_slot; C.o_a = o_a; C.o_rel = o_rel;
         C.o_l = o_l; C.o_u = o_u; C.o_sl = o_sl; C.o_slp1 = o_slp1; C.o_ss = o_ss;
         C.o_t = o_t; C.o_t2 = o_t2;
@@ -513,7 +514,8 @@ int Batch::upload_all()
     nt_tot_ = o_t; nt2_tot_ = o_t2;
     {
         std::vector<int64_t> cp, relp, soff;
-        std::vector<int> dp, pos, aptr, asrc, aslot, dst, lrp, lci, urp, uci, rowat, q, snend, ru, fcut, bcut, fpk, bpk;
+        std::vector<int> dp, pos, aptr, asrc, aslot, dst, lrp, lci, urp, uci, rowat, q, snend, ru, fcut, bcut, fpk, bpk,
+            twide;
         std::vector<int> dpos;
         std::vector<int> srlen, scol, sperm, ssrc;
         std::vector<uint16_t> rel;
@@ -540,6 +542,7 @@ int Batch::upload_all()
             bcut.insert(bcut.end(), P.bcut.begin(), P.bcut.end());
             fpk.insert(fpk.end(), P.fpk.begin(), P.fpk.end());
             bpk.insert(bpk.end(), P.bpk.begin(), P.bpk.end());
+            twide.insert(twide.end(), P.twide.begin(), P.twide.end());
             rel.insert(rel.end(), P.rel.begin(), P.rel.end());
             for (int k = 0; k < C.n; k++) dpos.push_back(P.pinv[P.q[k]]);
             const LspSell &S = C.sell;
@@ -563,7 +566,7 @@ int Batch::upload_all()
         d_lrp_ = upload(lrp); d_lci_ = upload(lci); d_urp_ = upload(urp); d_uci_ = upload(uci);
         d_rowat_ = upload(rowat); d_q_ = upload(q); d_rel_ = upload(rel); d_snend_ = upload(snend);
         d_ru_ = upload(ru); d_fcut_ = upload(fcut); d_bcut_ = upload(bcut);
-        d_fpk_ = upload(fpk); d_bpk_ = upload(bpk);
+        d_fpk_ = upload(fpk); d_bpk_ = upload(bpk); d_twide_ = upload(twide);
         d_srlen_ = upload(srlen); d_scol_ = upload(scol); d_sperm_ = upload(sperm);
         d_ssrc_ = upload(ssrc);
         d_dpos_ = upload(dpos);
@@ -591,7 +594,7 @@ int Batch::upload_all()
         }
         if (!d_cp_ || !d_relp_ || !d_soff_ || !d_dp_ || !d_pos_ || !d_aptr_ || !d_asrc_ ||
             !d_aslot_ || !d_dst_ || !d_lrp_ || !d_lci_ || !d_urp_ || !d_uci_ || !d_rowat_ ||
-            !d_q_ || !d_rel_ || !d_snend_ || !d_ru_ || !d_fcut_ || !d_bcut_ || !d_fpk_ || !d_bpk_ || !d_srlen_ || !d_scol_ || !d_sperm_ || !d_ssrc_)
+            !d_q_ || !d_rel_ || !d_snend_ || !d_ru_ || !d_fcut_ || !d_bcut_ || !d_fpk_ || !d_bpk_ || !d_twide_ || !d_srlen_ || !d_scol_ || !d_sperm_ || !d_ssrc_)
             return LSP_E_ALLOC;
     }
     // flags start at 0; epochs start at 1

@@ -42,6 +42,7 @@ struct CircuitHost {
     long o_sl = 0, o_slp1 = 0, o_ss = 0; // SELL slices, slices+1, slots
     long o_t = 0, o_t2 = 0;              // dense block: T, T^2
     long o_fpk = 0, o_bpk = 0;           // trisolve packets forward / backward
+    long o_tw = 0;                       // wide-supernode block pairs
     long o_lf = 0, o_lb = 0;             // fused IR step: level pointers forward / backward
 };
 
@@ -72,7 +73,7 @@ struct DeviceArrays {
     uint16_t *d_rel_ = nullptr;
     int *d_snend_ = nullptr, *d_ru_ = nullptr, *d_fcut_ = nullptr, *d_bcut_ = nullptr;
     int *d_dpos_ = nullptr;              // per column: position of the diagonal row q[k]
-    int *d_fpk_ = nullptr, *d_bpk_ = nullptr;
+    int *d_fpk_ = nullptr, *d_bpk_ = nullptr, *d_twide_ = nullptr;
     double *d_tailW_ = nullptr;
     // Ruiz equilibration (FP16 factor storage)
     int *d_arow_ = nullptr, *d_acol_ = nullptr;

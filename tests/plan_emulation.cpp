@@ -15,7 +15,8 @@
 //    substitution with the same factors.
 //
 // usage: plan_emulation <kind: 0 = c0, 1 = c1, else n of a sized circuit> <tail max T (0 = none)> <seed>
-//                       [plant: 1 = forward tasks in reverse order, 2 = one run's map advance lost]
+//                       [plant: 1 = forward tasks in reverse order, 2 = one run's map advance lost,
+//                               3 = the first two blocks of a wide supernode swapped (forward and backward)]
 // prints one line "ok ..." and exits 0, or a message and a nonzero code.  A
 // planted mistake must make it fail (the checks are not vacuous).
 #include <algorithm>
@@ -166,6 +167,14 @@ int main(int argc, char **argv)
 
     // ---- 2. trisolve task lists (factors: the plain left-looking ones) ----
     if (plant == 1) std::reverse(P.fwd_task.begin(), P.fwd_task.end());
+    if (plant == 3) {
+        for (auto *tl : {&P.fwd_task, &P.bwd_task})
+            for (size_t t = 0; t + 1 < tl->size(); t++)
+                if ((*tl)[t] >= kTriWideBase && (*tl)[t] < kTriPacketBase && (*tl)[t + 1] == (*tl)[t] + 1) {
+                    std::swap((*tl)[t], (*tl)[t + 1]);
+                    break;
+                }
+    }
     std::vector<double> &val = vref;
     // tail columns: factor them plainly so that every row has factors
     for (int k = k0; k < n; k++) {
@@ -249,10 +258,23 @@ int main(int argc, char **argv)
             }
             continue;
         }
-        if (code < 0) {
-            const int s = -code - 1, e = P.snend[s];
-            for (int i = s; i < e; i++) if (ff[i] != -EP) return fail("fwd wait", i);
-            for (int i = s; i < e; i++) {
+        if (code < 0 || code >= kTriWideBase) {
+            // a whole supernode (at most 64 rows), or block b of a wide one
+            // (its earlier blocks must be final)
+            int s = -code - 1, b = 0, whole = 1;
+            if (code >= 0) {
+                const long x = code - kTriWideBase;
+                if (2 * x + 1 >= (long)P.twide.size()) return fail("wide block code", x);
+                s = P.twide[2 * x], b = P.twide[2 * x + 1], whole = 0;
+            }
+            const int e = P.snend[s];
+            if (whole && e - s > kTriBlockRows) return fail("wide supernode in one task", s);
+            const int r0 = whole ? s : s + kTriBlockRows * b;
+            const int r1 = whole ? e : std::min(e, r0 + kTriBlockRows);
+            if (r0 >= r1) return fail("empty block", s, b);
+            for (int i = s; i < r0; i++) if (ff[i] != EP) return fail("fwd block wait", i);
+            for (int i = r0; i < r1; i++) if (ff[i] != -EP) return fail("fwd wait", i);
+            for (int i = r0; i < r1; i++) {
                 double p = y[i];
                 for (int j = s; j < i; j++) p -= val[P.dp[j] + (i - j)] * y[j];
                 y[i] = p;
@@ -326,10 +348,23 @@ int main(int argc, char **argv)
             }
             continue;
         }
-        if (code < 0) {
-            const int s = -code - 1, e = P.snend[s];
-            for (int i = s; i < e; i++) if (fb[i] != -EP) return fail("bwd wait", i);
-            for (int i = e - 1; i >= s; i--) {
+        if (code < 0 || code >= kTriWideBase) {
+            // a whole supernode, or block b from the bottom of a wide one
+            // (its later blocks must be final)
+            int s = -code - 1, b = 0, whole = 1;
+            if (code >= 0) {
+                const long x = code - kTriWideBase;
+                if (2 * x + 1 >= (long)P.twide.size()) return fail("wide block code", x);
+                s = P.twide[2 * x], b = P.twide[2 * x + 1], whole = 0;
+            }
+            const int e = P.snend[s];
+            if (whole && e - s > kTriBlockRows) return fail("wide supernode in one task", s);
+            const int r1 = whole ? e : e - kTriBlockRows * b;
+            const int r0 = whole ? s : std::max(s, r1 - kTriBlockRows);
+            if (r0 >= r1) return fail("empty block", s, b);
+            for (int i = r1; i < e; i++) if (fb[i] != EP) return fail("bwd block wait", i);
+            for (int i = r0; i < r1; i++) if (fb[i] != -EP) return fail("bwd wait", i);
+            for (int i = r1 - 1; i >= r0; i--) {
                 double num = z[i];
                 for (int j = i + 1; j < e; j++) {
                     const int d = j - i;
@@ -358,7 +393,8 @@ int main(int argc, char **argv)
         eb = std::max(eb, std::fabs(z[i] - zr[i]) / (1.0 + std::fabs(zr[i])));
     }
     if (!(ef < 1e-10 && eb < 1e-10)) return fail("trisolve differs", (long)(ef * 1e15), (long)(eb * 1e15));
-    std::printf("ok n %d k0 %d parts %ld segment-diff %.2e tasks %zu/%zu packets %ld trisolve-diff %.2e/%.2e\n", n, k0,
-                parts, maxdiff, P.fwd_task.size(), P.bwd_task.size(), packets, ef, eb);
+    std::printf("ok n %d k0 %d parts %ld segment-diff %.2e tasks %zu/%zu packets %ld wide-blocks %zu "
+                "trisolve-diff %.2e/%.2e\n", n, k0, parts, maxdiff, P.fwd_task.size(), P.bwd_task.size(), packets,
+                P.twide.size() / 2, ef, eb);
     return 0;
 }

@@ -7,10 +7,11 @@ library and run on generated circuits.
   streaming waits cut them.  Every index is checked, and the result must match
   a per-dependency update.
 * The forward and backward trisolve task lists, with their row and triangle
-  packets, are run in ticket order.  Every wait must already be satisfied, and
-  the solution must match plain substitution.
-* Two planted mistakes must be caught: a reversed task order and a lost map
-  advance.
+  packets and the 64-row block tasks of wide supernodes, are run in ticket
+  order.  Every wait must already be satisfied, and the solution must match
+  plain substitution.
+* Three planted mistakes must be caught: a reversed task order, a lost map
+  advance and two blocks of a wide supernode swapped.
 """
 import os
 import shutil
@@ -39,13 +40,14 @@ def _run(exe, *args):
 
 
 @pytest.mark.parametrize("kind,tail,seed", [(0, 0, 1), (0, 300, 2), (3000, 0, 3), (3000, 1000, 4),
-                                            (20000, 4000, 5)])
+                                            (20000, 4000, 5), (20000, 0, 6)])
 def test_plan_emulation(emulator, kind, tail, seed):
     r = _run(emulator, kind, tail, seed)
     assert r.returncode == 0 and r.stdout.startswith("ok"), r.stdout + r.stderr
 
 
-@pytest.mark.parametrize("plant,what", [(1, "fwd wait"), (2, "map cursor")])
-def test_plan_emulation_catches_planted_mistakes(emulator, plant, what):
-    r = _run(emulator, 3000, 1000, 4, plant)
+@pytest.mark.parametrize("plant,what,kind,tail", [(1, "fwd wait", 3000, 1000), (2, "map cursor", 3000, 1000),
+                                                  (3, "fwd block wait", 3000, 0)])
+def test_plan_emulation_catches_planted_mistakes(emulator, plant, what, kind, tail):
+    r = _run(emulator, kind, tail, 4, plant)
     assert r.returncode != 0 and what in r.stdout, r.stdout
