@@ -405,13 +405,13 @@ constexpr int kTri = kWave - 1;         // triangle steps held in registers (64-
 // tb-1): step q broadcasts x[j_{ta+q}] and subtracts L(j_t, j_{ta+q}) x from
 // every later lane t of the run.  The NT triangle columns are all loaded
 // before the first step (columns past the run load a valid duplicate and are
-// masked), so the chain waits on registers only.
+// masked), so the chain waits on registers only; a caller may load them
+// (load_triangle) before it waits for the right-hand side.
 template <int NT, bool kCoherent = true, typename T>
-__device__ __forceinline__ double run_triangle(const T *val, double xc, int lb, int ta, int tb)
+__device__ __forceinline__ void load_triangle(const T *val, int lb, int ta, int tb, T (&tv)[NT])
 {
     const int lane = lane_id();
     const int bw = tb - ta;
-    T tv[NT];
 #pragma unroll
     for (int q = 0; q < NT; q++) {
         const int qq = min(q, bw - 2);
@@ -420,6 +420,11 @@ __device__ __forceinline__ double run_triangle(const T *val, double xc, int lb, 
         const int off = min(max(lane - tp - 1, 0), bw - qq - 2);
         tv[q] = ld_val<kCoherent>(gp(val) + (uint32_t)base, (uint32_t)off);
     }
+}
+template <int NT, typename T>
+__device__ __forceinline__ double apply_triangle(const T (&tv)[NT], double xc, int ta, int tb)
+{
+    const int lane = lane_id();
 #pragma unroll
     for (int q = 0; q < NT; q++) {
         const int tp = min(ta + q, kWave - 1);
@@ -429,17 +434,23 @@ __device__ __forceinline__ double run_triangle(const T *val, double xc, int lb, 
     }
     return xc;
 }
+template <int NT, bool kCoherent = true, typename T>
+__device__ __forceinline__ double run_triangle(const T *val, double xc, int lb, int ta, int tb)
+{
+    T tv[NT];
+    load_triangle<NT, kCoherent>(val, lb, ta, tb, tv);
+    return apply_triangle<NT>(tv, xc, ta, tb);
+}
 // Upper triangular solve of a block of bw rows held one per lane (row t of
 // the block in lane t): column q' of the block's U is U(j_t, j_q') =
 // val[dpc_q' - (q' - t)] for t in [q' - ru_q', q'), zero above that run, and
 // rd_t = 1 / U(j_t, j_t).  Steps go from the last column to the first: lane
 // q' finalises z_q' = num_q' rd_q' and every earlier lane subtracts
-// U(j_t, j_q') z_q'.  Returns z_t.
+// U(j_t, j_q') z_q'.  apply_triangle_upper returns z_t.
 template <int NT, bool kCoherent = true, typename T>
-__device__ __forceinline__ double run_triangle_upper(const T *val, double num, double rd, int dpc, int ru, int bw)
+__device__ __forceinline__ void load_triangle_upper(const T *val, int dpc, int bw, T (&tv)[NT])
 {
     const int lane = lane_id();
-    T tv[NT];
 #pragma unroll
     for (int q = 0; q < NT; q++) {
         const int qq = min(q, bw - 1);
@@ -447,6 +458,11 @@ __device__ __forceinline__ double run_triangle_upper(const T *val, double num, d
         const int off = min(max(qq - lane, 0), qq);
         tv[q] = ld_val<kCoherent>(gp(val) + (uint32_t)(base - off), 0u);
     }
+}
+template <int NT, typename T>
+__device__ __forceinline__ double apply_triangle_upper(const T (&tv)[NT], double num, double rd, int ru, int bw)
+{
+    const int lane = lane_id();
 #pragma unroll
     for (int q = NT - 1; q >= 0; q--) {
         if (q < bw) {

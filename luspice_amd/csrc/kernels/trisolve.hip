@@ -42,21 +42,24 @@ constexpr int kSolveBatch = 32;         // dense-part values in flight per pass 
 // entries left of s are in yp; the earlier blocks of D, each solved by a task
 // of its own (ticketed before this one), are subtracted first (L(i, j) =
 // val[dp[j] + i - j]), 32 columns at a time, their L values loaded before
-// their y values are awaited; then the block's unit-lower triangle is solved
-// in registers.  A supernode of at most 64 rows is one block.
-template <typename T>
-__device__ void tri_forward(const TriDesc<T> &M, int s, int b, double *y, const double *yp, int *abort_word)
+// their y values are awaited; then the block's unit-lower triangle, loaded
+// before any wait (NT steps, NT = 0 for one row), is solved in registers.  A
+// supernode of at most 64 rows is one block.
+template <int NT, typename T>
+__device__ __forceinline__ void tri_forward_nt(const TriDesc<T> &M, int s, int b, int w, double *y,
+                                               const double *yp, int *abort_word)
 {
     const int lane = lane_id();
     const LSP_GLOBAL int *dp = gp(M.dp);
-    const int w = M.snend[s] - s;
     const int b0 = kWave * b;
     const int bw = min(kWave, w - b0);
     const int row = s + b0 + lane;
     const int rr = s + b0 + min(lane, bw - 1);
+    T tv[NT > 0 ? NT : 1];
+    if constexpr (NT > 0) load_triangle<NT, false>(M.val, dp[rr] + 1, 0, bw, tv);
     constexpr int SB = kSolveBatch;
     double acc = 0.0;
-    for (int q0 = 0; q0 < b0; q0 += kWave) {            // earlier blocks are full
+    for (int q0 = 0; q0 < b0; q0 += kWave) {            // earlier blocks are full; the last one is the latest
         double yq = 0.0;
         for (int h0 = 0; h0 < kWave; h0 += SB) {
             T v[SB];
@@ -73,36 +76,49 @@ __device__ void tri_forward(const TriDesc<T> &M, int s, int b, double *y, const 
     double p;
     if (!wave_wait_value(lane < bw ? &yp[row] : nullptr, p, abort_word)) return;
     p -= acc;
-    const int lb = dp[rr] + 1;
-    if (bw == 1) {
-    } else if (bw <= 9) p = run_triangle<8, false>(M.val, p, lb, 0, bw);
-    else if (bw <= 25) p = run_triangle<24, false>(M.val, p, lb, 0, bw);
-    else p = run_triangle<kTri, false>(M.val, p, lb, 0, bw);
+    if constexpr (NT > 0) p = apply_triangle<NT>(tv, p, 0, bw);
     if (lane < bw) st_coherent(&y[row], p);
+}
+template <typename T>
+__device__ void tri_forward(const TriDesc<T> &M, int s, int b, double *y, const double *yp, int *abort_word)
+{
+    const int w = M.snend[s] - s;
+    const int bw = min(kWave, w - kWave * b);
+    if (bw == 1) tri_forward_nt<0>(M, s, b, w, y, yp, abort_word);
+    else if (bw <= 9) tri_forward_nt<8>(M, s, b, w, y, yp, abort_word);
+    else if (bw <= 25) tri_forward_nt<24>(M, s, b, w, y, yp, abort_word);
+    else tri_forward_nt<kTri>(M, s, b, w, y, yp, abort_word);
 }
 
 // Backward triangle task: block b from the bottom of D = [s, e).  Its rows'
 // partial sums q_i = y_i - sum over U entries right of e are in zp; the
-// later blocks of D (solved by earlier tasks) are subtracted first (U(i, j) =
-// val[dp[j] - (j - i)] within column j's run ru[j], else 0), then the block's
-// upper triangle is solved and the solution written.
-template <typename T>
-__device__ void tri_backward(const TriDesc<T> &M, int s, int b, double *z, const double *zp, double *x,
-                             const double *cs, int *abort_word)
+// later blocks of D (solved by earlier tasks) are subtracted first, the
+// oldest (the bottom one) first and the one just below this block last
+// (U(i, j) = val[dp[j] - (j - i)] within column j's run ru[j], else 0); then
+// the block's upper triangle, loaded before any wait, is solved and the
+// solution written.
+template <int NT, typename T>
+__device__ __forceinline__ void tri_backward_nt(const TriDesc<T> &M, int s, int b, int w, double *z,
+                                                const double *zp, double *x, const double *cs, int *abort_word)
 {
     const int lane = lane_id();
     const LSP_GLOBAL int *dp = gp(M.dp);
     const LSP_GLOBAL int *ru = gp(M.ru);
-    const int w = M.snend[s] - s;
     const int b1 = w - kWave * b;                        // rows [s, s + b1) not yet solved
     const int bw = min(kWave, b1);
     const int base = s + b1 - bw;
     const int row = base + lane;
     const int rr = base + min(lane, bw - 1);
     const int c0 = base + bw;                            // first column of the later blocks
+    const int dpc = dp[rr];
+    T tv[NT];
+    load_triangle_upper<NT, false>(M.val, dpc, bw, tv);
+    const double rd = 1.0 / to_f64(M.diag[rr]);
+    const int run = ru[rr];
     constexpr int SB = kSolveBatch;
     double acc = 0.0;
-    for (int q0 = 0; q0 < kWave * b; q0 += kWave) {     // later blocks are full
+    for (int k = b - 1; k >= 0; k--) {                   // later blocks are full
+        const int q0 = kWave * k;
         double zq = 0.0;
         for (int h0 = 0; h0 < kWave; h0 += SB) {
             T v[SB];
@@ -121,21 +137,25 @@ __device__ void tri_backward(const TriDesc<T> &M, int s, int b, double *z, const
                 if (in[h]) acc = fma(to_f64(v[h]), bcast_f64(zq, h0 + h), acc);
         }
     }
-    const double rd = 1.0 / to_f64(M.diag[rr]);
-    const int dpc = dp[rr];
-    const int run = ru[rr];
     double num;
     if (!wave_wait_value(lane < bw ? &zp[row] : nullptr, num, abort_word)) return;
     num -= acc;
-    double zt;
-    if (bw <= 8) zt = run_triangle_upper<8, false>(M.val, num, rd, dpc, run, bw);
-    else if (bw <= 24) zt = run_triangle_upper<24, false>(M.val, num, rd, dpc, run, bw);
-    else zt = run_triangle_upper<kWave, false>(M.val, num, rd, dpc, run, bw);
+    const double zt = apply_triangle_upper<NT>(tv, num, rd, run, bw);
     if (lane < bw) {
         st_coherent(&z[row], zt);
         const int qo = M.q[row];
         x[qo] = cs ? zt * cs[qo] : zt;
     }
+}
+template <typename T>
+__device__ void tri_backward(const TriDesc<T> &M, int s, int b, double *z, const double *zp, double *x,
+                             const double *cs, int *abort_word)
+{
+    const int w = M.snend[s] - s;
+    const int bw = min(kWave, w - kWave * b);
+    if (bw <= 8) tri_backward_nt<8>(M, s, b, w, z, zp, x, cs, abort_word);
+    else if (bw <= 24) tri_backward_nt<24>(M, s, b, w, z, zp, x, cs, abort_word);
+    else tri_backward_nt<kWave>(M, s, b, w, z, zp, x, cs, abort_word);
 }
 
 // Packet task (code kTriPacket + o): up to 64 / g row tasks of one level,
