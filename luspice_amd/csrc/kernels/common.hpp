@@ -466,7 +466,7 @@ __device__ __forceinline__ double apply_triangle_upper(const T (&tv)[NT], double
 #pragma unroll
     for (int q = NT - 1; q >= 0; q--) {
         if (q < bw) {
-            const double zq = bcast_f64(num, q) * bcast_f64(rd, q);
+            const double zq = bcast_f64(num * rd, q);
             const int run = __builtin_amdgcn_readlane(ru, q);
             const bool in = lane < q && q - lane <= run;
             const double nn = fma(-to_f64(tv[q]), zq, num);

@@ -35,7 +35,11 @@ namespace lsp {
 
 constexpr int kGather = 4;              // 64-entry chunks of a row per wait round
 
-constexpr int kSolveBatch = 32;         // dense-part values in flight per pass of a triangle task
+// Values of an earlier (later) block's 64 columns loaded per pass of a
+// triangle task: all 64 for 4-byte and 2-byte factors, so that the latest
+// block's slab is in registers before its values are awaited; two passes of
+// 32 for fp64 (register budget).
+template <typename T> constexpr int kSolveBatch = sizeof(T) <= 4 ? 64 : 32;
 
 // Forward triangle task: block b (rows s + 64b .. s + 64b + bw - 1) of the
 // supernode D = [s, e).  Its rows' partial sums p_i = b_i - sum over L
@@ -57,7 +61,7 @@ __device__ __forceinline__ void tri_forward_nt(const TriDesc<T> &M, int s, int b
     const int rr = s + b0 + min(lane, bw - 1);
     T tv[NT > 0 ? NT : 1];
     if constexpr (NT > 0) load_triangle<NT, false>(M.val, dp[rr] + 1, 0, bw, tv);
-    constexpr int SB = kSolveBatch;
+    constexpr int SB = kSolveBatch<T>;
     double acc = 0.0;
     for (int q0 = 0; q0 < b0; q0 += kWave) {            // earlier blocks are full; the last one is the latest
         double yq = 0.0;
@@ -115,26 +119,25 @@ __device__ __forceinline__ void tri_backward_nt(const TriDesc<T> &M, int s, int 
     load_triangle_upper<NT, false>(M.val, dpc, bw, tv);
     const double rd = 1.0 / to_f64(M.diag[rr]);
     const int run = ru[rr];
-    constexpr int SB = kSolveBatch;
+    constexpr int SB = kSolveBatch<T>;
     double acc = 0.0;
     for (int k = b - 1; k >= 0; k--) {                   // later blocks are full
         const int q0 = kWave * k;
         double zq = 0.0;
         for (int h0 = 0; h0 < kWave; h0 += SB) {
             T v[SB];
-            bool in[SB];
+            int top[SB];                                 // column jq's run covers rows top .. jq - 1
 #pragma unroll
             for (int h = 0; h < SB; h++) {
                 const int jq = c0 + q0 + h0 + h;
-                const int d = jq - rr;                   // >= 1
                 const int rj = ru[jq];
-                in[h] = d <= rj;
-                v[h] = ld_plain(gp(M.val), (uint32_t)(dp[jq] - min(d, rj)));
+                top[h] = jq - rj;
+                v[h] = ld_plain(gp(M.val), (uint32_t)(dp[jq] - min(jq - rr, rj)));
             }
             if (h0 == 0 && !wave_wait_value(&z[c0 + q0 + lane], zq, abort_word)) return;
 #pragma unroll
             for (int h = 0; h < SB; h++)
-                if (in[h]) acc = fma(to_f64(v[h]), bcast_f64(zq, h0 + h), acc);
+                if (rr >= top[h]) acc = fma(to_f64(v[h]), bcast_f64(zq, h0 + h), acc);
         }
     }
     double num;
