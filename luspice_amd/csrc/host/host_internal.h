@@ -97,8 +97,8 @@ struct LspPlan {
     // dense diagonal block for all its rows.
     // Row tasks of one level with at most 32 kTriPacketPerLane entries are
     // packed: code kTriPacketBase + o names a packet at offset o of fpk /
-    // bpk, laid out as [g, count, row code x count]: g lanes per row (4, 8,
-    // 16 or 32, with g kTriPacketPerLane >= the row's entry count), up to
+    // bpk, laid out as [g, count, row code x count]: g lanes per row (2, 4,
+    // 8, 16 or 32, with g kTriPacketPerLane >= the row's entry count), up to
     // 64 / g rows.  Triangle tasks of supernodes of at most 16 rows are
     // packed as [-g, count, supernode start x count], g >= the width.
     // Supernodes wider than kTriBlockRows are solved by one task per 64-row

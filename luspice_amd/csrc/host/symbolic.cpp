@@ -424,8 +424,10 @@ static void plan_tri_supernodes(LspPlan &P)
         std::stable_sort(items.begin(), items.end(),
                          [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first < b.first; });
         // Pack the short row tasks of each key: rows with at most
-        // g kTriPacketPerLane entries (g = 4, 8, 16, 32 lanes) go 64 / g to a
-        // packet, so one ticket serves up to 16 rows.  Rows and triangles of
+        // g kTriPacketPerLane entries (g = 2, 4, 8, 16, 32 lanes) go 64 / g to
+        // a packet, so one ticket serves up to 32 rows.  (One lane per row,
+        // 64 rows to a packet, made c1's solves faster and a 512-circuit
+        // batch slower: a packet completes with its slowest row.)  Rows and triangles of
         // one key are independent, so any grouping of them keeps the order a
         // valid schedule; the blocks of a wide supernode stay in their order.
         auto entries = [&](int code) {
@@ -440,7 +442,7 @@ static void plan_tri_supernodes(LspPlan &P)
         std::vector<std::pair<int, int>> out;
         out.reserve(items.size());
         pk.clear();
-        std::vector<int> cls[4], tcls[3];
+        std::vector<int> cls[5], tcls[3];
         auto emit = [&](int key_v, std::vector<int> &v, int g, bool tri) {
             const int per = kWave64 / g;
             for (size_t x = 0; x < v.size(); x += per) {
@@ -469,7 +471,8 @@ static void plan_tri_supernodes(LspPlan &P)
                 if (code >= 0) {
                     const int ne = entries(code);
                     const int k = kTriPacketPerLane;
-                    const int c = ne <= 4 * k ? 0 : ne <= 8 * k ? 1 : ne <= 16 * k ? 2 : ne <= 32 * k ? 3 : -1;
+                    const int c = ne <= 2 * k ? 4 : ne <= 4 * k ? 0 : ne <= 8 * k ? 1 : ne <= 16 * k ? 2
+                                : ne <= 32 * k ? 3 : -1;
                     if (c >= 0) { cls[c].push_back(code); continue; }
                 } else {
                     const int s0 = -code - 1;
@@ -480,6 +483,7 @@ static void plan_tri_supernodes(LspPlan &P)
                 out.push_back(items[t]);
             }
             for (int c = 0; c < 4; c++) emit(items[a].first, cls[c], 4 << c, false);
+            emit(items[a].first, cls[4], 2, false);   // rows of at most 8 entries, 32 to a packet
             for (int c = 0; c < 3; c++) emit(items[a].first, tcls[c], 4 << c, true);
             a = b;
         }

@@ -240,7 +240,8 @@ int main(int argc, char **argv)
                 }
                 continue;
             }
-            if (!(g == 4 || g == 8 || g == 16 || g == 32) || cnt < 2 || cnt > 64 / g) return fail("row packet", o);
+            if (!(g == 2 || g == 4 || g == 8 || g == 16 || g == 32) || cnt < 2 || cnt > 64 / g)
+                return fail("row packet", o);
             std::vector<double> acc(cnt, 0.0);
             for (int r = 0; r < cnt; r++) {
                 const int rc = P.fpk[o + 2 + r], i = rc >> 1, part = rc & 1;
@@ -330,7 +331,8 @@ int main(int argc, char **argv)
                 }
                 continue;
             }
-            if (!(g == 4 || g == 8 || g == 16 || g == 32) || cnt < 2 || cnt > 64 / g) return fail("row packet", o);
+            if (!(g == 2 || g == 4 || g == 8 || g == 16 || g == 32) || cnt < 2 || cnt > 64 / g)
+                return fail("row packet", o);
             std::vector<double> acc(cnt, 0.0);
             for (int r = 0; r < cnt; r++) {
                 const int rc = P.bpk[o + 2 + r], i = rc >> 1, part = rc & 1;
