@@ -164,7 +164,7 @@ __device__ void tri_backward(const TriDesc<T> &M, int s, int b, double *z, const
 // Packet task (code kTriPacket + o): up to 64 / g row tasks of one level,
 // each with at most g kTriPacketLane entries (g = 2, 4, 8, 16 or 32); lanes
 // [r g, (r + 1) g) gather row r, lane l taking entries l, l + g, ...  The
-// rows' flags are polled together, each lane group sums with g-lane
+// rows' values are polled together, each lane group sums with g-lane
 // shuffles, and every row is written as its own task would be.
 template <typename T, bool kForward>
 __device__ void tri_packet(const TriLaunch &L, const TriDesc<T> &M, int o, long vo)
