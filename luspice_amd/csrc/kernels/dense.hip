@@ -32,8 +32,8 @@
 //   tail_fwd_gather_kernel  yp_T = b_T - L21 y  (one wave per row)
 //   tail_trsv_kernel<lower> y_T = L22^-1 yp_T
 //   tail_trsv_kernel<upper> z_T = U22^-1 y_T, x[q] = z
-// A trsv launch has one workgroup per 64-row block; blocks take tickets in
-// solve order and wait (bounded) only on blocks ticketed earlier,
+// A trsv launch has one workgroup per two 64-row blocks; workgroups take
+// tickets in solve order and wait (bounded) only on blocks ticketed earlier,
 // accumulating each finished block's contribution as it arrives.  Blocks
 // hand over their values as the sparse solves do: y and z are unset before
 // the solve and a waiting block polls the values themselves.
@@ -689,19 +689,26 @@ __global__ void __launch_bounds__(256) tail_fwd_gather_kernel(const void *mats, 
     }
 }
 
-// Dense triangular solve of the block rows (see the file comment).
+// Dense triangular solve of the block rows (see the file comment), two
+// 64-row blocks (128 rows, in solve order) per workgroup:
 //   lower: y_T <- L22^-1 yp_T (unit diagonal)
 //   upper: z_T <- U22^-1 y_T, x[q] = z (the sparse backward solve, a later
 //          launch, reads these z values)
-// Waves 1..3 accumulate the contributions of the blocks solved before this
-// one: per block the 64 x 64 factor slab (one row per lane, in registers) is
-// loaded before the block's values are awaited, so the wait overlaps the
-// loads; the polls return the solved values, broadcast by readlane.
-// Meanwhile wave 0 holds the diagonal block (and, upper, the reciprocals of
-// its diagonal), so after the last contribution only the 64-step solve and
-// the stores remain on the chain from block to block.
+// Waves 2..7 accumulate the contributions of the blocks solved before this
+// pair (waves 2, 4, 6 for the first block's rows, 3, 5, 7 for the second's,
+// three earlier blocks each in turn): per block the 64 x 64 factor slab (one
+// row per lane, in registers) is loaded before the block's values are
+// awaited, so the wait overlaps the loads; the polls return the solved
+// values, broadcast by readlane.  Waves 0 and 1 hold the two diagonal blocks,
+// right-hand sides and diagonal reciprocals.  After the first block is
+// solved, the six accumulator waves apply it to the second block's rows (a
+// slice of its columns each, from LDS), and wave 1 solves the second block:
+// one hand-off through memory per 128 rows instead of per 64.
+constexpr int kTailAcc = 6;                 // accumulator waves per workgroup
+constexpr int kTailSlice = (kNb + kTailAcc - 1) / kTailAcc;   // coupling columns per accumulator wave
+
 template <typename T, bool kLower>
-__global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const int *tails,
+__global__ void __launch_bounds__(512) tail_trsv_kernel(const void *mats, const int *tails,
                                                         const long *voff, const int *active,
                                                         double *y, const double *yp, double *z, double *x,
                                                         int *abort_word, const double *cs)
@@ -711,66 +718,90 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
     const int lane = lane_id();
     const int wv = threadIdx.x >> 6;
     __shared__ int s_ticket;
-    __shared__ double red[4][kWave];
+    __shared__ double red[kTailAcc][kWave];            // accumulators' sums over the earlier blocks
+    __shared__ double red2[kTailAcc][kWave];           // the first block's contribution to the second's rows
+    __shared__ double ys[kWave];                       // the first block's solution
     const bool live = !(active && !active[mi]);
     const int nblk = (M.tail_T + kNb - 1) / kNb;
+    const int npair = (nblk + 1) / 2;
     if (live) {
         if (threadIdx.x == 0) s_ticket = atomicAdd(&M.tail_ctr[0], 1);
         __syncthreads();
         const int t = uniform(s_ticket);
-        if (t < nblk) {
-            const int ib = kLower ? t : nblk - 1 - t;
-            const int r0 = ib * kNb;
-            const int bi = min(kNb, M.tail_T - r0);
+        if (t < npair) {
             const size_t ld = (size_t)M.tail_T;
             const long vo = voff[mi];
             const int k0 = M.tail_k0;
             const double *src = kLower ? y : z;
-            const int rr = r0 + min(lane, bi - 1);               // clamped row of this lane
+            const int nsub = min(2, nblk - 2 * t);      // blocks in this pair
+            // block in solve order so -> its first row
+            auto row0 = [&](int so) { return (kLower ? so : nblk - 1 - so) * kNb; };
+            // the rows of the pair's block u, clamped per lane
+            const int u = wv < 2 ? wv : (wv - 2) & 1;  // the block whose rows this wave handles
+            const bool has = u < nsub;
+            const int r0 = has ? row0(2 * t + u) : 0;
+            const int bi = has ? min(kNb, M.tail_T - r0) : 1;
+            const int rr = r0 + min(lane, bi - 1);
             double acc = 0.0;
             bool ok = true;
-            T dg[kNb];                                           // wave 0: row lane of the diagonal block
-            double rd = 1.0;                                     // wave 0, upper: 1 / U(rr, rr)
-            double yb = 0.0;                                     // wave 0: right-hand side of row rr
-            int qo = 0;                                          // wave 0, upper: output index of row rr
-            if (wv == 0) {
+            T dg[kNb];                                  // waves 0, 1: row lane of the diagonal block
+            T fc[kTailSlice];                           // waves 2..7: row of block 1 against a slice of block 0
+            double rd = 1.0;                            // waves 0, 1, upper: 1 / U(rr, rr)
+            double yb = 0.0;                            // waves 0, 1: right-hand side of row rr
+            int qo = 0;                                 // waves 0, 1, upper: output index of row rr
+            if (wv < 2) {
+                if (has) {
 #pragma unroll
-                for (int c = 0; c < kNb; c++) dg[c] = M.tail_F[(size_t)(r0 + min(c, bi - 1)) * ld + rr];
-                if (!kLower) {
-                    rd = 1.0 / to_f64(M.tail_F[(size_t)rr * ld + rr]);
-                    qo = M.q[k0 + rr];
+                    for (int c = 0; c < kNb; c++) dg[c] = M.tail_F[(size_t)(r0 + min(c, bi - 1)) * ld + rr];
+                    if (!kLower) {
+                        rd = 1.0 / to_f64(M.tail_F[(size_t)rr * ld + rr]);
+                        qo = M.q[k0 + rr];
+                    }
+                    yb = kLower ? yp[vo + k0 + rr] : y[vo + k0 + rr];
                 }
-                yb = kLower ? yp[vo + k0 + rr] : y[vo + k0 + rr];
             } else {
-                for (int s = wv - 1; s < t && ok; s += 3) {
-                    const int jb = kLower ? s : nblk - 1 - s;
-                    const int c0 = jb * kNb;
-                    const int bj = min(kNb, M.tail_T - c0);
-                    T fv[kNb];
+                if (nsub == 2) {
+                    // slice of the coupling block: rows of block 1, columns of block 0
+                    // (block 0 is partial only for the upper solve's first pair;
+                    // its missing columns meet zeros in ys)
+                    const int ra = row0(2 * t), rb = row0(2 * t + 1);
+                    const int ba = min(kNb, M.tail_T - ra);
+                    const int bb = min(kNb, M.tail_T - rb);
+                    const int rr1 = rb + min(lane, bb - 1);
+                    const int c0 = (wv - 2) * kTailSlice;
 #pragma unroll
-                    for (int c = 0; c < kNb; c++) fv[c] = M.tail_F[(size_t)(c0 + min(c, bj - 1)) * ld + rr];
-                    double sv;                                       // zero beyond bj
-                    ok = wave_wait_value(lane < bj ? &src[vo + k0 + c0 + lane] : nullptr, sv, abort_word);
-                    if (!ok) break;
+                    for (int c = 0; c < kTailSlice; c++)
+                        fc[c] = M.tail_F[(size_t)(ra + min(c0 + c, ba - 1)) * ld + rr1];
+                }
+                if (has) {
+                    for (int s = (wv - 2) >> 1; s < 2 * t && ok; s += 3) {
+                        const int c0 = row0(s);
+                        const int bj = min(kNb, M.tail_T - c0);
+                        T fv[kNb];
 #pragma unroll
-                    for (int c = 0; c < kNb; c++) {
-                        const double yc = bcast(sv, c);              // zero beyond bj
-                        acc += to_f64(fv[c]) * yc;
+                        for (int c = 0; c < kNb; c++) fv[c] = M.tail_F[(size_t)(c0 + min(c, bj - 1)) * ld + rr];
+                        double sv;                                   // zero beyond bj
+                        ok = wave_wait_value(lane < bj ? &src[vo + k0 + c0 + lane] : nullptr, sv, abort_word);
+                        if (!ok) break;
+#pragma unroll
+                        for (int c = 0; c < kNb; c++) {
+                            const double yc = bcast(sv, c);          // zero beyond bj
+                            acc += to_f64(fv[c]) * yc;
+                        }
                     }
                 }
+                red[wv - 2][lane] = has && lane < bi ? acc : 0.0;
             }
-            red[wv][lane] = lane < bi ? acc : 0.0;
             const bool all_ok = __syncthreads_and(ok);
-            if (all_ok && wv == 0) {
-                double tr = 0.0;
-                if (lane < bi) tr = yb - (red[1][lane] + red[2][lane] + red[3][lane]);
+            // triangle of one block on its solver wave: tr = right-hand side
+            // minus the contributions gathered, solved in place
+            auto solve = [&](double tr) {
                 if (kLower) {
 #pragma unroll
                     for (int c = 0; c < kNb; c++) {
                         const double yc = bcast(tr, c);
                         if (lane > c && c < bi) tr -= to_f64(dg[c]) * yc;
                     }
-                    if (lane < bi) st_coherent(&y[vo + k0 + r0 + lane], tr);
                 } else {
 #pragma unroll
                     for (int c = kNb - 1; c >= 0; c--) {
@@ -780,10 +811,42 @@ __global__ void __launch_bounds__(256) tail_trsv_kernel(const void *mats, const 
                             if (lane < c) tr -= to_f64(dg[c]) * zc;
                         }
                     }
-                    if (lane < bi) {
+                }
+                if (lane < bi) {
+                    if (kLower) {
+                        st_coherent(&y[vo + k0 + rr], tr);
+                    } else {
                         st_coherent(&z[vo + k0 + rr], tr);
                         x[vo + qo] = cs ? tr * cs[vo + qo] : tr;
                     }
+                }
+                return tr;
+            };
+            if (all_ok && wv == 0) {
+                double tr = 0.0;
+                if (lane < bi) tr = yb - (red[0][lane] + red[2][lane] + red[4][lane]);
+                tr = solve(tr);
+                ys[lane] = lane < bi ? tr : 0.0;
+            }
+            if (nsub == 2) {
+                __syncthreads();                        // ys written
+                if (all_ok && wv >= 2) {
+                    const int c0 = (wv - 2) * kTailSlice;
+                    double a2 = 0.0;
+#pragma unroll
+                    for (int c = 0; c < kTailSlice; c++)
+                        if (c0 + c < kNb) a2 += to_f64(fc[c]) * ys[c0 + c];
+                    red2[wv - 2][lane] = a2;
+                }
+                __syncthreads();                        // red2 written
+                if (all_ok && wv == 1) {
+                    double tr = 0.0;
+                    if (lane < bi) {
+                        tr = yb - (red[1][lane] + red[3][lane] + red[5][lane]);
+#pragma unroll
+                        for (int w = 0; w < kTailAcc; w++) tr -= red2[w][lane];
+                    }
+                    solve(tr);
                 }
             }
         }
@@ -851,13 +914,14 @@ hipError_t launch_tail_solve(const void *mats, const int *tails, int ntails, int
                              hipStream_t st)
 {
     const int nblk = (maxT + kNb - 1) / kNb;
+    const int npair = (nblk + 1) / 2;                   // two 64-row blocks per workgroup
     if (forward) {
         hipLaunchKernelGGL(tail_fwd_gather_kernel<T>, dim3((maxT + 3) / 4, ntails), dim3(256), 0, st, mats,
                            tails, voff, active, b, y, yp, rs);
-        hipLaunchKernelGGL((tail_trsv_kernel<T, true>), dim3(nblk, ntails), dim3(256), 0, st, mats, tails,
+        hipLaunchKernelGGL((tail_trsv_kernel<T, true>), dim3(npair, ntails), dim3(512), 0, st, mats, tails,
                            voff, active, y, yp, z, x, abort_word, cs);
     } else {
-        hipLaunchKernelGGL((tail_trsv_kernel<T, false>), dim3(nblk, ntails), dim3(256), 0, st, mats, tails,
+        hipLaunchKernelGGL((tail_trsv_kernel<T, false>), dim3(npair, ntails), dim3(512), 0, st, mats, tails,
                            voff, active, y, yp, z, x, abort_word, cs);
     }
     return hipGetLastError();
