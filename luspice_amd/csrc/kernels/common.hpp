@@ -85,6 +85,42 @@ __device__ __forceinline__ int wave_ticket(int *ctr)
     return __builtin_amdgcn_readlane(v, leader);
 }
 
+// Task queues of the persistent kernels.  One counter serving every wave of
+// a wide grid bounds the task rate by its atomics (about 85 tasks/us on
+// MI355X), so a queue may be split over nq = 1, 2 or 4 counters: task t
+// belongs to counter t % nq, wave w of each workgroup takes its tickets from
+// counter w % nq, and ticket k of counter q is task k * nq + q.  Every
+// counter has waves in every workgroup, and each hands out its tasks in
+// order, so the lowest unfinished task is always held by a working wave (its
+// counter's waves finish that counter's lower tasks first) and its waits,
+// on lower tasks only, end.  Block layout (ints): [0] counter 0, [1] waves
+// finished, [32 q] counter q (q >= 1, a cache line each); kQueueInts in all.
+
+__device__ __forceinline__ int queue_count(int nq) { return nq == 2 || nq == 4 ? nq : 1; }
+
+__device__ __forceinline__ int next_task(int *ctr, int nq)
+{
+    // wave-uniform: with a per-lane address every lane would send its atomic
+    const int q = uniform((int)(threadIdx.x >> 6)) % nq;
+    return wave_ticket(ctr + 32 * q) * nq + q;
+}
+
+// Called once per wave on exit: the last wave out re-arms the counters and
+// returns true.
+__device__ __forceinline__ bool leave_queue(int *ctr, int nq, int total_waves)
+{
+    bool last = false;
+    if (lane_id() == 0) {
+        const int fin = atomicAdd(&ctr[1], 1);
+        if (fin == total_waves - 1) {
+            for (int q = 0; q < nq; q++) st_relaxed(&ctr[32 * q], 0);
+            st_relaxed(&ctr[1], 0);
+            last = true;
+        }
+    }
+    return last;
+}
+
 // Epochs live in device counters so a launch needs no host-chosen argument and
 // a captured graph can be replayed: every wave reads the counter when it
 // starts, and the last wave out (after all others have read it) advances it.

@@ -16,6 +16,10 @@ enum : int {
     kErrCapacity = 2,    // a column/row exceeded its scratch capacity
 };
 
+// Ints in one persistent kernel's queue block: up to 4 queue counters, a
+// cache line each, and the count of waves finished.
+constexpr int kQueueInts = 128;
+
 // Status words per matrix written by the refactor (FactorDesc::mstat).
 constexpr int kMstat = 8;
 

@@ -383,8 +383,9 @@ __global__ void __launch_bounds__(256) refactor_kernel(FactorLaunch L)
     const FactorDesc<T> *mats = (const FactorDesc<T> *)L.mats;
     const int epoch = read_epoch(L.epoch);
 
+    const int nq = queue_count(L.nq);
     for (;;) {
-        const int t = wave_ticket(&L.ctr[0]);
+        const int t = next_task(L.ctr, nq);
         if (t >= L.ntask) break;
         if (aborted(L.abort_word)) continue;                  // drain
         const int2 tk = L.task[t];
@@ -411,14 +412,7 @@ __global__ void __launch_bounds__(256) refactor_kernel(FactorLaunch L)
         }
     }
     // the last wave out re-arms the queue and advances the epoch
-    if (lane_id() == 0) {
-        const int fin = atomicAdd(&L.ctr[1], 1);
-        if (fin == L.total_waves - 1) {
-            st_relaxed(&L.ctr[0], 0);
-            st_relaxed(&L.ctr[1], 0);
-            advance_epoch(L.epoch, epoch);
-        }
-    }
+    if (leave_queue(L.ctr, nq, L.total_waves)) advance_epoch(L.epoch, epoch);
 }
 
 // Per-matrix status words are reset by a kernel (not a memset) so the whole

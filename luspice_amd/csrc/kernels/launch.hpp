@@ -15,7 +15,7 @@ struct FactorLaunch {
     const int *active;      // per-matrix mask (nullptr = all)
     int ntask;
     int *epoch;             // device epoch counter: read at start, bumped by the last wave
-    int *ctr;               // [0] queue head, [1] waves finished
+    int *ctr;               // kQueueInts: queue counters, waves finished
     int *abort_word;
     double *scratch;        // per-wave global fallback
     long scratch_stride;    // doubles per wave slice
@@ -28,6 +28,7 @@ struct FactorLaunch {
                             // published (end for a dense-block column), end of the
                             // last dependency wait, column
     int poll_backoff;       // dependency polls back off to at most poll_backoff + 1 sleeps
+    int nq;                 // queue counters (1, 2 or 4) in the ctr block
 };
 
 struct TriLaunch {
@@ -43,10 +44,10 @@ struct TriLaunch {
     const int *n_active;    // optional: when *n_active == 0 no circuit is active, tasks are skipped
     long long *timeline;    // optional diagnostic, 2 per task: ticket granted, task finished
     int ntask;
-    int *ctr;
+    int *ctr;               // kQueueInts: queue counters, waves finished
     int *abort_word;
     int total_waves;
-    int pad_;
+    int nq;                 // queue counters (1, 2 or 4) in the ctr block
 };
 
 struct KrylovArgs {

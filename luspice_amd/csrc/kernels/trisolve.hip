@@ -397,8 +397,9 @@ __device__ __forceinline__ void trisolve_body(const TriLaunch &L)
     // Every wave of the grid reads the same count (written by an earlier
     // launch) and returns at once: no ticket and no finish count.
     if (L.n_active && uniform(ld_relaxed(L.n_active)) == 0) return;
+    const int nq = queue_count(L.nq);
     for (;;) {
-        const int t = wave_ticket(&L.ctr[0]);
+        const int t = next_task(L.ctr, nq);
         if (t >= L.ntask) break;
         if (aborted(L.abort_word)) continue;
         const long long tg = L.timeline ? (long long)wall_clock64() : 0;
@@ -409,13 +410,7 @@ __device__ __forceinline__ void trisolve_body(const TriLaunch &L)
             L.timeline[2L * t + 1] = (long long)wall_clock64();
         }
     }
-    if (lane_id() == 0) {
-        const int fin = atomicAdd(&L.ctr[1], 1);
-        if (fin == L.total_waves - 1) {               // the last wave out re-arms the queue
-            st_relaxed(&L.ctr[0], 0);
-            st_relaxed(&L.ctr[1], 0);
-        }
-    }
+    leave_queue(L.ctr, nq, L.total_waves);             // the last wave out re-arms the queue
 }
 
 template <typename T>

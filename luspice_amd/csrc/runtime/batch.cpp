@@ -602,7 +602,8 @@ int Batch::upload_all()
         std::vector<int> zeros(ntot_, 0);
         d_done_ = upload(zeros);
         std::vector<int> z6(8, 0);
-        d_ctr_ = upload(z6); d_abort_ = upload(z6);
+        std::vector<int> zq(3 * kQueueInts, 0);    // refactor, forward, backward queue blocks
+        d_ctr_ = upload(zq); d_abort_ = upload(z6);
         std::vector<int> e1(4, 1);                 // epochs start at 1, flags at 0
         d_epoch_ = upload(e1);
         std::vector<int> ms(kMstat * nc, 0);
@@ -814,6 +815,14 @@ int Batch::upload_all()
         tblocks_b_ = blocks_for(width_tb_, cu_count_);
         if (opt_.refactor_blocks > 0) fblocks_ = opt_.refactor_blocks;
         if (opt_.trisolve_blocks > 0) tblocks_f_ = tblocks_b_ = opt_.trisolve_blocks;
+        // Wide grids split each task queue over 4 counters: one counter's
+        // atomics held the task rate to about 85 tasks/us (c1 refactor 3.54
+        // -> 3.11 ms, solve pair 0.856 -> 0.728 ms; 512 circuits 29.2k ->
+        // 46.6k fp32 and 39.1k -> 56.8k fp64 circuit steps/s; 2 counters
+        // gave about two thirds of that).  Small grids keep one counter,
+        // shared by all their waves.
+        nq_f_ = fblocks_ >= 16 ? 4 : 1;
+        nq_t_ = std::min(tblocks_f_, tblocks_b_) >= 16 ? 4 : 1;
         int maxcol = 0;
         for (auto &C : circ_) maxcol = std::max(maxcol, C.plan.max_col);
         scratch_stride_ = maxcol > cap ? maxcol : 1;
@@ -967,6 +976,7 @@ int Batch::refactor_t(FactorSet<T> &fs, const int *d_active)
     L.scratch_stride = scratch_stride_;
     L.cap = opt_.lds_cap;
     L.total_waves = fblocks_ * 4;
+    L.nq = nq_f_;
     L.poll_backoff = opt_.poll_backoff > 0 ? opt_.poll_backoff : 7;
     L.ptol = opt_.refactor_ptol;
     L.stol = opt_.repivot == 3 ? opt_.pivot_tol : 0.0;
@@ -1229,9 +1239,10 @@ int Batch::trisolve_t(FactorSet<T> &fs, const double *d_b, double *d_x, const in
     Lf.yp = d_typ_;
     Lf.zp = d_tzp_;
     Lf.ntask = (int)nfw_tot_;
-    Lf.ctr = d_ctr_ + 2;
+    Lf.ctr = d_ctr_ + kQueueInts;
     Lf.abort_word = d_abort_ + 1;
     Lf.total_waves = tblocks_f_ * 4;
+    Lf.nq = nq_t_;
     const bool scaled = std::is_same<T, __half>::value;
     Lf.rs = scaled ? d_rscale_ : nullptr;
     Lf.cs = scaled ? d_cscale_ : nullptr;
@@ -1245,8 +1256,9 @@ int Batch::trisolve_t(FactorSet<T> &fs, const double *d_b, double *d_x, const in
     Lb.timeline = Lf.timeline ? Lf.timeline + 2 * nfw_tot_ : nullptr;
     Lb.task = d_bwtask_;
     Lb.ntask = (int)nbw_tot_;
-    Lb.ctr = d_ctr_ + 4;
+    Lb.ctr = d_ctr_ + 2 * kQueueInts;
     Lb.total_waves = tblocks_b_ * 4;
+    Lb.nq = nq_t_;
     if (!capturing_) (void)hipEventRecord(ev_[2], stream_);
     int rc = check(launch_solve_unset(Lf, ntot_, stream_), "solve unset");
     if (!rc) rc = check(launch_trisolve_half<T>(Lf, true, tblocks_f_, stream_), "trisolve fwd");

@@ -101,6 +101,7 @@ struct DeviceArrays {
     double *d_scratch_ = nullptr;
     long scratch_stride_ = 0;
     int fblocks_ = 0, tblocks_f_ = 0, tblocks_b_ = 0;
+    int nq_f_ = 1, nq_t_ = 1;                          // queue counters: refactor, solves
     long width_f_ = 0, width_tf_ = 0, width_tb_ = 0;   // widest merged level
     // dense trailing blocks
     int *d_tail_ds_ = nullptr, *d_tail_lcut_ = nullptr, *d_tail_ctr_ = nullptr, *d_tails_ = nullptr;
