@@ -754,7 +754,8 @@ __global__ void __launch_bounds__(512) tail_trsv_kernel(const void *mats, const 
 #pragma unroll
                     for (int c = 0; c < kNb; c++) dg[c] = M.tail_F[(size_t)(r0 + min(c, bi - 1)) * ld + rr];
                     if (!kLower) {
-                        rd = 1.0 / to_f64(M.tail_F[(size_t)rr * ld + rr]);
+                        // zero past the block's end: those lanes then pass zeros
+                        rd = lane < bi ? 1.0 / to_f64(M.tail_F[(size_t)rr * ld + rr]) : 0.0;
                         qo = M.q[k0 + rr];
                     }
                     yb = kLower ? yp[vo + k0 + rr] : y[vo + k0 + rr];
@@ -803,13 +804,13 @@ __global__ void __launch_bounds__(512) tail_trsv_kernel(const void *mats, const 
                         if (lane > c && c < bi) tr -= to_f64(dg[c]) * yc;
                     }
                 } else {
+                    // z_c = tr_c / U(c, c), one broadcast per step; rows past
+                    // bi hold tr = 0 and rd = 0 and broadcast zeros
 #pragma unroll
                     for (int c = kNb - 1; c >= 0; c--) {
-                        if (c < bi) {
-                            const double zc = bcast(tr, c) * bcast(rd, c);
-                            if (lane == c) tr = zc;
-                            if (lane < c) tr -= to_f64(dg[c]) * zc;
-                        }
+                        const double zc = bcast(tr * rd, c);
+                        if (lane == c) tr = zc;
+                        if (lane < c) tr -= to_f64(dg[c]) * zc;
                     }
                 }
                 if (lane < bi) {
