@@ -74,16 +74,29 @@ __device__ __forceinline__ void st_relaxed(int *p, int v)
 // scalar and the wave moves as one.
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Takes the next ticket from a queue counter: every lane executes the atomic,
-// the first active lane adds 1 and the others 0, and its result is the
-// ticket.  No lane-dependent branch surrounds the atomic.
-__device__ __forceinline__ int wave_ticket(int *ctr)
+// Takes the next ticket from a queue counter (the first active lane's atomic).
+// ticket_issue sends the atomic and ticket_take reads its result, so a wave
+// can request its next ticket while it finishes work that waits on no other
+// wave (leader < 0: no request).
+struct TicketReq {
+    int v;          // per-lane result of the atomic
+    int leader;     // the lane that added 1 (wave-uniform)
+};
+// The leader alone sends the atomic.  The branch holds nothing but the
+// atomic and the result is read with readlane, which needs no other lane, so
+// no lane can run ahead of the others.  (Files built with the compiler's
+// atomic optimizer on turn the atomic into the same single-lane form and
+// broadcast its result at once; factor.hip is built with it off, so that
+// the result is awaited only where it is read.)
+__device__ __forceinline__ TicketReq ticket_issue(int *ctr)
 {
     const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
-    const int v = __hip_atomic_fetch_add(gp(ctr), lane_id() == leader ? 1 : 0, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-    return __builtin_amdgcn_readlane(v, leader);
+    int v = 0;
+    if (lane_id() == leader) v = __hip_atomic_fetch_add(gp(ctr), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return {v, leader};
 }
+__device__ __forceinline__ int ticket_take(const TicketReq &r) { return __builtin_amdgcn_readlane(r.v, r.leader); }
+__device__ __forceinline__ int wave_ticket(int *ctr) { return ticket_take(ticket_issue(ctr)); }
 
 // Task queues of the persistent kernels.  One counter serving every wave of
 // a wide grid bounds the task rate by its atomics (about 85 tasks/us on
@@ -98,10 +111,13 @@ __device__ __forceinline__ int wave_ticket(int *ctr)
 
 __device__ __forceinline__ int queue_count(int nq) { return nq == 2 || nq == 4 ? nq : 1; }
 
+// This wave's counter; wave-uniform: with a per-lane address every lane
+// would send its atomic.
+__device__ __forceinline__ int queue_index(int nq) { return uniform((int)(threadIdx.x >> 6)) % nq; }
+
 __device__ __forceinline__ int next_task(int *ctr, int nq)
 {
-    // wave-uniform: with a per-lane address every lane would send its atomic
-    const int q = uniform((int)(threadIdx.x >> 6)) % nq;
+    const int q = queue_index(nq);
     return wave_ticket(ctr + 32 * q) * nq + q;
 }
 
