@@ -331,10 +331,11 @@ __device__ __forceinline__ int wave_wait_prefix(const int *flag, int epoch, int 
     }
 }
 
-// The solves back off only to two sleeps (~0.2 us) between polls: their
-// waits are short and their chains long (c1 pair 1.346 -> 1.313 ms against
-// the factorization's ~0.9 us cap, no change on a 512-circuit batch).
-constexpr int kSolveBackoff = 1;
+// The solves poll with one sleep (~0.1 us) between polls: their waits are
+// short and their chains long (c1 pair 0.640 -> 0.632 ms against two sleeps,
+// 0.662 ms at four; no change on a 512-circuit batch; the factorization's
+// cap is ~0.9 us).
+constexpr int kSolveBackoff = 0;
 
 // Wave-collective wait on values instead of flags: every lane loads the
 // values its non-null pointers name (agent-coherent) until none of them holds

@@ -805,13 +805,15 @@ __global__ void __launch_bounds__(512) tail_trsv_kernel(const void *mats, const 
                     }
                 } else {
                     // z_c = tr_c / U(c, c), one broadcast per step; rows past
-                    // bi hold tr = 0 and rd = 0 and broadcast zeros
+                    // bi hold tr = 0 and rd = 0 and broadcast zeros.  Row c's
+                    // tr is final from step c on, so every row's z is the
+                    // same product tr * rd, formed once at the end.
 #pragma unroll
                     for (int c = kNb - 1; c >= 0; c--) {
                         const double zc = bcast(tr * rd, c);
-                        if (lane == c) tr = zc;
                         if (lane < c) tr -= to_f64(dg[c]) * zc;
                     }
+                    tr = tr * rd;
                 }
                 if (lane < bi) {
                     if (kLower) {
